@@ -1,0 +1,106 @@
+"""Hamiltonian generators and the FCIDUMP reader.
+
+Integrals are returned in the physicist convention V[i, j, k, l] = <ij|kl> = (ik|jl)
+with orbital irreps in CheMPS2's XOR labelling (Irreps.h), the layout that
+``Hamiltonian::setVmat`` receives (Hamiltonian.cpp:296-423 for the FCIDUMP reader).
+"""
+import numpy as np
+
+def _molpro_from_xor(group):
+    """XOR label -> Molpro irrep number, per point group (Irreps.cpp psi2molpro tables)."""
+    return {
+        0: [1],
+        1: [1, 2], 2: [1, 2], 3: [1, 2],
+        4: [1, 4, 3, 2],
+        5: [1, 4, 2, 3], 6: [1, 4, 2, 3],
+        7: [1, 4, 6, 7, 8, 5, 3, 2],
+    }[group]
+
+
+def molpro_to_xor(group, molpro_irrep):
+    """Map a Molpro irrep number (1-based) to CheMPS2's XOR label."""
+    return _molpro_from_xor(group).index(molpro_irrep)
+
+
+def hubbard(L, t=1.0, U=4.0, periodic=False):
+    """1D Hubbard chain: T[i,i+1] = -t, <ii|ii> = U."""
+    T = np.zeros((L, L))
+    for i in range(L - 1):
+        T[i, i + 1] = T[i + 1, i] = -t
+    if periodic and L > 2:
+        T[0, L - 1] = T[L - 1, 0] = -t
+    V = np.zeros((L, L, L, L))
+    for i in range(L):
+        V[i, i, i, i] = U
+    return T, V
+
+
+def random_abinitio(L, seed, group=7):
+    """Seeded synthetic ab-initio Hamiltonian with D2h (or smaller) symmetry.
+
+    Orbital irreps uniform in {0..nirrep-1}; h_ij ~ N(0, (0.1 exp(-|i-j|/4))^2) for
+    equal irreps, h_ii ~ U[-2, 0]; (ij|kl) = sum_P B^P_ij B^P_kl over 3L symmetric
+    factors with B^P_ij ~ N(0, (0.05 exp(-|i-j|/6))^2) masked to irrep_i ^ irrep_j =
+    Gamma_P. This gives a positive semidefinite ERI tensor with 8-fold symmetry and
+    the selection rule irrep_i ^ irrep_j = irrep_k ^ irrep_l.
+    Returns (irreps, T, V).
+    """
+    nirrep = {0: 1, 1: 2, 2: 2, 3: 2, 4: 4, 5: 4, 6: 4, 7: 8}[group]
+    rng = np.random.default_rng(seed)
+    irreps = rng.integers(0, nirrep, size=L).astype(np.int32)
+    idx = np.arange(L)
+    dist = np.abs(idx[:, None] - idx[None, :])
+    same = irreps[:, None] == irreps[None, :]
+    h = rng.normal(size=(L, L)) * 0.1 * np.exp(-dist / 4.0)
+    h = np.triu(h, 1)
+    h = h + h.T
+    h[~same] = 0.0
+    h[idx, idx] = rng.uniform(-2.0, 0.0, size=L)
+    eri = np.zeros((L, L, L, L))
+    xor = irreps[:, None] ^ irreps[None, :]
+    for _ in range(3 * L):
+        g = rng.integers(0, nirrep)
+        B = rng.normal(size=(L, L)) * 0.05 * np.exp(-dist / 6.0)
+        B = np.triu(B) + np.triu(B, 1).T
+        B[xor != g] = 0.0
+        eri += np.einsum("ij,kl->ijkl", B, B)
+    V = np.ascontiguousarray(eri.transpose(0, 2, 1, 3))  # (ij|kl) -> <ik|jl>
+    return irreps, h, V
+
+
+def read_fcidump(path):
+    """Reads an FCIDUMP file (Hamiltonian.cpp:296-423). Returns a dict with L, N, TwoS,
+    irreps (XOR labels, D2h subgroup given by ``group``), T, V (physicist), econst."""
+    with open(path) as f:
+        text = f.read()
+    head, _, body = text.partition("&END")
+    if not body:
+        head, _, body = text.partition("/")
+    hd = head.replace("\n", " ").replace(",", " ")
+    import re
+    L = int(re.search(r"NORB\s*=\s*(\d+)", hd).group(1))
+    N = int(re.search(r"NELEC\s*=\s*(\d+)", hd).group(1))
+    ms2 = re.search(r"MS2\s*=\s*(\d+)", hd)
+    TwoS = int(ms2.group(1)) if ms2 else 0
+    orbsym = re.search(r"ORBSYM\s*=\s*([\d\s]+)", hd)
+    osym = [int(x) for x in orbsym.group(1).split()][:L] if orbsym else [1] * L
+    T = np.zeros((L, L))
+    V = np.zeros((L, L, L, L))
+    econst = 0.0
+    for line in body.strip().splitlines():
+        parts = line.split()
+        if len(parts) != 5:
+            continue
+        val = float(parts[0].replace("D", "E").replace("d", "e"))
+        i, j, k, l = (int(x) for x in parts[1:])
+        if i == j == k == l == 0:
+            econst = val
+        elif k == 0 and l == 0:
+            T[i - 1, j - 1] = T[j - 1, i - 1] = val
+        else:
+            # chemist (ij|kl) -> physicist <ik|jl>, all 8 permutations
+            a, b, c, d = i - 1, k - 1, j - 1, l - 1  # <ab|cd> = (ac|bd)
+            for (p, q, r, s) in ((a, b, c, d), (b, a, d, c), (c, b, a, d), (a, d, c, b),
+                                 (c, d, a, b), (d, c, b, a), (b, c, d, a), (d, a, b, c)):
+                V[p, q, r, s] = val
+    return {"L": L, "N": N, "TwoS": TwoS, "molpro_irreps": osym, "T": T, "V": V, "econst": econst}

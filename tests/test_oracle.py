@@ -1,0 +1,178 @@
+"""Pins of the CPU oracle against mathematics and the reference's own known answers.
+
+Each check is chosen so that a plausible transcription mistake (dropped diagram,
+wrong phase, wrong index or transposed operand) makes it fail:
+* DMRG == FCI (independent determinant code, tests/fci.py) for generic integrals with
+  every irrep and several spins: every Heff diagram contributes there.
+* The reference's known answers: tests/test5.cpp.in:83 (N2 STO-3G) and the
+  DMRG == FCI checks of tests/test4.cpp.in:103 and tests/test3.cpp.in:115.
+* Hermiticity of Heff and the diagonal against e_k^T H e_k.
+* Wigner 6j / 9j against sympy's exact values; Davidson against numpy eigh.
+"""
+import numpy as np
+import pytest
+from fractions import Fraction
+
+from chemps2_amd.problems import hubbard, molpro_to_xor, random_abinitio, read_fcidump
+from fci import fci_energy
+
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+def test_wigner_vs_sympy(oracle_lib):
+    from sympy.physics.wigner import wigner_6j, wigner_9j
+
+    def exact(f, a):
+        try:
+            return float(f(*[Fraction(int(x), 2) for x in a]))
+        except ValueError:  # sympy rejects half-integer sums that break a triangle
+            return 0.0
+
+    import itertools
+
+    def tri(x, y, z):
+        return (x + y + z) % 2 == 0 and abs(x - y) <= z <= x + y
+
+    six = [a for a in itertools.product(range(5), repeat=6)
+           if tri(a[0], a[1], a[2]) and tri(a[3], a[4], a[2]) and tri(a[0], a[4], a[5]) and tri(a[3], a[1], a[5])]
+    assert len(six) > 200
+    for a in six[::3]:
+        assert abs(oracle_lib.wigner6j(*a) - exact(wigner_6j, a)) < 1e-12, a
+    rng = np.random.default_rng(0)
+    nine = []
+    while len(nine) < 120:
+        a = rng.integers(0, 5, size=9)
+        if all(tri(*t) for t in ((a[0], a[1], a[2]), (a[3], a[4], a[5]), (a[6], a[7], a[8]),
+                                 (a[0], a[3], a[6]), (a[1], a[4], a[7]), (a[2], a[5], a[8]))):
+            nine.append(a)
+    for a in nine:
+        assert abs(oracle_lib.wigner9j(*a) - exact(wigner_9j, a)) < 1e-12, a
+
+
+def test_fci_closed_form_u0():
+    L, N = 8, 6
+    T, V = hubbard(L, U=0.0)
+    exact = 2 * sum(-2 * np.cos(k * np.pi / (L + 1)) for k in range(1, N // 2 + 1))
+    assert abs(fci_energy(L, [0] * L, N, 0, 0, T, V) - exact) < 1e-10
+
+
+def test_davidson_dense(oracle_lib):
+    rng = np.random.default_rng(3)
+    n = 300
+    A = rng.normal(size=(n, n))
+    A = A + A.T + np.diag(np.arange(n) * 0.5)
+    e, x, nm = oracle_lib.davidson_dense(A, rng.normal(size=n), rtol=1e-9)
+    w = np.linalg.eigvalsh(A)
+    assert abs(e - w[0]) < 1e-9
+    assert np.linalg.norm(A @ x - e * x) < 1e-8
+    assert nm > 32  # went through at least one deflation
+
+
+@pytest.mark.parametrize("L,N,TwoS,seed", [(6, 6, 0, 1), (7, 7, 1, 2), (8, 6, 2, 3), (6, 7, 3, 4)])
+def test_dmrg_equals_fci_abinitio_d2h(oracle_lib, L, N, TwoS, seed):
+    irr, T, V = random_abinitio(L, seed, group=7)
+    for target in range(8):
+        try:
+            ef = fci_energy(L, irr, N, TwoS, target, T, V)
+            break
+        except ValueError:
+            continue
+    o = oracle_lib.OracleDMRG(L, 7, irr, N, TwoS, target, T, V)
+    o.init(200, 5)
+    e = o.solve([(200, 1e-11, 10, 0.0, 1e-10)])
+    assert abs(e - ef) < 1e-9
+
+
+def test_dmrg_all_irreps_c2v(oracle_lib):
+    irr, T, V = random_abinitio(5, 9, group=5)
+    for target in range(4):
+        ef = fci_energy(5, irr, 4, 0, target, T, V)
+        o = oracle_lib.OracleDMRG(5, 5, irr, 4, 0, target, T, V)
+        o.init(100, 2)
+        assert abs(o.solve([(100, 1e-11, 10, 0.0, 1e-10)]) - ef) < 1e-9
+
+
+def test_reference_test4_hubbard(oracle_lib):
+    # tests/test4.cpp.in:36-58 and :103 (DMRG within 1e-8 of FCI)
+    T, V = hubbard(10, t=1.0, U=2.0)
+    o = oracle_lib.OracleDMRG(10, 0, [0] * 10, 9, 5, 0, T, V)
+    o.init(30, 7)
+    e = o.solve([(30, 1e-10, 3, 0.1, 1e-5), (1000, 1e-10, 10, 0.0, 1e-5)])
+    ef = fci_energy(10, [0] * 10, 9, 5, 0, T, V)
+    assert abs(e - ef) < 1e-8
+
+
+def n2_sto3g():
+    d = read_fcidump(GOLDEN + "/N2.STO3G.FCIDUMP")
+    d["irreps"] = [molpro_to_xor(7, m) for m in d["molpro_irreps"]]
+    return d
+
+
+def test_reference_test5_n2_kat(oracle_lib):
+    # tests/test5.cpp.in:39-83: N2 STO-3G, D2h reordered, D=1000, E0 = -107.648250974014
+    d = n2_sto3g()
+    o = oracle_lib.OracleDMRG(d["L"], 7, d["irreps"], d["N"], 0, 0, d["T"], d["V"], d["econst"], reorder=True)
+    o.init(1000, 1)
+    e = o.solve([(1000, 1e-12, 100, 0.0, 1e-5)])
+    assert abs(e + 107.648250974014) < 1e-8
+
+
+def test_reference_test3_ch4_fci(oracle_lib):
+    # tests/test3.cpp.in:40-115: CH4 STO-3G in C2v, DMRG within 1e-8 of FCI
+    d = read_fcidump(GOLDEN + "/CH4.STO3G.FCIDUMP")
+    irr = [molpro_to_xor(5, m) for m in d["molpro_irreps"]]
+    L, N = d["L"], d["N"]
+    o = oracle_lib.OracleDMRG(L, 5, irr, N, 0, 0, d["T"], d["V"], d["econst"])
+    o.init(30, 3)
+    e = o.solve([(30, 1e-10, 3, 0.1, 1e-5), (1000, 1e-10, 10, 0.0, 1e-5)])
+    ef = fci_energy(L, irr, N, 0, 0, d["T"], d["V"], d["econst"])
+    assert abs(e - ef) < 1e-8
+
+
+def _site_setup(oracle_lib, L=8, N=8, TwoS=0, seed=4, D=12, index=3, group=7):
+    irr, T, V = random_abinitio(L, seed, group=group)
+    for target in range(8):
+        o = oracle_lib.OracleDMRG(L, group, irr, N, TwoS, target, T, V)
+        try:
+            fci_energy(L, irr, N, TwoS, target, T, V)
+        except ValueError:
+            continue
+        break
+    o.init(D, seed)
+    o.sweep_left_to(index, D, rtol=1e-4)
+    o.site(index)
+    return o
+
+
+@pytest.mark.parametrize("index", [0, 1, 3, 5, 6])
+def test_heff_hermitian_and_diagonal(oracle_lib, index):
+    o = _site_setup(oracle_lib, index=index)
+    V = o.V
+    rng = np.random.default_rng(index)
+    x, z = rng.normal(size=V), rng.normal(size=V)
+    hx, _ = o.apply(x)
+    hz, _ = o.apply(z)
+    assert abs(x @ hz - hx @ z) < 1e-10 * (abs(x @ hz) + 1)
+    d = o.diag()
+    idx = rng.choice(V, size=min(V, 40), replace=False)
+    for k in idx:
+        e = np.zeros(V)
+        e[k] = 1.0
+        assert abs(o.apply(e)[0][k] - d[k]) < 1e-11
+
+
+def test_every_site_exact_after_convergence(oracle_lib):
+    # With D above every FCI dimension and a converged MPS, the lowest Heff eigenvalue
+    # at every site is the FCI energy.
+    L, N = 7, 6
+    irr, T, V = random_abinitio(L, 21, group=7)
+    ef = fci_energy(L, irr, N, 0, 0, T, V)
+    o = oracle_lib.OracleDMRG(L, 7, irr, N, 0, 0, T, V)
+    o.init(5000, 2)
+    assert abs(o.solve([(5000, 1e-12, 10, 0.0, 1e-10)]) - ef) < 1e-9
+    for index in range(L - 2, -1, -1):
+        o.site(index)
+        e, x, nm = o.davidson(rtol=1e-10)
+        assert abs(e - ef) < 1e-9, index
+        if index > 0:
+            o.step_left(index, 5000, rtol=1e-10)
