@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark of the two-site effective-Hamiltonian product Heff x on one MI355X.
+
+Workload: random ab-initio Hamiltonian with D2h symmetry (chemps2_amd.problems.
+random_abinitio, seed 0), L = 32 orbitals, N = 32 electrons, 2S = 0, virtual dimension
+D = 2000 distributed over the sectors as SyBookkeeper does; the two-site step at the
+middle of the chain (sites 15, 16) with pseudo-random operator tensors of the
+reference's shapes. A step is one application y = Heff x with x, y and all operators
+already resident in HBM. value = F_H / time, where F_H is the GEMM flop count of the
+reference's makeHeff at this site.
+
+One JSON line goes to stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed rocprof PMC runs."""
+    path = os.path.join(ROOT, "profiles", "bspmv_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(L, D, site, seed, irr, T, V, x, y_gpu, budget_s):
+    """The oracle (single-threaded FP64 restatement of the reference) on a bounded sample
+    of output blocks of the same step; also checks the GPU result on that sample."""
+    import numpy as np
+
+    from oracle.wrap import OracleDMRG
+
+    o = OracleDMRG(L, 7, irr, L, 0, 0, T, V)
+    t0 = time.time()
+    n = o.synthetic(D, site, seed + 1, seed + 2)
+    log(f"cpu: oracle workload built in {time.time() - t0:.1f}s (V={n})")
+    lab, k2i = o.kappa()
+    nk = len(lab)
+    # grow the sample geometrically until it costs about budget_s seconds
+    k1, flops, secs = 0, 0.0, 0.0
+    step = max(1, nk // 512)
+    ys = np.zeros(n)
+    while k1 < nk and secs < budget_s:
+        k_next = min(nk, k1 + step)
+        y, f, s = o.apply_range(x, k1, k_next)
+        ys[k2i[k1]:k2i[k_next]] = y[k2i[k1]:k2i[k_next]]
+        flops += f
+        secs += s
+        k1 = k_next
+        step *= 2
+    a, b = k2i[0], k2i[k1]
+    err = float(np.abs(ys[a:b] - y_gpu[a:b]).max() / max(1e-300, np.abs(ys[a:b]).max()))
+    log(f"cpu: {k1}/{nk} output blocks in {secs:.1f}s, {flops:.3e} flop; GPU vs oracle on the sample: {err:.2e}")
+    return {
+        "value": flops / secs / 1e12,
+        "unit": "TFLOP/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle Heff x on output blocks 0..{k1 - 1} of {nk} at the same site and data "
+                  f"({flops:.3e} of the step's GEMM flops, {secs:.1f} s, one thread)",
+    }, err
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--L", type=int, default=32)
+    ap.add_argument("--D", type=int, default=2000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import chemps2_amd
+    from chemps2_amd.problems import mx_table, random_abinitio
+
+    L, D, seed = args.L, args.D, args.seed
+    site = L // 2 - 1
+    irr, T, V = random_abinitio(L, seed, group=7)
+    ctx = chemps2_amd.Context(0)
+    ctx.set_problem(L, 7, irr, L, 0, 0, mx_table(T, V, L))
+    ctx.set_sectors_auto(D)
+    ctx.fill_synthetic(site - 1, 1, seed + 1)
+    ctx.fill_synthetic(site + 1, 0, seed + 2)
+    t0 = time.time()
+    Vlen = ctx.prepare(site)
+    t_prep = time.time() - t0
+    F_H, B_H = ctx.flops()
+    st = ctx.plan_stats()
+    log(f"prepare: {t_prep:.1f}s V={Vlen} F_H={F_H:.4e} stats={st}")
+
+    torch.cuda.set_device(0)
+    rng = np.random.default_rng(seed)
+    xh = rng.normal(size=Vlen)
+    x = torch.from_numpy(xh).to("cuda")
+    y = torch.empty_like(x)
+    stream = torch.cuda.ExternalStream(ctx.stream())
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        ctx.apply(x, y)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        ctx.apply(x, y)
+    e1.record(stream)
+    e1.synchronize()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    y_gpu = y.cpu().numpy()
+    log(f"Heff x: {ms:.3f} ms per application, {F_H / ms / 1e9:.2f} TFLOP/s (F_H basis)")
+
+    # per-kernel durations with events around each launch
+    ctx.timing(True)
+    for _ in range(args.steps):
+        ctx.apply(x, y)
+    kms, cnt = ctx.timing_read()
+    ctx.timing(False)
+    kms = [v / max(cnt, 1) for v in kms]
+    log(f"kernels (ms per application): chain phase 1 {kms[0]:.4f}, explicit block product {kms[1]:.4f}, "
+        f"chain phase 2 {kms[2]:.4f}")
+    nnz = st["explicit_doubles"]
+    ncol_bytes = 4.0 * st.get("explicit_cols", 0)
+    bspmv_bytes = 8.0 * nnz + 16.0 * Vlen + ncol_bytes
+    achieved = bspmv_bytes / (kms[1] * 1e-3) / 1e9 if kms[1] > 0 else 0.0
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / PEAK_HBM_GBS, 4),
+        "traffic": read_traffic(),
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        cpu, err = cpu_baseline(L, D, site, seed, irr, T, V, xh, y_gpu, args.cpu_seconds)
+        if err > 1e-10:
+            log(f"WARNING: GPU result differs from the oracle on the sample: {err:.2e}")
+
+    out = {
+        "metric": "Heff·x TFLOP/s (F_H basis) at L=32 D=2000, two-site step at the chain middle",
+        "value": round(F_H / (ms * 1e-3) / 1e12, 4),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "Heff x, random ab-initio D2h Hamiltonian, L=32 N=32 2S=0 D=2000, sites 15-16",
+            "L": L, "D": D, "site": site, "V": Vlen, "F_H": F_H, "B_H": B_H,
+            "explicit_doubles": nnz, "prepare_s": round(t_prep, 2),
+            "kernel_ms": [round(v, 4) for v in kms],
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

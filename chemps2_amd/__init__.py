@@ -1,0 +1,217 @@
+"""chemps2_amd: MI355X implementation of CheMPS2's two-site effective-Hamiltonian path.
+
+Python access to the C ABI of ``libchemps2_amd.so`` (declared in
+``include/chemps2_amd.h``). ``Heff`` mirrors the reference's ``CheMPS2::Heff``
+(Heff.h:50-89): construct it for a prepared two-site step, then ``makeHeff``,
+``fillHeffDiag`` and ``SolveDAVIDSON`` run on the GPU. There is no CPU fallback: loading
+fails loudly when the library has not been built or no GPU is present.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libchemps2_amd.so")
+
+KINDS = {"L": 0, "F0": 1, "F1": 2, "S0": 3, "S1": 4, "A": 5, "B": 6, "C": 7, "D": 8, "Q": 9, "X": 10}
+EXPORTS = [
+    "chm_ctx_create", "chm_ctx_destroy", "chm_last_error", "chm_set_problem", "chm_set_sectors",
+    "chm_set_sectors_auto", "chm_get_sectors", "chm_tensor_upload", "chm_tensor_fill_synthetic",
+    "chm_tensor_clear", "chm_heff_prepare", "chm_heff_veclen", "chm_heff_apply", "chm_heff_diag",
+    "chm_davidson", "chm_flops", "chm_plan_stats", "chm_stream", "chm_timing_enable", "chm_timing_read",
+]
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class ChmError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libchemps2_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ChmError(f"{LIB_PATH} is missing: build it with chemps2_amd/build.py (hipcc, gfx950)")
+    l = C.CDLL(LIB_PATH)
+    vp, i, d, i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+    sig = {
+        "chm_ctx_create": (i, [i, C.POINTER(vp)]),
+        "chm_ctx_destroy": (i, [vp]),
+        "chm_last_error": (C.c_char_p, [vp]),
+        "chm_set_problem": (i, [vp, i, i, _ip, i, i, i, _dp, d]),
+        "chm_set_sectors": (i, [vp, i, i, _ip, _ip, _ip, _ip]),
+        "chm_set_sectors_auto": (i, [vp, i]),
+        "chm_get_sectors": (i, [vp, i, C.POINTER(i), vp, vp, vp, vp]),
+        "chm_tensor_upload": (i, [vp, i, i, i, i, i, i, i, i, _dp, i64]),
+        "chm_tensor_fill_synthetic": (i, [vp, i, i, C.c_uint64]),
+        "chm_tensor_clear": (i, [vp, i]),
+        "chm_heff_prepare": (i, [vp, i]),
+        "chm_heff_veclen": (i, [vp, C.POINTER(i64)]),
+        "chm_heff_apply": (i, [vp, vp, vp, i]),
+        "chm_heff_diag": (i, [vp, vp, i]),
+        "chm_davidson": (i, [vp, vp, i, d, C.POINTER(d), C.POINTER(i)]),
+        "chm_flops": (i, [vp, C.POINTER(d), C.POINTER(d)]),
+        "chm_plan_stats": (i, [vp, C.POINTER(i64)]),
+        "chm_stream": (i, [vp, C.POINTER(vp)]),
+        "chm_timing_enable": (i, [vp, i]),
+        "chm_timing_read": (i, [vp, C.POINTER(d), C.POINTER(i64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(l, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = l
+    return l
+
+
+def _ptr(a):
+    """Address of a numpy array or a torch tensor (device pointers for on_device calls)."""
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+class Context:
+    """One library context on one GPU (chm_ctx)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        self.h = C.c_void_p()
+        rc = self.lib.chm_ctx_create(int(device), C.byref(self.h))
+        if rc != 0:
+            raise ChmError(f"chm_ctx_create failed ({rc}): no usable HIP device {device}")
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.chm_last_error(self.h)
+            raise ChmError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.lib.chm_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_problem(self, L, group, orb_irreps, N, two_s, irrep, mx, econst=0.0):
+        mx = np.ascontiguousarray(mx, dtype=np.float64).ravel()
+        self.L = L
+        self._check(self.lib.chm_set_problem(self.h, L, group, np.ascontiguousarray(orb_irreps, dtype=np.int32), N,
+                                             two_s, irrep, mx, float(econst)), "chm_set_problem")
+
+    def set_sectors(self, boundary, sectors):
+        s = np.asarray(sectors, dtype=np.int32).reshape(-1, 4)
+        cols = [np.ascontiguousarray(s[:, c]) for c in range(4)]
+        self._check(self.lib.chm_set_sectors(self.h, boundary, len(s), *cols), "chm_set_sectors")
+
+    def set_sectors_auto(self, D):
+        self._check(self.lib.chm_set_sectors_auto(self.h, int(D)), "chm_set_sectors_auto")
+
+    def get_sectors(self, boundary):
+        n = C.c_int(0)
+        self._check(self.lib.chm_get_sectors(self.h, boundary, C.byref(n), None, None, None, None), "chm_get_sectors")
+        a = [np.zeros(n.value, dtype=np.int32) for _ in range(4)]
+        self._check(self.lib.chm_get_sectors(self.h, boundary, C.byref(n), *[x.ctypes.data for x in a]), "chm_get_sectors")
+        return np.stack(a, axis=1)
+
+    def upload(self, cnt, moving_right, kind, k, m, two_j, n_elec, n_irrep, data):
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        self._check(self.lib.chm_tensor_upload(self.h, cnt, int(moving_right), KINDS[kind] if isinstance(kind, str) else kind,
+                                               k, m, two_j, n_elec, n_irrep, data, data.size), "chm_tensor_upload")
+
+    def fill_synthetic(self, cnt, moving_right, seed):
+        self._check(self.lib.chm_tensor_fill_synthetic(self.h, cnt, int(moving_right), int(seed)), "chm_tensor_fill_synthetic")
+
+    def clear(self, cnt):
+        self._check(self.lib.chm_tensor_clear(self.h, cnt), "chm_tensor_clear")
+
+    def prepare(self, site):
+        self._check(self.lib.chm_heff_prepare(self.h, int(site)), "chm_heff_prepare")
+        v = C.c_int64(0)
+        self._check(self.lib.chm_heff_veclen(self.h, C.byref(v)), "chm_heff_veclen")
+        self.V = v.value
+        return self.V
+
+    def apply(self, x, y=None):
+        if isinstance(x, np.ndarray):
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            y = np.zeros(self.V) if y is None else y
+            self._check(self.lib.chm_heff_apply(self.h, _ptr(x), _ptr(y), 0), "chm_heff_apply")
+            return y
+        self._check(self.lib.chm_heff_apply(self.h, _ptr(x), _ptr(y), 1), "chm_heff_apply")
+        return y
+
+    def diag(self):
+        d = np.zeros(self.V)
+        self._check(self.lib.chm_heff_diag(self.h, _ptr(d), 0), "chm_heff_diag")
+        return d
+
+    def davidson(self, x, rtol):
+        """x: program-convention guess (numpy, updated in place). Returns (eigenvalue, n_matvec)."""
+        e = C.c_double(0.0)
+        nm = C.c_int(0)
+        on_dev = 0 if isinstance(x, np.ndarray) else 1
+        self._check(self.lib.chm_davidson(self.h, _ptr(x), on_dev, float(rtol), C.byref(e), C.byref(nm)), "chm_davidson")
+        return e.value, nm.value
+
+    def flops(self):
+        f, b = C.c_double(0), C.c_double(0)
+        self._check(self.lib.chm_flops(self.h, C.byref(f), C.byref(b)), "chm_flops")
+        return f.value, b.value
+
+    def plan_stats(self):
+        s = (C.c_int64 * 16)()
+        self._check(self.lib.chm_plan_stats(self.h, s), "chm_plan_stats")
+        keys = ["terms", "combos", "combo_doubles", "W_doubles", "tiles1", "tiles2", "explicit_pairs", "chain_terms",
+                "blocks", "V", "diag_terms", "explicit_doubles", "explicit_cols", "explicit_threshold", "_14", "_15"]
+        return dict(zip(keys, list(s)))
+
+    def stream(self):
+        p = C.c_void_p()
+        self._check(self.lib.chm_stream(self.h, C.byref(p)), "chm_stream")
+        return p.value
+
+    def timing(self, enable):
+        self._check(self.lib.chm_timing_enable(self.h, int(enable)), "chm_timing_enable")
+
+    def timing_read(self):
+        """Accumulated milliseconds per kernel (chain phase 1, explicit product, chain
+        phase 2) and the number of timed applications."""
+        ms = (C.c_double * 3)()
+        n = C.c_int64(0)
+        self._check(self.lib.chm_timing_read(self.h, ms, C.byref(n)), "chm_timing_read")
+        return list(ms), n.value
+
+
+class Heff:
+    """Mirror of CheMPS2::Heff (Heff.h:50-89) for one prepared two-site step."""
+
+    def __init__(self, ctx, site, dvdson_rtol=1e-5):
+        self.ctx = ctx
+        self.site = site
+        self.rtol = dvdson_rtol
+        self.V = ctx.prepare(site)
+
+    def makeHeff(self, x):
+        """y = Heff x in the symmetric convention (Heff.cpp:43-248)."""
+        return self.ctx.apply(x)
+
+    def fillHeffDiag(self):
+        """diag(Heff) (Heff.cpp:250-315)."""
+        return self.ctx.diag()
+
+    def SolveDAVIDSON(self, s_storage):
+        """Lowest eigenpair from the Sobject storage (program convention, updated in
+        place); returns the eigenvalue (Heff.cpp:330-386)."""
+        e, self.n_matvec = self.ctx.davidson(s_storage, self.rtol)
+        return e

@@ -1,0 +1,212 @@
+// Host-side data structures of the chemps2_amd library: sector bookkeeping, the block
+// layouts of the reference's operator tensors and two-site object, and the device work
+// lists that one two-site effective-Hamiltonian application executes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace chm {
+
+// Error codes returned through the C ABI.
+enum {
+  OK = 0,
+  ERR_ARG = -1,
+  ERR_STATE = -2,
+  ERR_MISSING = -3,
+  ERR_SIZE = -4,
+  ERR_HIP = -5,
+  ERR_NUMERIC = -6,
+};
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+// (-1)^(two_p/2) for even two_p (CheMPS2 Special.h:36).
+inline int phase(int two_p) { return (((two_p / 2) % 2) != 0) ? -1 : 1; }
+double wigner6j(int a, int b, int c, int d, int e, int f);
+double wigner9j(int a, int b, int c, int d, int e, int f, int g, int h, int i);
+
+inline uint64_t key3(int n, int s, int i) { return ((uint64_t)(n & 0xffff)) | ((uint64_t)(s & 0xffff) << 16) | ((uint64_t)(i & 0xff) << 32); }
+
+// Current virtual dimensions d(b, N, 2S, I) of every boundary b = 0..L
+// (SyBookkeeper::gCurrentDim, SyBookkeeper.cpp:165).
+struct Sectors {
+  struct Sec { int N, S, I, d; };
+  std::vector<std::vector<Sec>> list;                        // [b], sorted by (N, 2S, I)
+  std::vector<std::unordered_map<uint64_t, int>> dimmap;     // [b] -> d
+  void resize(int L) { list.assign(L + 1, {}); dimmap.assign(L + 1, {}); }
+  int dim(int b, int N, int S, int I) const {
+    if (b < 0 || b >= (int)dimmap.size() || N < 0 || S < 0) return 0;
+    auto it = dimmap[b].find(key3(N, S, I));
+    return it == dimmap[b].end() ? 0 : it->second;
+  }
+};
+
+// Where a dense block lives: buffer id (an operator tensor, the combination arena,
+// the Heff input vector x) and the element offset inside it.
+struct Ref {
+  int32_t buf = -1;
+  int64_t off = 0;
+  bool ok() const { return buf >= 0; }
+};
+
+enum : int32_t { BUF_X = 0, BUF_COMBO = 1, BUF_TENSOR0 = 2 };
+
+// Block layout of one operator tensor (TensorOperator.cpp:29-102): blocks
+// (n, 2s, i) x (n + n_elec, 2s', i ^ n_irrep) with |s - s'| <= two_j.
+struct OpT {
+  int boundary = 0, two_j = 0, n_elec = 0, n_irrep = 0;
+  int32_t buf = -1;  // buffer id of its device copy
+  int64_t size = 0;
+  std::unordered_map<uint64_t, int64_t> off;  // key (n, s, i, s') -> element offset
+  void build(const Sectors& sec, int b, int two_j, int n_elec, int n_irrep);
+  Ref blk(int n1, int s1, int i1, int n2, int s2, int i2) const;
+  static uint64_t key(int n, int s, int i, int s2) { return key3(n, s, i) | ((uint64_t)(s2 & 0xffff) << 40); }
+};
+
+// All operators of one boundary, indexed like DMRG::allocateTensors
+// (DMRGoperators.cpp:909-1145): slot cnt holds boundary cnt + 1.
+struct OpSet {
+  int dir = 0;  // 1 built moving right, 2 built moving left
+  std::vector<OpT*> L, Q;
+  std::vector<std::vector<OpT*>> F0, F1, S0, S1, A, B, C, D;
+  OpT* X = nullptr;
+  std::vector<std::unique_ptr<OpT>> owned;
+};
+
+// Two-site object layout (Sobject.cpp:36-147): blocks (NL, 2SL, IL, N1, N2, 2J, NR,
+// 2SR, IR), column-major dL x dR at k2i[kappa].
+struct SLayout {
+  int index = 0;
+  std::vector<int> NL, SL, IL, N1, N2, TJ, NR, SR, IR;
+  std::vector<int64_t> k2i;
+  std::unordered_map<uint64_t, int> lut;
+  void build(const Sectors& sec, int index, int I1, int I2);
+  int kappa(int nl, int sl, int il, int n1, int n2, int tj, int nr, int sr, int ir) const;
+  int nk() const { return (int)NL.size(); }
+  int64_t size() const { return k2i.empty() ? 0 : k2i.back(); }
+};
+
+// ----------------------------------------------------------- work lists ----
+// y_out += alpha * op(Lb) * x_in * op(Rb); an absent Lb or Rb is the identity. With
+// both present the reference's association order is kept: (op(L) x) op(R), or
+// op(L) (x op(R)) when xr_first is set.
+struct Term {
+  int out, in;
+  double alpha;
+  Ref Lb, Rb;
+  char tl, tr;
+  bool xr_first;
+};
+
+// out = sum_q coef_q * src_q over equally shaped blocks (x-independent, built once per
+// site before the Davidson iterations).
+struct Combo {
+  int64_t out_off, size;
+  std::vector<std::pair<Ref, double>> src;
+};
+
+// Diagonal contribution to block kappa of diag(Heff): c + a * diag(L) (x) 1 + b * 1 (x)
+// diag(R) + w * diag(L) (x) diag(R).
+struct DiagTerm {
+  int k;
+  int kind;  // 0 constant, 1 left, 2 right, 3 outer
+  double coef;
+  Ref Lb, Rb;
+};
+
+// ---------------------------------------------------- device-side records --
+// Pointers flagged relative hold an element offset into the Heff input vector x
+// (operands) or output vector y (tile outputs), resolved by the kernel.
+struct DevGemmOp {  // one product accumulated into a tile
+  const double* A;
+  const double* B;
+  double alpha;
+  int lda, ldb, k;
+  int ta, tb;   // 0 'N', 1 'T'
+  int kind;     // 0 gemm, 1 axpy: C += alpha * A (A read with lda)
+  int flags;    // bit 0: A relative to x, bit 1: B relative to x
+  int pad;
+};
+struct DevTile {  // one 64 x 64 output tile and the products that accumulate into it
+  double* C;
+  int ldc, m0, n0, m, n;  // m, n: full extents of C
+  int op0, nop;
+  int flags;  // bit 0: C relative to y; bit 1: add to C instead of overwriting it
+};
+// Explicit block-sparse Heff: output block kappa holds a dense R x C column-major
+// matrix (R = |kappa|) whose column c multiplies x[col[c]].
+struct DevRowBlock {
+  int64_t moff;  // start of the R x C matrix in the value array
+  int64_t coff;  // start of its column indices
+  int64_t yoff;  // k2i[kappa]
+  int R, C;
+};
+// One (kappa, kappa') pair: its R x |kappa'| sub-block is sum_t alpha_t op(A_t) (x) op(B_t).
+struct DevPair {
+  int64_t moff;  // first element of the sub-block (leading dimension R)
+  int R, dL, dR, dLi, dRi;
+  int t0, nt;
+  int pad;
+};
+struct DevKTerm {
+  const double* A;  // op(A) is dL x dLi; null: identity
+  const double* B;  // op(B) is dRi x dR; null: identity
+  double alpha;
+  int lda, ldb, ta, tb;
+};
+struct DevCombo {
+  double* out;
+  int64_t size;
+  int src0, nsrc;
+};
+struct DevSrc {
+  const double* p;
+  double c;
+};
+struct DevDiag {
+  int kind;
+  int pad;
+  double coef;
+  const double* L;
+  const double* R;
+};
+struct DevDiagBlock {
+  double* out;
+  int dL, dR;
+  int t0, nt;
+};
+
+// Launchers (kernels.hip).
+void launch_tiles(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const double* xbase, double* ybase,
+                  hipStream_t s);
+void launch_combos(const DevCombo* d_combos, const DevSrc* d_src, int ncombo, const int64_t* d_chunk_start,
+                   int64_t nchunks, hipStream_t s);
+void launch_diag(const DevDiagBlock* blocks, const DevDiag* terms, int nblocks, hipStream_t s);
+void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs, double* M, hipStream_t s);
+void launch_bspmv(const DevRowBlock* blocks, int nblocks, const double* M, const int* col, const double* x, double* y,
+                  hipStream_t s);
+int dot_parts(int64_t n);
+void launch_dot(const double* a, const double* b, int64_t n, double* partial, double* out, hipStream_t s);
+void launch_multi_dot(const double* const* d_vecs, int nv, const double* b, int64_t n, double* partial, double* out,
+                      hipStream_t s);
+void launch_axpy_dev(const double* coef, double sign, const double* x, double* y, int64_t n, hipStream_t s);
+void launch_axpy(double a, const double* x, double* y, int64_t n, hipStream_t s);
+void launch_scale(double a, double* x, int64_t n, hipStream_t s);
+void launch_lincomb_vecs(const double* const* d_vecs, const double* d_coefs, int nv, double* out, int64_t n,
+                         hipStream_t s);
+void launch_precond(const double* diag, const double* u, double shift, double cutoff, double* work, double* t,
+                    int64_t n, int stage, double alpha, hipStream_t s);
+void launch_symm_scale(double* x, const int64_t* d_k2i, const double* d_scale, int nk, hipStream_t s);
+
+}  // namespace chm

@@ -1,0 +1,888 @@
+// Context, device work-list construction, the Davidson driver and the C ABI.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+
+#include "../../include/chemps2_amd.h"
+#include "context.hpp"
+
+namespace chm {
+
+void launch_fill_hash(double* p, int64_t n, uint64_t seed, double scale, hipStream_t s);
+
+Ctx::~Ctx() {
+  for (double* p : bufs)
+    if (p) (void)hipFree(p);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Ctx::set_problem(int L_, int group_, const int* irr, int N_, int TwoS_, int Irrep_, const double* mx_,
+                      double econst_) {
+  static const int nirr_of[8] = {1, 2, 2, 2, 4, 4, 4, 8};
+  if (L_ < 2 || group_ < 0 || group_ > 7 || N_ < 0 || TwoS_ < 0 || N_ > 2 * L_ || (N_ % 2) != (TwoS_ % 2))
+    throw Error{ERR_ARG, "inconsistent problem (L, group, N, 2S)"};
+  L = L_; group = group_; nirrep = nirr_of[group_]; N = N_; TwoS = TwoS_; Irrep = Irrep_;
+  if (Irrep < 0 || Irrep >= nirrep) throw Error{ERR_ARG, "target irrep out of range"};
+  irreps.assign(irr, irr + L);
+  for (int v : irreps)
+    if (v < 0 || v >= nirrep) throw Error{ERR_ARG, "orbital irrep out of range"};
+  mx.assign(mx_, mx_ + (size_t)L * L * L * L);
+  econst = econst_;
+  sec.resize(L);
+  for (double* p : bufs)
+    if (p) (void)hipFree(p);
+  bufs.assign(BUF_TENSOR0, nullptr);
+  buf_len.assign(BUF_TENSOR0, 0);
+  sets.clear();
+  sets.resize(L - 1);
+  site = -1;
+  plan.reset();
+  have_problem = true;
+}
+
+void Ctx::set_sectors(int b, int n, const int* Ns, const int* Ss, const int* Is, const int* ds) {
+  if (!have_problem) throw Error{ERR_STATE, "set the problem first"};
+  if (b < 0 || b > L || n < 0) throw Error{ERR_ARG, "boundary out of range"};
+  std::vector<Sectors::Sec> v;
+  for (int q = 0; q < n; q++) {
+    if (ds[q] <= 0) continue;
+    if (Ns[q] < 0 || Ss[q] < 0 || Is[q] < 0 || Is[q] >= nirrep) throw Error{ERR_ARG, "bad sector label"};
+    v.push_back({Ns[q], Ss[q], Is[q], ds[q]});
+  }
+  std::sort(v.begin(), v.end(), [](const Sectors::Sec& a, const Sectors::Sec& b2) {
+    return std::make_tuple(a.N, a.S, a.I) < std::make_tuple(b2.N, b2.S, b2.I);
+  });
+  sec.list[b] = v;
+  sec.dimmap[b].clear();
+  for (auto& s : v) sec.dimmap[b][key3(s.N, s.S, s.I)] = s.d;
+  site = -1;
+}
+
+OpT* Ctx::add_tensor(int cnt, bool mr, int kind, int k, int m, int two_j, int n_elec, int n_irrep, int64_t n) {
+  if (!have_problem) throw Error{ERR_STATE, "set the problem first"};
+  if (cnt < 0 || cnt > L - 2) throw Error{ERR_ARG, "slot out of range"};
+  if (kind < 0 || kind >= KIND_COUNT || k < 0 || m < 0) throw Error{ERR_ARG, "bad tensor index"};
+  OpSet& s = sets[cnt];
+  const int dir = mr ? 1 : 2;
+  if (s.dir != dir) clear_set(cnt);
+  s.dir = dir;
+  auto owned = std::make_unique<OpT>();
+  owned->build(sec, cnt + 1, two_j, n_elec, n_irrep);
+  if (n >= 0 && owned->size != n)
+    throw Error{ERR_SIZE, "tensor size " + std::to_string(n) + " != layout size " + std::to_string(owned->size)};
+  OpT* t = owned.get();
+  auto slot1 = [&](std::vector<OpT*>& v) -> OpT*& {
+    if ((int)v.size() <= k) v.resize(k + 1, nullptr);
+    return v[k];
+  };
+  auto slot2 = [&](std::vector<std::vector<OpT*>>& v) -> OpT*& {
+    if ((int)v.size() <= k) v.resize(k + 1);
+    if ((int)v[k].size() <= m) v[k].resize(m + 1, nullptr);
+    return v[k][m];
+  };
+  OpT** where = nullptr;
+  switch (kind) {
+    case KIND_L: where = &slot1(s.L); break;
+    case KIND_Q: where = &slot1(s.Q); break;
+    case KIND_X: where = &s.X; break;
+    case KIND_F0: where = &slot2(s.F0); break;
+    case KIND_F1: where = &slot2(s.F1); break;
+    case KIND_S0: where = &slot2(s.S0); break;
+    case KIND_S1: where = &slot2(s.S1); break;
+    case KIND_A: where = &slot2(s.A); break;
+    case KIND_B: where = &slot2(s.B); break;
+    case KIND_C: where = &slot2(s.C); break;
+    case KIND_D: where = &slot2(s.D); break;
+  }
+  if (*where) {  // replace: free the old device copy
+    if ((*where)->buf >= 0 && bufs[(*where)->buf]) {
+      (void)hipFree(bufs[(*where)->buf]);
+      bufs[(*where)->buf] = nullptr;
+      buf_len[(*where)->buf] = 0;
+    }
+  }
+  double* p = nullptr;
+  if (t->size > 0) hip_check(hipMalloc((void**)&p, t->size * sizeof(double)), "hipMalloc tensor");
+  t->buf = (int32_t)bufs.size();
+  bufs.push_back(p);
+  buf_len.push_back(t->size);
+  *where = t;
+  s.owned.push_back(std::move(owned));
+  site = -1;
+  return t;
+}
+
+void Ctx::clear_set(int cnt) {
+  OpSet& s = sets[cnt];
+  for (auto& o : s.owned)
+    if (o->buf >= 0 && bufs[o->buf]) {
+      (void)hipFree(bufs[o->buf]);
+      bufs[o->buf] = nullptr;
+      buf_len[o->buf] = 0;
+    }
+  s = OpSet();
+  site = -1;
+}
+
+const double* Ctx::resolve(const Ref& r) const {
+  if (r.buf == BUF_COMBO) return combo_arena.p + r.off;
+  if (r.buf >= BUF_TENSOR0 && r.buf < (int)bufs.size() && bufs[r.buf]) return bufs[r.buf] + r.off;
+  throw Error{ERR_MISSING, "unresolvable block reference"};
+}
+
+void Ctx::prepare(int st) {
+  if (!have_problem) throw Error{ERR_STATE, "set the problem first"};
+  if (st < 0 || st > L - 2) throw Error{ERR_ARG, "site out of range"};
+  site = -1;
+  OpSet* Lo = nullptr;
+  OpSet* Ro = nullptr;
+  if (st > 0) { validate_set(st - 1, true); Lo = &sets[st - 1]; }
+  if (st < L - 2) { validate_set(st + 1, false); Ro = &sets[st + 1]; }
+  S.build(sec, st, irreps[st], irreps[st + 1]);
+  V = S.size();
+  if (V <= 0) throw Error{ERR_STATE, "empty two-site object"};
+  plan = std::make_unique<HeffPlanner>();
+  plan->init(&sec, &irreps, nirrep, mx.data(), L, &S, Lo, Ro);
+  plan->build();
+  F_H = plan->flops;
+
+  // ---- x-independent combinations, evaluated now
+  combo_arena.alloc(std::max<int64_t>(plan->combo_size, 1));
+  {
+    std::vector<DevCombo> dc;
+    std::vector<DevSrc> ds;
+    std::vector<int64_t> chunk_start;
+    int64_t nch = 0;
+    for (const Combo& c : plan->combos) {
+      DevCombo d;
+      d.out = combo_arena.p + c.out_off;
+      d.size = c.size;
+      d.src0 = (int)ds.size();
+      d.nsrc = (int)c.src.size();
+      for (auto& s : c.src) ds.push_back({resolve(s.first), s.second});
+      dc.push_back(d);
+      chunk_start.push_back(nch);
+      nch += (c.size + 1023) / 1024;
+    }
+    if (!dc.empty()) {
+      DBuf<DevCombo> d_dc;
+      DBuf<DevSrc> d_ds;
+      DBuf<int64_t> d_cs;
+      d_dc.upload(dc, stream);
+      d_ds.upload(ds, stream);
+      d_cs.upload(chunk_start, stream);
+      launch_combos(d_dc.p, d_ds.p, (int)dc.size(), d_cs.p, nch, stream);
+      hip_check(hipGetLastError(), "combination kernel launch");
+      hip_check(hipStreamSynchronize(stream), "combination kernel");
+    }
+  }
+
+  // ---- split the terms: (kappa, kappa') pairs small enough are assembled into an
+  // explicit block-sparse matrix once per site; the rest run as chained GEMM tiles.
+  auto dLof = [&](int k) { return (int)(S.k2i[k + 1] - S.k2i[k]) / std::max(1, sec.dim(st + 2, S.NR[k], S.SR[k], S.IR[k])); };
+  auto dRof = [&](int k) { return sec.dim(st + 2, S.NR[k], S.SR[k], S.IR[k]); };
+  auto bsize = [&](int k) { return (int64_t)(S.k2i[k + 1] - S.k2i[k]); };
+  const std::vector<Term>& terms = plan->terms;
+  std::vector<int64_t> order(terms.size());
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    return terms[a].out != terms[b].out ? terms[a].out < terms[b].out : terms[a].in < terms[b].in;
+  });
+  struct HP { int out, in; int64_t q0, q1, elems; };
+  std::vector<HP> hp;
+  for (int64_t i = 0; i < (int64_t)order.size(); i++) {
+    const Term& t = terms[order[i]];
+    if (hp.empty() || hp.back().out != t.out || hp.back().in != t.in) hp.push_back({t.out, t.in, i, i, bsize(t.out) * bsize(t.in)});
+    hp.back().q1 = i + 1;
+  }
+  int64_t thr = explicit_max;
+  {
+    size_t freeb = 0, totb = 0;
+    hip_check(hipMemGetInfo(&freeb, &totb), "hipMemGetInfo");
+    const double budget = explicit_budget_gb > 0 ? explicit_budget_gb * 1e9 : 0.4 * (double)freeb;
+    while (thr > 0) {
+      double nnz = 0.0;
+      for (const HP& h : hp) if (h.elems <= thr) nnz += (double)h.elems;
+      if (nnz * 8.0 <= budget) break;
+      thr /= 2;
+    }
+  }
+  explicit_threshold = thr;
+  std::vector<DevRowBlock> rb(S.nk());
+  std::vector<int> colidx;
+  std::vector<DevPair> dp;
+  std::vector<DevKTerm> kt;
+  std::vector<char> chain(terms.size(), 0);
+  {
+    int64_t moff = 0;
+    size_t h = 0;
+    for (int k = 0; k < S.nk(); k++) {
+      const int R = (int)bsize(k);
+      size_t h0 = h;
+      while (h < hp.size() && hp[h].out == k) h++;
+      int C = 0;
+      for (size_t j = h0; j < h; j++) if (hp[j].elems <= thr) C += (int)bsize(hp[j].in);
+      rb[k].moff = moff;
+      rb[k].coff = (int64_t)colidx.size();
+      rb[k].yoff = S.k2i[k];
+      rb[k].R = R;
+      rb[k].C = C;
+      int cpos = 0;
+      for (size_t j = h0; j < h; j++) {
+        const HP& P = hp[j];
+        if (P.elems > thr) {
+          for (int64_t i = P.q0; i < P.q1; i++) chain[order[i]] = 1;
+          continue;
+        }
+        DevPair pr{};
+        pr.moff = moff + (int64_t)R * cpos;
+        pr.R = R; pr.dL = dLof(k); pr.dR = dRof(k); pr.dLi = dLof(P.in); pr.dRi = dRof(P.in);
+        pr.t0 = (int)kt.size();
+        for (int64_t i = P.q0; i < P.q1; i++) {
+          const Term& t = terms[order[i]];
+          DevKTerm e{};
+          e.alpha = t.alpha;
+          if (t.tl != 'I') { e.A = resolve(t.Lb); e.lda = (t.tl == 'N') ? pr.dL : pr.dLi; e.ta = (t.tl == 'T'); }
+          if (t.tr != 'I') { e.B = resolve(t.Rb); e.ldb = (t.tr == 'N') ? pr.dRi : pr.dR; e.tb = (t.tr == 'T'); }
+          kt.push_back(e);
+        }
+        pr.nt = (int)kt.size() - pr.t0;
+        dp.push_back(pr);
+        const int64_t base = S.k2i[P.in];
+        for (int c = 0; c < (int)bsize(P.in); c++) colidx.push_back((int)(base + c));
+        cpos += (int)bsize(P.in);
+      }
+      moff += (int64_t)R * C;
+    }
+    if ((int64_t)kt.size() > INT32_MAX) throw Error{ERR_SIZE, "too many explicit terms"};
+    Mvals.alloc(std::max<int64_t>(moff, 1));
+    Mcol.upload(colidx.empty() ? std::vector<int>{0} : colidx, stream);
+    rowblocks.upload(rb, stream);
+    nrowblocks = S.nk();
+    if (!dp.empty()) {
+      DBuf<DevPair> d_dp;
+      DBuf<DevKTerm> d_kt;
+      d_dp.upload(dp, stream);
+      d_kt.upload(kt, stream);
+      for (auto& pr : dp) (void)pr;
+      launch_assemble(d_dp.p, d_kt.p, (int64_t)dp.size(), Mvals.p, stream);
+      hip_check(hipGetLastError(), "assembly kernel launch");
+      hip_check(hipStreamSynchronize(stream), "assembly");
+    }
+    stats[11] = moff;
+    stats[12] = (int64_t)colidx.size();
+    stats[13] = thr;
+  }
+
+  // ---- chained GEMM tiles for the remaining terms
+  std::vector<DevGemmOp> o1, o2;
+  std::vector<DevTile> t1, t2;
+  int64_t Wsize = 0;
+  struct WRec { int64_t off; int rows, cols; };
+  std::vector<WRec> wrec(terms.size(), {-1, 0, 0});
+  auto rel_x = [&](int k) { return (const double*)(intptr_t)S.k2i[k]; };
+  // phase 1: one group per two-operand term
+  for (size_t q = 0; q < terms.size(); q++) {
+    const Term& t = terms[q];
+    if (!chain[q] || t.tl == 'I' || t.tr == 'I') continue;
+    const int dL = dLof(t.out), dR = dRof(t.out), dLi = dLof(t.in), dRi = dRof(t.in);
+    DevGemmOp op{};
+    WRec w;
+    if (!t.xr_first) {
+      w = {Wsize, dL, dRi};
+      op.A = resolve(t.Lb); op.lda = (t.tl == 'N') ? dL : dLi; op.ta = (t.tl == 'T');
+      op.B = rel_x(t.in); op.ldb = dLi; op.tb = 0; op.k = dLi; op.flags = 2;
+    } else {
+      w = {Wsize, dLi, dR};
+      op.A = rel_x(t.in); op.lda = dLi; op.ta = 0;
+      op.B = resolve(t.Rb); op.ldb = (t.tr == 'N') ? dRi : dR; op.tb = (t.tr == 'T'); op.k = dRi; op.flags = 1;
+    }
+    op.alpha = t.alpha;
+    op.kind = 0;
+    wrec[q] = w;
+    Wsize += ((int64_t)w.rows * w.cols + 7) / 8 * 8;
+    const int opi = (int)o1.size();
+    o1.push_back(op);
+    for (int m0 = 0; m0 < w.rows; m0 += 64)
+      for (int n0 = 0; n0 < w.cols; n0 += 64) {
+        DevTile tl{};
+        tl.C = (double*)(intptr_t)w.off;  // patched below once W exists
+        tl.ldc = w.rows; tl.m0 = m0; tl.n0 = n0; tl.m = w.rows; tl.n = w.cols; tl.op0 = opi; tl.nop = 1; tl.flags = 0;
+        t1.push_back(tl);
+      }
+  }
+  W.alloc(std::max<int64_t>(Wsize, 1));
+  for (auto& tl : t1) tl.C = W.p + (intptr_t)tl.C;
+  // phase 2: one group per output block, added to the explicit product
+  std::vector<std::vector<int64_t>> by_out(S.nk());
+  for (size_t q = 0; q < terms.size(); q++)
+    if (chain[q]) by_out[terms[q].out].push_back((int64_t)q);
+  for (int k = 0; k < S.nk(); k++) {
+    if (by_out[k].empty()) continue;
+    const int dL = dLof(k), dR = dRof(k);
+    const int op0 = (int)o2.size();
+    for (int64_t q : by_out[k]) {
+      const Term& t = terms[q];
+      const int dLi = dLof(t.in), dRi = dRof(t.in);
+      DevGemmOp op{};
+      op.alpha = t.alpha;
+      if (t.tl == 'I' && t.tr == 'I') {
+        op.kind = 1; op.A = rel_x(t.in); op.lda = dL; op.flags = 1;
+      } else if (t.tr == 'I') {
+        op.A = resolve(t.Lb); op.lda = (t.tl == 'N') ? dL : dLi; op.ta = (t.tl == 'T');
+        op.B = rel_x(t.in); op.ldb = dLi; op.k = dLi; op.flags = 2;
+      } else if (t.tl == 'I') {
+        op.A = rel_x(t.in); op.lda = dLi; op.ta = 0;
+        op.B = resolve(t.Rb); op.ldb = (t.tr == 'N') ? dRi : dR; op.tb = (t.tr == 'T'); op.k = dRi; op.flags = 1;
+      } else if (!t.xr_first) {
+        const WRec& w = wrec[q];
+        op.alpha = 1.0;
+        op.A = W.p + w.off; op.lda = w.rows; op.ta = 0;
+        op.B = resolve(t.Rb); op.ldb = (t.tr == 'N') ? dRi : dR; op.tb = (t.tr == 'T'); op.k = dRi;
+      } else {
+        const WRec& w = wrec[q];
+        op.alpha = 1.0;
+        op.A = resolve(t.Lb); op.lda = (t.tl == 'N') ? dL : dLi; op.ta = (t.tl == 'T');
+        op.B = W.p + w.off; op.ldb = w.rows; op.tb = 0; op.k = dLi;
+      }
+      o2.push_back(op);
+    }
+    const int nop = (int)o2.size() - op0;
+    for (int m0 = 0; m0 < dL; m0 += 64)
+      for (int n0 = 0; n0 < dR; n0 += 64) {
+        DevTile tl{};
+        tl.C = (double*)(intptr_t)S.k2i[k];
+        tl.ldc = dL; tl.m0 = m0; tl.n0 = n0; tl.m = dL; tl.n = dR; tl.op0 = op0; tl.nop = nop; tl.flags = 3;
+        t2.push_back(tl);
+      }
+  }
+  ops1.upload(o1, stream);
+  ops2.upload(o2, stream);
+  tiles1.upload(t1, stream);
+  tiles2.upload(t2, stream);
+  ntiles1 = (int)t1.size();
+  ntiles2 = (int)t2.size();
+  stats[6] = (int64_t)dp.size();
+  stats[7] = (int64_t)std::count(chain.begin(), chain.end(), 1);
+
+  // ---- diagonal
+  {
+    std::vector<DevDiagBlock> blocks(S.nk());
+    std::vector<DevDiag> terms;
+    std::vector<std::vector<const DiagTerm*>> per(S.nk());
+    for (const DiagTerm& d : plan->diag) per[d.k].push_back(&d);
+    for (int k = 0; k < S.nk(); k++) {
+      blocks[k].out = (double*)(intptr_t)S.k2i[k];
+      blocks[k].dL = dLof(k);
+      blocks[k].dR = dRof(k);
+      blocks[k].t0 = (int)terms.size();
+      for (const DiagTerm* d : per[k]) {
+        DevDiag t{};
+        t.kind = d->kind;
+        t.coef = d->coef;
+        t.L = d->Lb.ok() ? resolve(d->Lb) : nullptr;
+        t.R = d->Rb.ok() ? resolve(d->Rb) : nullptr;
+        terms.push_back(t);
+      }
+      blocks[k].nt = (int)terms.size() - blocks[k].t0;
+    }
+    hblocks = blocks;
+    dblocks.alloc(blocks.size());
+    dterms.upload(terms, stream);
+    stats[10] = (int64_t)terms.size();
+  }
+
+  // ---- program <-> symmetric convention factors (Sobject.cpp:624-650)
+  {
+    std::vector<double> fs(S.nk()), fp(S.nk());
+    for (int k = 0; k < S.nk(); k++) { fs[k] = std::sqrt(S.SR[k] + 1.0); fp[k] = 1.0 / fs[k]; }
+    d_sym.upload(fs, stream);
+    d_prog.upload(fp, stream);
+    d_k2i.upload(S.k2i, stream);
+  }
+  dx.alloc(V);
+  dy.alloc(V);
+  partial.alloc((size_t)dot_parts(V) * 64);
+  scal.alloc(256);
+  hip_check(hipStreamSynchronize(stream), "prepare");
+
+  // ---- algorithmic bytes: the vectors and every operator tensor the step reads
+  {
+    std::vector<char> seen(bufs.size(), 0);
+    auto mark = [&](const Ref& r) { if (r.buf >= BUF_TENSOR0) seen[r.buf] = 1; };
+    for (const Term& t : plan->terms) { mark(t.Lb); mark(t.Rb); }
+    for (const Combo& c : plan->combos) for (auto& s : c.src) mark(s.first);
+    double bytes = 16.0 * V;
+    for (size_t b = 0; b < bufs.size(); b++) if (seen[b]) bytes += 8.0 * buf_len[b];
+    B_H = bytes;
+  }
+  stats[0] = (int64_t)plan->terms.size();
+  stats[1] = (int64_t)plan->combos.size();
+  stats[2] = plan->combo_size;
+  stats[3] = Wsize;
+  stats[4] = ntiles1;
+  stats[5] = ntiles2;
+  stats[8] = S.nk();
+  stats[9] = V;
+  site = st;
+}
+
+void Ctx::apply(const double* xp, double* yp) {
+  if (site < 0) throw Error{ERR_STATE, "no prepared site"};
+  if (timing) hip_check(hipEventRecord(ev[0], stream), "event");
+  launch_tiles(tiles1.p, ops1.p, ntiles1, xp, yp, stream);
+  if (timing) hip_check(hipEventRecord(ev[1], stream), "event");
+  launch_bspmv(rowblocks.p, nrowblocks, Mvals.p, Mcol.p, xp, yp, stream);
+  if (timing) hip_check(hipEventRecord(ev[2], stream), "event");
+  launch_tiles(tiles2.p, ops2.p, ntiles2, xp, yp, stream);
+  if (timing) hip_check(hipEventRecord(ev[3], stream), "event");
+  hip_check(hipGetLastError(), "Heff kernels");
+  if (timing) {
+    hip_check(hipEventSynchronize(ev[3]), "event sync");
+    for (int q = 0; q < 3; q++) {
+      float a = 0.f;
+      hip_check(hipEventElapsedTime(&a, ev[q], ev[q + 1]), "elapsed");
+      t_ms[q] += a;
+    }
+    t_count++;
+  }
+}
+
+void Ctx::diag(double* d) {
+  if (site < 0) throw Error{ERR_STATE, "no prepared site"};
+  std::vector<DevDiagBlock> blocks = hblocks;  // outputs relative to d
+  for (int k = 0; k < S.nk(); k++) blocks[k].out = d + S.k2i[k];
+  dblocks.upload(blocks, stream);
+  launch_diag(dblocks.p, dterms.p, S.nk(), stream);
+  hip_check(hipGetLastError(), "diag kernel");
+  hip_check(hipStreamSynchronize(stream), "diag");
+}
+
+void Ctx::symm(double* x, bool to_symm) {
+  launch_symm_scale(x, d_k2i.p, to_symm ? d_sym.p : d_prog.p, S.nk(), stream);
+}
+
+// ------------------------------------------------------------ Davidson -----
+static void jacobi_eig(int n, const std::vector<double>& A, int lda, std::vector<double>& w, std::vector<double>& Vv) {
+  std::vector<double> a((size_t)n * n), v((size_t)n * n, 0.0);
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) a[i + (size_t)n * j] = 0.5 * (A[i + (size_t)lda * j] + A[j + (size_t)lda * i]);
+  for (int i = 0; i < n; i++) v[i + (size_t)n * i] = 1.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0, tot = 0.0;
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < n; i++) {
+        const double x = a[i + (size_t)n * j] * a[i + (size_t)n * j];
+        tot += x;
+        if (i != j) off += x;
+      }
+    if (off == 0.0 || off <= 1e-32 * tot) break;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        const double apq = a[p + (size_t)n * q];
+        if (apq == 0.0) continue;
+        const double th = (a[q + (size_t)n * q] - a[p + (size_t)n * p]) / (2.0 * apq);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; k++) {
+          const double kp = a[k + (size_t)n * p], kq = a[k + (size_t)n * q];
+          a[k + (size_t)n * p] = c * kp - s * kq;
+          a[k + (size_t)n * q] = s * kp + c * kq;
+        }
+        for (int k = 0; k < n; k++) {
+          const double pk = a[p + (size_t)n * k], qk = a[q + (size_t)n * k];
+          a[p + (size_t)n * k] = c * pk - s * qk;
+          a[q + (size_t)n * k] = s * pk + c * qk;
+        }
+        for (int k = 0; k < n; k++) {
+          const double kp = v[k + (size_t)n * p], kq = v[k + (size_t)n * q];
+          v[k + (size_t)n * p] = c * kp - s * kq;
+          v[k + (size_t)n * q] = s * kp + c * kq;
+        }
+      }
+  }
+  std::vector<int> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](int x, int y) { return a[x + (size_t)n * x] < a[y + (size_t)n * y]; });
+  w.assign(n, 0.0);
+  Vv.assign((size_t)n * n, 0.0);
+  for (int j = 0; j < n; j++) {
+    w[j] = a[ord[j] + (size_t)n * ord[j]];
+    for (int i = 0; i < n; i++) Vv[i + (size_t)n * j] = v[i + (size_t)n * ord[j]];
+  }
+}
+
+// Davidson.cpp:31-536 in eigenvalue mode with the reference's constants
+// (DAVIDSON_NUM_VEC = 32, DAVIDSON_NUM_VEC_KEEP = 3, DAVIDSON_PRECOND_CUTOFF = 1e-12;
+// Options.h:70-74). Vectors live on the device; the projected problem on the host.
+double Ctx::davidson(double* x, double rtol, int* n_matvec) {
+  if (site < 0) throw Error{ERR_STATE, "no prepared site"};
+  const int MAXV = 32, KEEP = 3;
+  const double CUT = 1e-12;
+  const int64_t n = V;
+  DBuf<double> store;
+  store.alloc((size_t)(2 * MAXV + 4 + KEEP) * n);
+  std::vector<double*> vec(MAXV), hv(MAXV);
+  for (int i = 0; i < MAXV; i++) { vec[i] = store.p + (size_t)i * n; hv[i] = store.p + (size_t)(MAXV + i) * n; }
+  double* t = store.p + (size_t)(2 * MAXV) * n;
+  double* u = t + n;
+  double* work = u + n;
+  double* dg = work + n;
+  double* E = dg + n;
+  DBuf<const double*> d_ptrs;
+  DBuf<double> d_coef;
+  d_ptrs.alloc(2 * MAXV + 8);
+  d_coef.alloc(2 * MAXV + 8);
+  auto host_scalars = [&](int cnt) {
+    std::vector<double> h(cnt);
+    hip_check(hipMemcpyAsync(h.data(), scal.p, cnt * sizeof(double), hipMemcpyDeviceToHost, stream), "d2h");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    return h;
+  };
+  auto dot = [&](const double* a, const double* b) {
+    launch_dot(a, b, n, partial.p, scal.p, stream);
+    return host_scalars(1)[0];
+  };
+  auto lincomb = [&](const std::vector<const double*>& vs, const std::vector<double>& cs, double* out) {
+    hip_check(hipMemcpy(d_ptrs.p, vs.data(), vs.size() * sizeof(double*), hipMemcpyHostToDevice), "ptrs");
+    hip_check(hipMemcpy(d_coef.p, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice), "coefs");
+    launch_lincomb_vecs(d_ptrs.p, d_coef.p, (int)vs.size(), out, n, stream);
+  };
+  auto matvec = [&](int i) { apply(vec[i], hv[i]); };
+
+  // 'A': guess in the symmetric convention and the diagonal
+  hip_check(hipMemcpyAsync(t, x, n * sizeof(double), hipMemcpyDeviceToDevice, stream), "guess");
+  symm(t, true);
+  diag(dg);
+  std::vector<double> M((size_t)MAXV * MAXV, 0.0), eig, evec;
+  int num_vec = 0, nmult = 0;
+  auto add_new_vec = [&]() {  // Davidson.cpp:214-240 (modified Gram-Schmidt, scalars stay on the device)
+    for (int c = 0; c < num_vec; c++) {
+      launch_dot(t, vec[c], n, partial.p, scal.p, stream);
+      launch_axpy_dev(scal.p, -1.0, vec[c], t, n, stream);
+    }
+    const double nrm = std::sqrt(dot(t, t));
+    if (!(nrm > 0.0) || !std::isfinite(nrm)) throw Error{ERR_NUMERIC, "Davidson: new vector has zero or invalid norm"};
+    launch_scale(1.0 / nrm, t, n, stream);
+    std::swap(vec[num_vec], t);
+  };
+  // 'U': SafetyCheckGuess (Davidson.cpp:202-212)
+  if (dot(t, t) == 0.0) {
+    std::vector<double> h(n);
+    uint64_t s = 0x9E3779B97F4A7C15ULL;
+    for (int64_t i = 0; i < n; i++) {
+      s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+      h[i] = (s >> 11) * (1.0 / 9007199254740992.0);
+    }
+    hip_check(hipMemcpy(t, h.data(), n * sizeof(double), hipMemcpyHostToDevice), "random guess");
+  }
+  add_new_vec();
+  matvec(num_vec);
+  nmult++;
+  for (int iter = 0; iter < 100000; iter++) {
+    // Davidson.cpp:242-320
+    {
+      std::vector<const double*> hp(hv.begin(), hv.begin() + num_vec + 1);
+      hip_check(hipMemcpy(d_ptrs.p, hp.data(), hp.size() * sizeof(double*), hipMemcpyHostToDevice), "ptrs");
+      launch_multi_dot(d_ptrs.p, num_vec + 1, vec[num_vec], n, partial.p, scal.p, stream);
+      std::vector<double> h = host_scalars(num_vec + 1);
+      for (int c = 0; c <= num_vec; c++) { M[c + MAXV * num_vec] = h[c]; M[num_vec + MAXV * c] = h[c]; }
+    }
+    num_vec++;
+    jacobi_eig(num_vec, M, MAXV, eig, evec);
+    {
+      std::vector<const double*> vs;
+      std::vector<double> cs;
+      for (int c = 0; c < num_vec; c++) { vs.push_back(vec[c]); cs.push_back(evec[c]); }
+      lincomb(vs, cs, u);
+      for (int c = 0; c < num_vec; c++) { vs[c] = hv[c]; }
+      vs.push_back(u);
+      cs.push_back(-eig[0]);
+      lincomb(vs, cs, t);
+    }
+    const double rnorm = std::sqrt(dot(t, t));
+    if (!std::isfinite(rnorm)) throw Error{ERR_NUMERIC, "Davidson: residual is not finite"};
+    if (rnorm <= rtol) break;
+    // Davidson.cpp:322-350
+    launch_precond(dg, u, eig[0], CUT, work, t, n, 0, 0.0, stream);
+    launch_dot(work, t, n, partial.p, scal.p, stream);
+    launch_dot(work, u, n, partial.p, scal.p + 1, stream);
+    std::vector<double> wr = host_scalars(2);
+    const double alpha = -wr[0] / wr[1];
+    launch_precond(dg, u, eig[0], CUT, work, t, n, 1, alpha, stream);
+    if (num_vec == MAXV) {  // Davidson.cpp:352-412, 510-536
+      hip_check(hipMemcpyAsync(E, u, n * sizeof(double), hipMemcpyDeviceToDevice, stream), "deflate");
+      for (int c = 1; c < KEEP; c++) {
+        std::vector<const double*> vs(vec.begin(), vec.end());
+        std::vector<double> cs(MAXV);
+        for (int v = 0; v < MAXV; v++) cs[v] = evec[v + (size_t)num_vec * c];
+        lincomb(vs, cs, E + (size_t)c * n);
+      }
+      std::vector<double> Sm(KEEP * KEEP);
+      for (int a = 0; a < KEEP; a++)
+        for (int b = a; b < KEEP; b++) Sm[a + KEEP * b] = Sm[b + KEEP * a] = dot(E + (size_t)a * n, E + (size_t)b * n);
+      std::vector<double> se, sv;
+      jacobi_eig(KEEP, Sm, KEEP, se, sv);
+      for (int c = 0; c < KEEP; c++) {
+        const double f = std::pow(se[c], -0.25);
+        for (int r = 0; r < KEEP; r++) sv[r + KEEP * c] *= f;
+      }
+      for (int v = 0; v < KEEP; v++) {
+        std::vector<const double*> vs;
+        std::vector<double> cs;
+        for (int v2 = 0; v2 < KEEP; v2++) {
+          double low = 0.0;
+          for (int c = 0; c < KEEP; c++) low += sv[v2 + KEEP * c] * sv[v + KEEP * c];
+          vs.push_back(E + (size_t)v2 * n);
+          cs.push_back(low);
+        }
+        lincomb(vs, cs, vec[v]);
+      }
+      num_vec = 0;
+      for (int v = 0; v < KEEP; v++) { matvec(v); nmult++; num_vec++; }
+      for (int a = 0; a < KEEP; a++)
+        for (int b = a; b < KEEP; b++) M[a + MAXV * b] = M[b + MAXV * a] = dot(vec[a], hv[b]);
+    }
+    add_new_vec();
+    matvec(num_vec);
+    nmult++;
+  }
+  // 'C': copy back and convert to the program convention
+  hip_check(hipMemcpyAsync(x, u, n * sizeof(double), hipMemcpyDeviceToDevice, stream), "solution");
+  symm(x, false);
+  hip_check(hipStreamSynchronize(stream), "davidson");
+  if (n_matvec) *n_matvec = nmult;
+  return eig[0];
+}
+
+}  // namespace chm
+
+// ================================================================= C ABI =====
+using chm::Ctx;
+using chm::Error;
+
+struct chm_ctx {
+  Ctx c;
+};
+
+template <class F>
+static int guard(chm_ctx* ctx, F&& f) {
+  if (!ctx) return CHM_ERR_ARG;
+  try {
+    f(ctx->c);
+    ctx->c.err.clear();
+    return CHM_OK;
+  } catch (const Error& e) {
+    ctx->c.err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    ctx->c.err = e.what();
+    return CHM_ERR_STATE;
+  }
+}
+
+extern "C" {
+
+int chm_ctx_create(int device, chm_ctx** out) {
+  if (!out) return CHM_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return CHM_ERR_HIP;
+  auto* ctx = new (std::nothrow) chm_ctx();
+  if (!ctx) return CHM_ERR_STATE;
+  ctx->c.device = device;
+  if (const char* e = std::getenv("CHM_EXPLICIT_MAX")) ctx->c.explicit_max = std::atoll(e);
+  if (const char* e = std::getenv("CHM_EXPLICIT_BUDGET_GB")) ctx->c.explicit_budget_gb = std::atof(e);
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return CHM_ERR_HIP;
+  }
+  for (auto& e : ctx->c.ev)
+    if (hipEventCreate(&e) != hipSuccess) { delete ctx; return CHM_ERR_HIP; }
+  *out = ctx;
+  return CHM_OK;
+}
+
+int chm_ctx_destroy(chm_ctx* ctx) {
+  if (!ctx) return CHM_ERR_ARG;
+  (void)hipSetDevice(ctx->c.device);
+  delete ctx;
+  return CHM_OK;
+}
+
+const char* chm_last_error(const chm_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+int chm_set_problem(chm_ctx* ctx, int L, int group, const int* orb_irreps, int N, int two_s, int irrep,
+                    const double* mx, double econst) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!orb_irreps || !mx) throw Error{CHM_ERR_ARG, "null pointer"};
+    (void)hipSetDevice(c.device);
+    c.set_problem(L, group, orb_irreps, N, two_s, irrep, mx, econst);
+  });
+}
+
+int chm_set_sectors(chm_ctx* ctx, int boundary, int n, const int* N, const int* two_s, const int* irrep,
+                    const int* dim) {
+  return guard(ctx, [&](Ctx& c) {
+    if (n > 0 && (!N || !two_s || !irrep || !dim)) throw Error{CHM_ERR_ARG, "null pointer"};
+    c.set_sectors(boundary, n, N, two_s, irrep, dim);
+  });
+}
+
+int chm_set_sectors_auto(chm_ctx* ctx, int D) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!c.have_problem) throw Error{CHM_ERR_STATE, "set the problem first"};
+    if (D < 1) throw Error{CHM_ERR_ARG, "D must be positive"};
+    chm::auto_sectors(c, D);
+    c.site = -1;
+  });
+}
+
+int chm_get_sectors(const chm_ctx* ctx, int boundary, int* n, int* N, int* two_s, int* irrep, int* dim) {
+  if (!ctx || !n) return CHM_ERR_ARG;
+  const Ctx& c = ctx->c;
+  if (!c.have_problem || boundary < 0 || boundary > c.L) return CHM_ERR_ARG;
+  const auto& v = c.sec.list[boundary];
+  if (N && two_s && irrep && dim) {
+    if (*n < (int)v.size()) return CHM_ERR_SIZE;
+    for (size_t q = 0; q < v.size(); q++) { N[q] = v[q].N; two_s[q] = v[q].S; irrep[q] = v[q].I; dim[q] = v[q].d; }
+  }
+  *n = (int)v.size();
+  return CHM_OK;
+}
+
+int chm_tensor_upload(chm_ctx* ctx, int cnt, int moving_right, int kind, int k, int m, int two_j, int n_elec,
+                      int n_irrep, const double* data, int64_t n) {
+  return guard(ctx, [&](Ctx& c) {
+    if (n > 0 && !data) throw Error{CHM_ERR_ARG, "null pointer"};
+    (void)hipSetDevice(c.device);
+    chm::OpT* t = c.add_tensor(cnt, moving_right != 0, kind, k, m, two_j, n_elec, n_irrep, n);
+    if (t->size > 0)
+      chm::hip_check(hipMemcpy(c.bufs[t->buf], data, t->size * sizeof(double), hipMemcpyHostToDevice), "tensor upload");
+  });
+}
+
+int chm_tensor_fill_synthetic(chm_ctx* ctx, int cnt, int moving_right, uint64_t seed) {
+  return guard(ctx, [&](Ctx& c) {
+    (void)hipSetDevice(c.device);
+    if (cnt < 0 || cnt > c.L - 2) throw Error{CHM_ERR_ARG, "slot out of range"};
+    c.clear_set(cnt);
+    uint64_t s = seed * 1000003ULL + 17;
+    for (const chm::OpShape& sh : chm::opset_shapes(c, cnt, moving_right != 0)) {
+      chm::OpT* t = c.add_tensor(cnt, moving_right != 0, sh.kind, sh.k, sh.m, sh.two_j, sh.n_elec, sh.n_irrep, -1);
+      if (t->size > 0) chm::launch_fill_hash(c.bufs[t->buf], t->size, s++, 1.0 / std::sqrt((double)t->size), c.stream);
+    }
+    chm::hip_check(hipGetLastError(), "fill kernel");
+    chm::hip_check(hipStreamSynchronize(c.stream), "fill");
+  });
+}
+
+int chm_tensor_clear(chm_ctx* ctx, int cnt) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!c.have_problem || cnt < 0 || cnt > c.L - 2) throw Error{CHM_ERR_ARG, "slot out of range"};
+    c.clear_set(cnt);
+  });
+}
+
+int chm_heff_prepare(chm_ctx* ctx, int site) {
+  return guard(ctx, [&](Ctx& c) {
+    (void)hipSetDevice(c.device);
+    c.prepare(site);
+  });
+}
+
+int chm_heff_veclen(const chm_ctx* ctx, int64_t* V) {
+  if (!ctx || !V) return CHM_ERR_ARG;
+  if (ctx->c.site < 0) return CHM_ERR_STATE;
+  *V = ctx->c.V;
+  return CHM_OK;
+}
+
+int chm_heff_apply(chm_ctx* ctx, const double* x, double* y, int on_device) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!x || !y) throw Error{CHM_ERR_ARG, "null pointer"};
+    if (c.site < 0) throw Error{CHM_ERR_STATE, "no prepared site"};
+    (void)hipSetDevice(c.device);
+    if (on_device) {
+      c.apply(x, y);
+    } else {
+      chm::hip_check(hipMemcpyAsync(c.dx.p, x, c.V * sizeof(double), hipMemcpyHostToDevice, c.stream), "x upload");
+      c.apply(c.dx.p, c.dy.p);
+      chm::hip_check(hipMemcpyAsync(y, c.dy.p, c.V * sizeof(double), hipMemcpyDeviceToHost, c.stream), "y download");
+      chm::hip_check(hipStreamSynchronize(c.stream), "apply");
+    }
+  });
+}
+
+int chm_heff_diag(chm_ctx* ctx, double* d, int on_device) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!d) throw Error{CHM_ERR_ARG, "null pointer"};
+    if (c.site < 0) throw Error{CHM_ERR_STATE, "no prepared site"};
+    (void)hipSetDevice(c.device);
+    if (on_device) {
+      c.diag(d);
+    } else {
+      c.diag(c.dy.p);
+      chm::hip_check(hipMemcpy(d, c.dy.p, c.V * sizeof(double), hipMemcpyDeviceToHost), "diag download");
+    }
+  });
+}
+
+int chm_davidson(chm_ctx* ctx, double* x, int on_device, double rtol, double* eigenvalue, int* n_matvec) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!x || !eigenvalue) throw Error{CHM_ERR_ARG, "null pointer"};
+    if (c.site < 0) throw Error{CHM_ERR_STATE, "no prepared site"};
+    if (!(rtol > 0.0)) throw Error{CHM_ERR_ARG, "rtol must be positive"};
+    (void)hipSetDevice(c.device);
+    if (on_device) {
+      *eigenvalue = c.davidson(x, rtol, n_matvec);
+    } else {
+      chm::DBuf<double> buf;
+      buf.alloc(c.V);
+      chm::hip_check(hipMemcpy(buf.p, x, c.V * sizeof(double), hipMemcpyHostToDevice), "x upload");
+      *eigenvalue = c.davidson(buf.p, rtol, n_matvec);
+      chm::hip_check(hipMemcpy(x, buf.p, c.V * sizeof(double), hipMemcpyDeviceToHost), "x download");
+    }
+  });
+}
+
+int chm_flops(const chm_ctx* ctx, double* F_H, double* B_H) {
+  if (!ctx || !F_H || !B_H) return CHM_ERR_ARG;
+  if (ctx->c.site < 0) return CHM_ERR_STATE;
+  *F_H = ctx->c.F_H;
+  *B_H = ctx->c.B_H;
+  return CHM_OK;
+}
+
+int chm_plan_stats(const chm_ctx* ctx, int64_t* stats) {
+  if (!ctx || !stats) return CHM_ERR_ARG;
+  if (ctx->c.site < 0) return CHM_ERR_STATE;
+  std::memcpy(stats, ctx->c.stats, sizeof(ctx->c.stats));
+  return CHM_OK;
+}
+
+int chm_stream(const chm_ctx* ctx, void** stream) {
+  if (!ctx || !stream) return CHM_ERR_ARG;
+  *stream = (void*)ctx->c.stream;
+  return CHM_OK;
+}
+
+int chm_timing_enable(chm_ctx* ctx, int enable) {
+  if (!ctx) return CHM_ERR_ARG;
+  ctx->c.timing = enable != 0;
+  for (double& t : ctx->c.t_ms) t = 0.0;
+  ctx->c.t_count = 0;
+  return CHM_OK;
+}
+
+int chm_timing_read(const chm_ctx* ctx, double* ms, int64_t* count) {
+  if (!ctx || !ms || !count) return CHM_ERR_ARG;
+  for (int q = 0; q < 3; q++) ms[q] = ctx->c.t_ms[q];
+  *count = ctx->c.t_count;
+  return CHM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+// Library context: problem data, sector dimensions, the device operator pool and the
+// per-site Heff work lists.
+#pragma once
+#include <tuple>
+
+#include "chm_internal.hpp"
+#include "heff_plan.hpp"
+
+namespace chm {
+
+enum { KIND_L = 0, KIND_F0, KIND_F1, KIND_S0, KIND_S1, KIND_A, KIND_B, KIND_C, KIND_D, KIND_Q, KIND_X, KIND_COUNT };
+
+struct OpShape {
+  int kind, k, m, two_j, n_elec, n_irrep;
+};
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error{ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+// Owning device allocation.
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    release();
+    if (count == 0) return;
+    hip_check(hipMalloc((void**)&p, count * sizeof(T)), "hipMalloc");
+    n = count;
+  }
+  // Synchronous copy: the host vector may be released right after the call.
+  void upload(const std::vector<T>& v, hipStream_t s) {
+    alloc(v.size());
+    if (!v.empty()) {
+      hip_check(hipStreamSynchronize(s), "upload");
+      hip_check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+    }
+  }
+};
+
+// Host-side state: the problem, the sector dimensions and the operator layouts.
+struct HostState {
+  // problem (DMRG orbital order)
+  int L = 0, group = 0, nirrep = 1, N = 0, TwoS = 0, Irrep = 0;
+  std::vector<int> irreps;
+  std::vector<double> mx;
+  double econst = 0.0;
+  bool have_problem = false;
+  Sectors sec;
+  std::vector<OpSet> sets;  // slot cnt -> operator set
+  // Throws unless slot cnt holds every operator DMRG::allocateTensors creates there.
+  void validate_set(int cnt, bool mr) const;
+};
+
+struct Ctx : HostState {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // device buffers by id (ids >= BUF_TENSOR0 are operator tensors)
+  std::vector<double*> bufs;
+  std::vector<int64_t> buf_len;
+
+  // prepared site
+  int site = -1;
+  SLayout S;
+  std::unique_ptr<HeffPlanner> plan;
+  double F_H = 0.0, B_H = 0.0;
+  int64_t V = 0;
+  DBuf<double> combo_arena, W, dx, dy, partial, scal;
+  DBuf<DevTile> tiles1, tiles2;
+  DBuf<DevGemmOp> ops1, ops2;
+  int ntiles1 = 0, ntiles2 = 0;
+  DBuf<DevDiagBlock> dblocks;
+  std::vector<DevDiagBlock> hblocks;  // block outputs as offsets into the diagonal
+  DBuf<DevDiag> dterms;
+  DBuf<int64_t> d_k2i;
+  DBuf<double> d_sym, d_prog;  // per-block scale factors
+  int64_t stats[16] = {0};
+  // explicit block-sparse part of Heff (pairs with |kappa||kappa'| <= threshold)
+  int64_t explicit_max = 4096;       // CHM_EXPLICIT_MAX
+  double explicit_budget_gb = 0.0;   // CHM_EXPLICIT_BUDGET_GB (0: 40 % of free memory)
+  int64_t explicit_threshold = 0;
+  DBuf<double> Mvals;
+  DBuf<int> Mcol;
+  DBuf<DevRowBlock> rowblocks;
+  int nrowblocks = 0;
+
+  // optional timing of the three kernels of one application (events on the library stream)
+  bool timing = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double t_ms[3] = {0.0, 0.0, 0.0};
+  int64_t t_count = 0;
+
+  ~Ctx();
+  void set_problem(int L, int group, const int* irr, int N, int TwoS, int Irrep, const double* mx, double econst);
+  void set_sectors(int b, int n, const int* Ns, const int* Ss, const int* Is, const int* ds);
+  OpT* add_tensor(int cnt, bool mr, int kind, int k, int m, int two_j, int n_elec, int n_irrep, int64_t n);
+  void clear_set(int cnt);
+  void prepare(int site);
+  const double* resolve(const Ref& r) const;
+  void apply(const double* dxp, double* dyp);
+  void diag(double* d);
+  double davidson(double* x, double rtol, int* n_matvec);
+  void symm(double* x, bool to_symm);
+};
+
+void auto_sectors(HostState& c, int D);
+std::vector<OpShape> opset_shapes(const HostState& c, int cnt, bool mr);
+
+}  // namespace chm

@@ -1,0 +1,411 @@
+// gfx950 kernels of the chemps2_amd library (FP64).
+//
+// k_tiles: output-stationary grouped GEMM. One workgroup owns one 64 x 64 tile of an
+// output block and accumulates every product of its group into registers before a
+// single store, so no atomics are needed and the summation order is fixed.
+// Operands: op(A) is m x k, op(B) is k x n, both column-major with leading dimensions
+// and optional transposition; "axpy" products add alpha * X[tile].
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "chm_internal.hpp"
+
+namespace chm {
+
+namespace {
+
+constexpr int TM = 64, TN = 64, TK = 16, NT = 256;
+
+__device__ inline const double* rel(const double* p, int is_rel, const double* base) {
+  return is_rel ? base + (intptr_t)p : p;
+}
+
+__global__ __launch_bounds__(NT) void k_tiles(const DevTile* __restrict__ tiles, const DevGemmOp* __restrict__ ops,
+                                              const double* __restrict__ xbase, double* __restrict__ ybase) {
+  __shared__ double As[TK][TM + 1];
+  __shared__ double Bs[TK][TN + 1];
+  const DevTile t = tiles[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+  const int mrem = min(TM, t.m - t.m0), nrem = min(TN, t.n - t.n0);
+
+  for (int o = 0; o < t.nop; o++) {
+    const DevGemmOp op = ops[t.op0 + o];
+    const double* A = rel(op.A, op.flags & 1, xbase);
+    if (op.kind == 1) {
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int r = tx + 16 * i, c = ty + 16 * j;
+          if (r < mrem && c < nrem) acc[i][j] += op.alpha * A[(size_t)(t.m0 + r) + (size_t)op.lda * (t.n0 + c)];
+        }
+      continue;
+    }
+    const double* B = rel(op.B, (op.flags >> 1) & 1, xbase);
+    for (int k0 = 0; k0 < op.k; k0 += TK) {
+      const int krem = min(TK, op.k - k0);
+      // stage alpha * op(A)[m0:m0+64, k0:k0+16] as As[kk][r]
+      if (op.ta == 0) {
+        const int r = tid & 63;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int kk = (tid >> 6) + 4 * q;
+          double v = 0.0;
+          if (r < mrem && kk < krem) v = A[(size_t)(t.m0 + r) + (size_t)op.lda * (k0 + kk)];
+          As[kk][r] = op.alpha * v;
+        }
+      } else {
+        const int kk = tid & 15;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int r = (tid >> 4) + 16 * q;
+          double v = 0.0;
+          if (r < mrem && kk < krem) v = A[(size_t)(k0 + kk) + (size_t)op.lda * (t.m0 + r)];
+          As[kk][r] = op.alpha * v;
+        }
+      }
+      // stage op(B)[k0:k0+16, n0:n0+64] as Bs[kk][c]
+      if (op.tb == 0) {
+        const int kk = tid & 15;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int c = (tid >> 4) + 16 * q;
+          double v = 0.0;
+          if (c < nrem && kk < krem) v = B[(size_t)(k0 + kk) + (size_t)op.ldb * (t.n0 + c)];
+          Bs[kk][c] = v;
+        }
+      } else {
+        const int c = tid & 63;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int kk = (tid >> 6) + 4 * q;
+          double v = 0.0;
+          if (c < nrem && kk < krem) v = B[(size_t)(t.n0 + c) + (size_t)op.ldb * (k0 + kk)];
+          Bs[kk][c] = v;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < TK; kk++) {
+        double a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+        for (int j = 0; j < 4; j++) b[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+      }
+      __syncthreads();
+    }
+  }
+  double* C = (t.flags & 1) ? ybase + (intptr_t)t.C : t.C;
+  const bool add = (t.flags & 2) != 0;
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int r = tx + 16 * i, c = ty + 16 * j;
+      if (r < mrem && c < nrem) {
+        double* p = C + (size_t)(t.m0 + r) + (size_t)t.ldc * (t.n0 + c);
+        *p = add ? *p + acc[i][j] : acc[i][j];
+      }
+    }
+}
+
+// Assembles the explicit sub-block of every (kappa, kappa') pair:
+// M[(l + dL r), (l' + dLi r')] = sum_t alpha_t op(A_t)[l, l'] op(B_t)[r', r].
+__global__ __launch_bounds__(256) void k_assemble(const DevPair* __restrict__ pairs, const DevKTerm* __restrict__ terms,
+                                                  double* __restrict__ M) {
+  const DevPair p = pairs[blockIdx.x];
+  const int ncol = p.dLi * p.dRi;
+  const int n = p.R * ncol;
+  for (int e = threadIdx.x; e < n; e += 256) {
+    const int row = e % p.R, col = e / p.R;
+    const int l = row % p.dL, r = row / p.dL;
+    const int li = col % p.dLi, ri = col / p.dLi;
+    double v = 0.0;
+    for (int q = 0; q < p.nt; q++) {
+      const DevKTerm t = terms[p.t0 + q];
+      const double a = t.A ? (t.ta ? t.A[li + (size_t)t.lda * l] : t.A[l + (size_t)t.lda * li]) : (l == li ? 1.0 : 0.0);
+      const double b = t.B ? (t.tb ? t.B[r + (size_t)t.ldb * ri] : t.B[ri + (size_t)t.ldb * r]) : (r == ri ? 1.0 : 0.0);
+      v += t.alpha * a * b;
+    }
+    M[p.moff + e] = v;
+  }
+}
+
+// y_kappa = M_kappa x[col] for every output block: one workgroup per block; thread t
+// takes row t % R and every (256 / R)-th column, then the column lanes are summed.
+__global__ __launch_bounds__(256) void k_bspmv(const DevRowBlock* __restrict__ blocks, const double* __restrict__ M,
+                                               const int* __restrict__ col, const double* __restrict__ x,
+                                               double* __restrict__ y) {
+  __shared__ double part[256];
+  const DevRowBlock b = blocks[blockIdx.x];
+  for (int r0 = 0; r0 < b.R; r0 += 256) {
+    const int R = min(256, b.R - r0);
+    const int ncl = 256 / R;
+    const int t = threadIdx.x;
+    const int r = t % R, cl = t / R;
+    double acc = 0.0;
+    if (cl < ncl) {
+      const double* m = M + b.moff + r0 + r;
+      const int* ci = col + b.coff;
+      for (int c = cl; c < b.C; c += ncl) acc = fma(m[(size_t)b.R * c], x[ci[c]], acc);
+    }
+    part[t] = (cl < ncl) ? acc : 0.0;
+    __syncthreads();
+    if (t < R) {
+      double s = 0.0;
+      for (int q = 0; q < ncl; q++) s += part[t + R * q];
+      y[b.yoff + r0 + t] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// out = sum_q c_q src_q for every combination; one 1024-element chunk per workgroup.
+__global__ __launch_bounds__(256) void k_combos(const DevCombo* __restrict__ combos, const DevSrc* __restrict__ src,
+                                                const int64_t* __restrict__ chunk_start, int ncombo) {
+  const int64_t chunk = blockIdx.x;
+  // binary search for the combination owning this chunk
+  int lo = 0, hi = ncombo - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (chunk_start[mid] <= chunk) lo = mid; else hi = mid - 1;
+  }
+  const DevCombo c = combos[lo];
+  const int64_t base = (chunk - chunk_start[lo]) * 1024;
+  for (int e = threadIdx.x; e < 1024; e += 256) {
+    const int64_t idx = base + e;
+    if (idx >= c.size) break;
+    double v = 0.0;
+    for (int q = 0; q < c.nsrc; q++) v += src[c.src0 + q].c * src[c.src0 + q].p[idx];
+    c.out[idx] = v;
+  }
+}
+
+// diag(Heff) block by block: one workgroup per block, threads over its elements.
+__global__ __launch_bounds__(256) void k_diag(const DevDiagBlock* __restrict__ blocks, const DevDiag* __restrict__ terms) {
+  const DevDiagBlock b = blocks[blockIdx.x];
+  const int n = b.dL * b.dR;
+  for (int e = threadIdx.x; e < n; e += 256) {
+    const int l = e % b.dL, r = e / b.dL;
+    double v = 0.0;
+    for (int q = 0; q < b.nt; q++) {
+      const DevDiag t = terms[b.t0 + q];
+      if (t.kind == 0) v += t.coef;
+      else if (t.kind == 1) v += t.coef * t.L[(size_t)l * (b.dL + 1)];
+      else if (t.kind == 2) v += t.coef * t.R[(size_t)r * (b.dR + 1)];
+      else v += t.coef * t.L[(size_t)l * (b.dL + 1)] * t.R[(size_t)r * (b.dR + 1)];
+    }
+    b.out[e] = v;
+  }
+}
+
+// --------------------------------------------------------- vector kernels --
+__global__ __launch_bounds__(256) void k_dot_partial(const double* __restrict__ a, const double* __restrict__ b,
+                                                     int64_t n, double* __restrict__ partial) {
+  __shared__ double s[256];
+  double v = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) v += a[i] * b[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+// Sum nparts partial sums per output (one workgroup per output), fixed order.
+__global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ partial, int nparts, double* __restrict__ out) {
+  __shared__ double s[256];
+  const double* p = partial + (size_t)blockIdx.x * nparts;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) v += p[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+// partial dots of many vectors against one: grid (nparts, nv)
+__global__ __launch_bounds__(256) void k_multi_dot_partial(const double* const* __restrict__ vecs,
+                                                           const double* __restrict__ b, int64_t n,
+                                                           double* __restrict__ partial) {
+  __shared__ double s[256];
+  const double* a = vecs[blockIdx.y];
+  double v = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) v += a[i] * b[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = s[0];
+}
+
+// y += (sign * *coef) x, with the coefficient read from device memory
+__global__ __launch_bounds__(256) void k_axpy_dev(const double* __restrict__ coef, double sign,
+                                                  const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+  const double a = sign * coef[0];
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) y[i] += a * x[i];
+}
+
+__global__ __launch_bounds__(256) void k_axpy(double a, const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) y[i] += a * x[i];
+}
+
+__global__ __launch_bounds__(256) void k_scale(double a, double* __restrict__ x, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) x[i] *= a;
+}
+
+// out = sum_v coefs[v] * vecs[v]
+__global__ __launch_bounds__(256) void k_lincomb_vecs(const double* const* __restrict__ vecs,
+                                                      const double* __restrict__ coefs, int nv,
+                                                      double* __restrict__ out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double v = 0.0;
+    for (int q = 0; q < nv; q++) v += coefs[q] * vecs[q][i];
+    out[i] = v;
+  }
+}
+
+// Davidson correction (Davidson.cpp:322-350). stage 0: work = u / K; stage 1:
+// t = -(t + alpha u) / K, with K = diag - shift clamped to +-cutoff.
+__global__ __launch_bounds__(256) void k_precond(const double* __restrict__ diag, const double* __restrict__ u,
+                                                 double shift, double cutoff, double* __restrict__ work,
+                                                 double* __restrict__ t, int64_t n, int stage, double alpha) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double d = diag[i] - shift;
+    const double k = (fabs(d) > cutoff) ? d : cutoff;
+    if (stage == 0) work[i] = u[i] / k;
+    else t[i] = -(t[i] + alpha * u[i]) / k;
+  }
+}
+
+// Scale block kappa by scale[kappa] (program <-> symmetric convention, Sobject.cpp:624-650).
+__global__ __launch_bounds__(256) void k_symm_scale(double* __restrict__ x, const int64_t* __restrict__ k2i,
+                                                    const double* __restrict__ scale, int nk) {
+  const int k = blockIdx.x;
+  if (k >= nk) return;
+  const double s = scale[k];
+  for (int64_t i = k2i[k] + threadIdx.x; i < k2i[k + 1]; i += 256) x[i] *= s;
+}
+
+// Deterministic pseudo-random fill in [-scale, scale) (splitmix64 of the index).
+__global__ __launch_bounds__(256) void k_fill_hash(double* __restrict__ p, int64_t n, uint64_t seed, double scale) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    uint64_t z = seed * 0x9E3779B97F4A7C15ULL + (uint64_t)i;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    p[i] = scale * (2.0 * ((z >> 11) * (1.0 / 9007199254740992.0)) - 1.0);
+  }
+}
+
+inline unsigned grid_for(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+}  // namespace
+
+void launch_tiles(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const double* xbase, double* ybase,
+                  hipStream_t s) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_tiles, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase);
+}
+
+void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs, double* M, hipStream_t s) {
+  for (int64_t p0 = 0; p0 < npairs; p0 += (1LL << 30)) {
+    const int64_t n = std::min<int64_t>(npairs - p0, 1LL << 30);
+    hipLaunchKernelGGL(k_assemble, dim3((unsigned)n), dim3(256), 0, s, pairs + p0, terms, M);
+  }
+}
+
+void launch_bspmv(const DevRowBlock* blocks, int nblocks, const double* M, const int* col, const double* x, double* y,
+                  hipStream_t s) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(k_bspmv, dim3(nblocks), dim3(256), 0, s, blocks, M, col, x, y);
+}
+
+void launch_combos(const DevCombo* d_combos, const DevSrc* d_src, int ncombo, const int64_t* d_chunk_start,
+                   int64_t nchunks, hipStream_t s) {
+  if (ncombo <= 0 || nchunks <= 0) return;
+  hipLaunchKernelGGL(k_combos, dim3((unsigned)nchunks), dim3(256), 0, s, d_combos, d_src, d_chunk_start, ncombo);
+}
+
+void launch_diag(const DevDiagBlock* blocks, const DevDiag* terms, int nblocks, hipStream_t s) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(k_diag, dim3(nblocks), dim3(256), 0, s, blocks, terms);
+}
+
+int dot_parts(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g > 1024) g = 1024;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+void launch_dot(const double* a, const double* b, int64_t n, double* partial, double* out, hipStream_t s) {
+  const int parts = dot_parts(n);
+  hipLaunchKernelGGL(k_dot_partial, dim3(parts), dim3(256), 0, s, a, b, n, partial);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, partial, parts, out);
+}
+
+void launch_multi_dot(const double* const* d_vecs, int nv, const double* b, int64_t n, double* partial, double* out,
+                      hipStream_t s) {
+  if (nv <= 0) return;
+  const int parts = dot_parts(n);
+  hipLaunchKernelGGL(k_multi_dot_partial, dim3(parts, nv), dim3(256), 0, s, d_vecs, b, n, partial);
+  hipLaunchKernelGGL(k_reduce, dim3(nv), dim3(256), 0, s, partial, parts, out);
+}
+
+void launch_axpy_dev(const double* coef, double sign, const double* x, double* y, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy_dev, dim3(grid_for(n)), dim3(256), 0, s, coef, sign, x, y, n);
+}
+
+void launch_axpy(double a, const double* x, double* y, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(256), 0, s, a, x, y, n);
+}
+
+void launch_scale(double a, double* x, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(256), 0, s, a, x, n);
+}
+
+void launch_lincomb_vecs(const double* const* d_vecs, const double* d_coefs, int nv, double* out, int64_t n,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_lincomb_vecs, dim3(grid_for(n)), dim3(256), 0, s, d_vecs, d_coefs, nv, out, n);
+}
+
+void launch_precond(const double* diag, const double* u, double shift, double cutoff, double* work, double* t,
+                    int64_t n, int stage, double alpha, hipStream_t s) {
+  hipLaunchKernelGGL(k_precond, dim3(grid_for(n)), dim3(256), 0, s, diag, u, shift, cutoff, work, t, n, stage, alpha);
+}
+
+void launch_fill_hash(double* p, int64_t n, uint64_t seed, double scale, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_hash, dim3(grid_for(n)), dim3(256), 0, s, p, n, seed, scale);
+}
+
+void launch_symm_scale(double* x, const int64_t* d_k2i, const double* d_scale, int nk, hipStream_t s) {
+  if (nk <= 0) return;
+  hipLaunchKernelGGL(k_symm_scale, dim3(nk), dim3(256), 0, s, x, d_k2i, d_scale, nk);
+}
+
+}  // namespace chm

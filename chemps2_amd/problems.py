@@ -68,6 +68,18 @@ def random_abinitio(L, seed, group=7):
     return irreps, h, V
 
 
+def mx_table(T, V, N):
+    """The L^4 table h_{ab;cd} = V_abcd + (T_ac delta_bd + delta_ac T_bd) / (N - 1) that
+    Problem::construct_mxelem builds (Problem.cpp:363-384), flattened as
+    mx[a + L(b + L(c + L d))]."""
+    T = np.asarray(T, dtype=np.float64)
+    L = T.shape[0]
+    V = np.asarray(V, dtype=np.float64).reshape(L, L, L, L)
+    eye = np.eye(L)
+    mx = V + (np.einsum("ac,bd->abcd", T, eye) + np.einsum("ac,bd->abcd", eye, T)) / (N - 1)
+    return np.ascontiguousarray(mx.transpose(3, 2, 1, 0)).ravel()
+
+
 def read_fcidump(path):
     """Reads an FCIDUMP file (Hamiltonian.cpp:296-423). Returns a dict with L, N, TwoS,
     irreps (XOR labels, D2h subgroup given by ``group``), T, V (physicist), econst."""

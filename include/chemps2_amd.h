@@ -1,0 +1,120 @@
+/*
+ * chemps2_amd: MI355X (gfx950) implementation of CheMPS2's two-site DMRG
+ * effective-Hamiltonian path, behind a C ABI.
+ *
+ * The library replaces, for one two-site step at sites (i, i+1), the work the
+ * reference does in Heff::SolveDAVIDSON (CheMPS2/Heff.cpp:317-386): the matrix-vector
+ * product Heff::makeHeff (Heff.cpp:43-248 with HeffDiagrams1-5.cpp), the diagonal
+ * Heff::fillHeffDiag (Heff.cpp:250-315, HeffDiagonal.cpp) and the Davidson solver
+ * (Davidson.cpp:31-536). The caller keeps the reference's bookkeeping and operator
+ * construction and hands over, per step, the data that Heff reads: the L^4 integral
+ * table, the sector dimensions and the renormalised operator tensors.
+ *
+ * Conventions
+ *  - All entry points return 0 on success and a negative CHM_ERR_* code otherwise;
+ *    chm_last_error() describes the last failure. Nothing throws across the ABI.
+ *  - Dense blocks are column-major FP64, in exactly the storage order of the
+ *    reference objects they replace (TensorOperator.cpp:29-102, Sobject.cpp:36-147).
+ *  - Pointers are host pointers unless an on_device flag says otherwise; device
+ *    pointers must be valid on the context's device.
+ *  - A context is used from one host thread at a time.
+ */
+#ifndef CHEMPS2_AMD_H
+#define CHEMPS2_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct chm_ctx chm_ctx;
+
+#define CHM_OK 0
+#define CHM_ERR_ARG (-1)
+#define CHM_ERR_STATE (-2)
+#define CHM_ERR_MISSING (-3)
+#define CHM_ERR_SIZE (-4)
+#define CHM_ERR_HIP (-5)
+#define CHM_ERR_NUMERIC (-6)
+
+/* Operator kinds, the arrays DMRG keeps per boundary (CheMPS2/include/chemps2/DMRG.h:
+ * Ltensors, F0tensors, F1tensors, S0tensors, S1tensors, Atensors, Btensors, Ctensors,
+ * Dtensors, Qtensors, Xtensors; allocation in DMRGoperators.cpp:909-1145). */
+enum chm_kind {
+  CHM_L = 0, CHM_F0 = 1, CHM_F1 = 2, CHM_S0 = 3, CHM_S1 = 4,
+  CHM_A = 5, CHM_B = 6, CHM_C = 7, CHM_D = 8, CHM_Q = 9, CHM_X = 10
+};
+
+/* Create / destroy a context bound to HIP device `device` with its own stream. */
+int chm_ctx_create(int device, chm_ctx** out);
+int chm_ctx_destroy(chm_ctx* ctx);
+const char* chm_last_error(const chm_ctx* ctx);
+
+/* The problem in DMRG orbital order: replaces Problem (Problem.h:44: L, N, 2S, target
+ * irrep, orbital irreps via gIrrep) and Problem::gMxElement (Problem.cpp:351-384).
+ * mx: the L^4 table h_{ab;cd} at mx[a + L(b + L(c + L d))]. group: 0..7 as Irreps.h. */
+int chm_set_problem(chm_ctx* ctx, int L, int group, const int* orb_irreps, int N, int two_s, int irrep,
+                    const double* mx, double econst);
+
+/* Current virtual dimensions of boundary b (0..L): replaces SyBookkeeper::gCurrentDim
+ * (SyBookkeeper.cpp:165). Sectors not listed have dimension 0. Replaces any previous
+ * list for b. */
+int chm_set_sectors(chm_ctx* ctx, int boundary, int n, const int* N, const int* two_s, const int* irrep,
+                    const int* dim);
+
+/* Dimensions of every boundary derived as SyBookkeeper(Prob, D) does (FCI dimensions,
+ * capped and scaled to D; SyBookkeeper.cpp:29-269). */
+int chm_set_sectors_auto(chm_ctx* ctx, int D);
+int chm_get_sectors(const chm_ctx* ctx, int boundary, int* n, int* N, int* two_s, int* irrep, int* dim);
+
+/* One operator tensor of slot cnt (boundary cnt+1), built moving right (1) or left
+ * (0): the storage of TensorOperator::gStorage() (TensorOperator.cpp:29-102) for
+ * DMRG::<kind>tensors[cnt][k][m] (k, m ignored where the array has fewer indices).
+ * two_j, n_elec, n_irrep as TensorOperator::get_2j/get_nelec/get_irrep. n must equal
+ * the tensor's size for the current sectors. data is a host pointer. */
+int chm_tensor_upload(chm_ctx* ctx, int cnt, int moving_right, int kind, int k, int m, int two_j, int n_elec,
+                      int n_irrep, const double* data, int64_t n);
+/* Allocates every operator of slot cnt with the reference's shapes
+ * (DMRGoperators.cpp:909-1145) and fills them with deterministic pseudo-random values
+ * (benchmark inputs). */
+int chm_tensor_fill_synthetic(chm_ctx* ctx, int cnt, int moving_right, uint64_t seed);
+/* Drops the operators of slot cnt (DMRG::deleteTensors). */
+int chm_tensor_clear(chm_ctx* ctx, int cnt);
+
+/* Two-site step at (site, site+1): builds the work lists of Heff (the reference's
+ * Heff(denBK, Prob, rtol) + Sobject(site, denBK), Heff.h:50, Sobject.cpp:36-147).
+ * Needs slot site-1 built moving right (site > 0) and slot site+1 moving left
+ * (site < L-2). */
+int chm_heff_prepare(chm_ctx* ctx, int site);
+/* Davidson vector length V = Sobject::gKappa2index(gNKappa()). */
+int chm_heff_veclen(const chm_ctx* ctx, int64_t* V);
+/* y = Heff x in the symmetric convention (Heff::makeHeff, Heff.cpp:43-248). */
+int chm_heff_apply(chm_ctx* ctx, const double* x, double* y, int on_device);
+/* d = diag(Heff) (Heff::fillHeffDiag, Heff.cpp:250-315; nLower = 0). */
+int chm_heff_diag(chm_ctx* ctx, double* d, int on_device);
+/* Lowest eigenpair of Heff from the guess x (program convention, as Sobject storage
+ * before Heff::SolveDAVIDSON; Heff.cpp:330-386 with Davidson(V, 32, 3, rtol, 1e-12)).
+ * On return x holds the eigenvector in the program convention. */
+int chm_davidson(chm_ctx* ctx, double* x, int on_device, double rtol, double* eigenvalue, int* n_matvec);
+
+/* F_H: 2mnk summed over every dgemm_ the reference's makeHeff issues at this site;
+ * B_H: 8 * (2V + sum of the sizes of the operator tensors it reads), in bytes. */
+int chm_flops(const chm_ctx* ctx, double* F_H, double* B_H);
+/* stats[16]: terms, combinations, combination doubles, W doubles, chain phase-1 tiles,
+ * chain phase-2 tiles, explicit block pairs, chain terms, blocks, V, diagonal terms,
+ * explicit matrix doubles, explicit column indices, explicit size threshold, 0, 0. */
+int chm_plan_stats(const chm_ctx* ctx, int64_t* stats);
+/* The context's HIP stream (hipStream_t) for callers that time or order work on it. */
+int chm_stream(const chm_ctx* ctx, void** stream);
+/* Timing of chm_heff_apply with HIP events on the context stream, accumulated per
+ * kernel: ms[0] chained-GEMM phase 1, ms[1] explicit block product, ms[2] chained-GEMM
+ * phase 2; count = number of applications timed. */
+int chm_timing_enable(chm_ctx* ctx, int enable);
+int chm_timing_read(const chm_ctx* ctx, double* ms, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHEMPS2_AMD_H */

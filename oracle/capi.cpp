@@ -100,6 +100,76 @@ double orc_step_left(void* p, int index, int D, double rtol, int change) {
   return E;
 }
 
+// splitmix64-based fill in [-scale, scale), element i of a tensor filled with `seed`
+// (the benchmark's synthetic operator values).
+static double hash_value(uint64_t seed, uint64_t i, double scale) {
+  uint64_t z = seed * 0x9E3779B97F4A7C15ULL + i;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  return scale * (2.0 * ((z >> 11) * (1.0 / 9007199254740992.0)) - 1.0);
+}
+
+// Benchmark workload: bookkeeper at D, the operator slots around `site` allocated with
+// the reference's shapes and filled, tensor by tensor in allocation order, with
+// seeds seed_l*1000003+17+j (slot site-1) and seed_r*1000003+17+j (slot site+1).
+long orc_synthetic(void* p, int D, int site, unsigned long long seed_l, unsigned long long seed_r) {
+  Handle* h = (Handle*)p;
+  DMRG& dm = h->dm;
+  dm.P.build_mx();
+  dm.bk.init(&dm.P, D);
+  dm.ops.init(&dm.P, &dm.bk);
+  auto fill_set = [&](int cnt, bool mr, unsigned long long seed) {
+    dm.ops.allocate(cnt, mr);
+    OpSet& o = dm.ops.ops[cnt];
+    uint64_t s = seed * 1000003ULL + 17;
+    auto fill = [&](TOp* t) {
+      if (!t) return;
+      const double sc = t->data.empty() ? 0.0 : 1.0 / std::sqrt((double)t->data.size());
+      for (size_t i = 0; i < t->data.size(); i++) t->data[i] = hash_value(s, i, sc);
+      s++;
+    };
+    for (auto& t : o.L) fill(t.get());
+    for (size_t k = 0; k < o.F0.size(); k++)
+      for (size_t m = 0; m < o.F0[k].size(); m++) {
+        fill(o.F0[k][m].get()); fill(o.F1[k][m].get()); fill(o.S0[k][m].get()); fill(o.S1[k][m].get());
+      }
+    for (size_t k = 0; k < o.A.size(); k++)
+      for (size_t m = 0; m < o.A[k].size(); m++) {
+        fill(o.A[k][m].get()); fill(o.B[k][m].get()); fill(o.C[k][m].get()); fill(o.D[k][m].get());
+      }
+    for (auto& t : o.Q) fill(t.get());
+    fill(o.X.get());
+  };
+  if (site > 0) fill_set(site - 1, true, seed_l);
+  if (site < dm.P.L - 2) fill_set(site + 1, false, seed_r);
+  h->S = std::make_unique<Sobject>();
+  h->S->init(site, &dm.bk);
+  h->H = std::make_unique<Heff>();
+  h->H->init(&dm.P, &dm.bk, h->S.get(), &dm.ops);
+  h->site = site;
+  return (long)h->S->size();
+}
+
+// Only the bookkeeper at D (sector dimensions), for layout comparisons.
+void orc_bookkeeper(void* p, int D) {
+  DMRG& dm = ((Handle*)p)->dm;
+  dm.P.build_mx();
+  dm.bk.init(&dm.P, D);
+}
+
+// Heff x restricted to output blocks [k0, k1); returns the GEMM flops of the sample
+// and writes the wall time to *seconds.
+double orc_site_apply_range(void* p, const double* x, double* y, int k0, int k1, double* seconds) {
+  Handle* h = (Handle*)p;
+  h->H->flops = 0.0;
+  auto t0 = std::chrono::steady_clock::now();
+  h->H->apply_range(x, y, k0, k1);
+  auto t1 = std::chrono::steady_clock::now();
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return h->H->flops;
+}
+
 // Builds the two-site object at `index` (joined from the current MPS) and its Heff.
 long orc_site(void* p, int index) {
   Handle* h = (Handle*)p;

@@ -50,10 +50,12 @@ void Heff::term(int ik, int kin, double alpha, char tl, const double* Lb, char t
   }
 }
 
-void Heff::apply(const double* x_, double* y_) {  // Heff.cpp:43-248
+void Heff::apply(const double* x_, double* y_) { apply_range(x_, y_, 0, S->nk()); }
+
+void Heff::apply_range(const double* x_, double* y_, int k0, int k1) {  // Heff.cpp:43-248
   x = x_;
   y = y_;
-  for (int ik = 0; ik < S->nk(); ik++) {
+  for (int ik = k0; ik < k1; ik++) {
     std::fill(y + S->k2i[ik], y + S->k2i[ik + 1], 0.0);
     d1(ik);
     d2loc(ik);

@@ -20,6 +20,8 @@ struct Heff {
 
   void init(const Problem* P, const Bookkeeper* bk, Sobject* S, Operators* ops);
   void apply(const double* x, double* y);  // makeHeff
+  // Output blocks k0 <= kappa < k1 only (a bounded sample of one application).
+  void apply_range(const double* x, double* y, int k0, int k1);
   void diag(double* d);                    // fillHeffDiag, Heff.cpp:250-315
 
   // --- internals -------------------------------------------------------

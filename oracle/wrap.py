@@ -41,6 +41,9 @@ def lib():
             "orc_matvec_flops": (d, [vp]),
             "orc_sweep_left_to": (d, [vp, i, i, d, i]),
             "orc_step_left": (d, [vp, i, i, d, i]),
+            "orc_synthetic": (L_, [vp, i, i, C.c_ulonglong, C.c_ulonglong]),
+            "orc_bookkeeper": (None, [vp, i]),
+            "orc_site_apply_range": (d, [vp, _dp, _dp, i, i, C.POINTER(C.c_double)]),
             "orc_site": (L_, [vp, i]),
             "orc_site_x": (None, [vp, _dp]),
             "orc_site_apply": (d, [vp, _dp, _dp]),
@@ -81,6 +84,7 @@ class OracleDMRG:
         Vf = np.ascontiguousarray(V.transpose(3, 2, 1, 0)).ravel()
         self.h = self.lib.orc_create(L, group, irreps, N, TwoS, Irrep, Tf, Vf, float(econst), int(reorder))
         self.L = L
+        self.group, self.N, self.TwoS, self.Irrep = group, N, TwoS, Irrep
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -102,6 +106,23 @@ class OracleDMRG:
 
     def sweep_left_to(self, stop, D, rtol=1e-5, change=True):
         return self.lib.orc_sweep_left_to(self.h, int(stop), int(D), float(rtol), int(change))
+
+    def synthetic(self, D, site, seed_l, seed_r):
+        """Benchmark workload: dimensions at D and pseudo-random operators around site."""
+        self.index = site
+        self.V = int(self.lib.orc_synthetic(self.h, int(D), int(site), int(seed_l), int(seed_r)))
+        return self.V
+
+    def bookkeeper(self, D):
+        self.lib.orc_bookkeeper(self.h, int(D))
+
+    def apply_range(self, x, k0, k1):
+        """Heff x on output blocks [k0, k1): (y, flops, seconds)."""
+        y = np.zeros(self.V)
+        sec = C.c_double(0.0)
+        fl = self.lib.orc_site_apply_range(self.h, np.ascontiguousarray(x, dtype=np.float64), y, int(k0), int(k1),
+                                           C.byref(sec))
+        return y, fl, sec.value
 
     def step_left(self, index, D, rtol=1e-5, change=True):
         return self.lib.orc_step_left(self.h, int(index), int(D), float(rtol), int(change))
