@@ -173,7 +173,7 @@ class Context:
         s = (C.c_int64 * 16)()
         self._check(self.lib.chm_plan_stats(self.h, s), "chm_plan_stats")
         keys = ["terms", "combos", "combo_doubles", "W_doubles", "tiles1", "tiles2", "explicit_pairs", "chain_terms",
-                "blocks", "V", "diag_terms", "explicit_doubles", "explicit_cols", "explicit_threshold", "_14", "_15"]
+                "blocks", "V", "diag_terms", "explicit_doubles", "explicit_cols", "explicit_threshold", "explicit_items", "_15"]
         return dict(zip(keys, list(s)))
 
     def stream(self):

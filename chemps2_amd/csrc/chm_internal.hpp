@@ -152,6 +152,21 @@ struct DevRowBlock {
   int64_t yoff;  // k2i[kappa]
   int R, C;
 };
+// Work item of the explicit product: rows [r0, r0 + R) of block kappa against
+// columns [c0, c1) of its matrix; the partial result goes to y (single item per
+// block) or to P[poff + row] for the reduction pass.
+struct DevBItem {
+  int64_t moff;  // element (r0, 0) of the block matrix (leading dimension ld)
+  int64_t coff;  // column indices of the block
+  int64_t out;   // y offset (direct) or partial offset
+  int ld, R, c0, c1;
+  int direct;
+  int pad;
+};
+struct DevBRed {  // y[yoff + r] = sum_q P[p0 + q * R + r], q < nparts
+  int64_t yoff, p0;
+  int R, nparts;
+};
 // One (kappa, kappa') pair: its R x |kappa'| sub-block is sum_t alpha_t op(A_t) (x) op(B_t).
 struct DevPair {
   int64_t moff;  // first element of the sub-block (leading dimension R)
@@ -194,8 +209,8 @@ void launch_combos(const DevCombo* d_combos, const DevSrc* d_src, int ncombo, co
                    int64_t nchunks, hipStream_t s);
 void launch_diag(const DevDiagBlock* blocks, const DevDiag* terms, int nblocks, hipStream_t s);
 void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs, double* M, hipStream_t s);
-void launch_bspmv(const DevRowBlock* blocks, int nblocks, const double* M, const int* col, const double* x, double* y,
-                  hipStream_t s);
+void launch_bspmv(const DevBItem* items, int nitems, const DevBRed* reds, int nreds, double* P, const double* M,
+                  const int* col, const double* x, double* y, hipStream_t s);
 int dot_parts(int64_t n);
 void launch_dot(const double* a, const double* b, int64_t n, double* partial, double* out, hipStream_t s);
 void launch_multi_dot(const double* const* d_vecs, int nv, const double* b, int64_t n, double* partial, double* out,

@@ -260,8 +260,40 @@ void Ctx::prepare(int st) {
     if ((int64_t)kt.size() > INT32_MAX) throw Error{ERR_SIZE, "too many explicit terms"};
     Mvals.alloc(std::max<int64_t>(moff, 1));
     Mcol.upload(colidx.empty() ? std::vector<int>{0} : colidx, stream);
-    rowblocks.upload(rb, stream);
-    nrowblocks = S.nk();
+    // work items: rows in chunks of at most 256, columns in chunks of about
+    // item_elems elements, so every workgroup streams a similar share of the matrix
+    std::vector<DevBItem> items;
+    std::vector<DevBRed> reds;
+    int64_t poff = 0;
+    for (int k = 0; k < S.nk(); k++) {
+      const DevRowBlock& B = rb[k];
+      for (int r0 = 0; r0 < B.R; r0 += 256) {
+        const int R = std::min(256, B.R - r0);
+        const int64_t cols_per = std::min<int64_t>(4096, std::max<int64_t>(1, item_elems / R));  // LDS staging limit
+        const int nparts = B.C == 0 ? 1 : (int)((B.C + cols_per - 1) / cols_per);
+        for (int q = 0; q < nparts; q++) {
+          DevBItem it{};
+          it.moff = B.moff + r0;
+          it.coff = B.coff;
+          it.ld = B.R;
+          it.R = R;
+          it.c0 = (int)std::min<int64_t>(B.C, q * cols_per);
+          it.c1 = (int)std::min<int64_t>(B.C, (q + 1) * cols_per);
+          it.direct = nparts == 1;
+          it.out = it.direct ? B.yoff + r0 : poff + (int64_t)q * R;
+          items.push_back(it);
+        }
+        if (nparts > 1) {
+          reds.push_back({B.yoff + r0, poff, R, nparts});
+          poff += (int64_t)nparts * R;
+        }
+      }
+    }
+    bitems.upload(items, stream);
+    breds.upload(reds, stream);
+    nbitems = (int)items.size();
+    nbreds = (int)reds.size();
+    bpart.alloc(std::max<int64_t>(poff, 1));
     if (!dp.empty()) {
       DBuf<DevPair> d_dp;
       DBuf<DevKTerm> d_kt;
@@ -275,6 +307,7 @@ void Ctx::prepare(int st) {
     stats[11] = moff;
     stats[12] = (int64_t)colidx.size();
     stats[13] = thr;
+    stats[14] = (int64_t)nbitems;
   }
 
   // ---- chained GEMM tiles for the remaining terms
@@ -435,7 +468,7 @@ void Ctx::apply(const double* xp, double* yp) {
   if (timing) hip_check(hipEventRecord(ev[0], stream), "event");
   launch_tiles(tiles1.p, ops1.p, ntiles1, xp, yp, stream);
   if (timing) hip_check(hipEventRecord(ev[1], stream), "event");
-  launch_bspmv(rowblocks.p, nrowblocks, Mvals.p, Mcol.p, xp, yp, stream);
+  launch_bspmv(bitems.p, nbitems, breds.p, nbreds, bpart.p, Mvals.p, Mcol.p, xp, yp, stream);
   if (timing) hip_check(hipEventRecord(ev[2], stream), "event");
   launch_tiles(tiles2.p, ops2.p, ntiles2, xp, yp, stream);
   if (timing) hip_check(hipEventRecord(ev[3], stream), "event");
@@ -696,6 +729,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   ctx->c.device = device;
   if (const char* e = std::getenv("CHM_EXPLICIT_MAX")) ctx->c.explicit_max = std::atoll(e);
   if (const char* e = std::getenv("CHM_EXPLICIT_BUDGET_GB")) ctx->c.explicit_budget_gb = std::atof(e);
+  if (const char* e = std::getenv("CHM_ITEM_ELEMS")) ctx->c.item_elems = std::max(256LL, std::atoll(e));
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return CHM_ERR_HIP;
@@ -773,7 +807,8 @@ int chm_tensor_fill_synthetic(chm_ctx* ctx, int cnt, int moving_right, uint64_t 
     uint64_t s = seed * 1000003ULL + 17;
     for (const chm::OpShape& sh : chm::opset_shapes(c, cnt, moving_right != 0)) {
       chm::OpT* t = c.add_tensor(cnt, moving_right != 0, sh.kind, sh.k, sh.m, sh.two_j, sh.n_elec, sh.n_irrep, -1);
-      if (t->size > 0) chm::launch_fill_hash(c.bufs[t->buf], t->size, s++, 1.0 / std::sqrt((double)t->size), c.stream);
+      const uint64_t ts = s++;  // one seed per tensor in allocation order, empty ones included
+      if (t->size > 0) chm::launch_fill_hash(c.bufs[t->buf], t->size, ts, 1.0 / std::sqrt((double)t->size), c.stream);
     }
     chm::hip_check(hipGetLastError(), "fill kernel");
     chm::hip_check(hipStreamSynchronize(c.stream), "fill");

@@ -94,8 +94,11 @@ struct Ctx : HostState {
   int64_t explicit_threshold = 0;
   DBuf<double> Mvals;
   DBuf<int> Mcol;
-  DBuf<DevRowBlock> rowblocks;
-  int nrowblocks = 0;
+  int64_t item_elems = 32768;        // CHM_ITEM_ELEMS: matrix elements per work item
+  DBuf<DevBItem> bitems;
+  DBuf<DevBRed> breds;
+  DBuf<double> bpart;
+  int nbitems = 0, nbreds = 0;
 
   // optional timing of the three kernels of one application (events on the library stream)
   bool timing = false;

@@ -143,32 +143,59 @@ __global__ __launch_bounds__(256) void k_assemble(const DevPair* __restrict__ pa
   }
 }
 
-// y_kappa = M_kappa x[col] for every output block: one workgroup per block; thread t
-// takes row t % R and every (256 / R)-th column, then the column lanes are summed.
-__global__ __launch_bounds__(256) void k_bspmv(const DevRowBlock* __restrict__ blocks, const double* __restrict__ M,
+// Explicit block product, pass 1: one workgroup per work item (at most BX_COLS
+// columns). The item's x values are gathered into LDS first; then thread t takes row
+// t % R and every ncl-th column (ncl = 256 / R column lanes) with eight matrix loads in
+// flight, streamed non-temporally since every element is used once per product. The
+// column lanes are summed through LDS in a fixed order.
+constexpr int BX_COLS = 4096;
+__global__ __launch_bounds__(256) void k_bspmv(const DevBItem* __restrict__ items, const double* __restrict__ M,
                                                const int* __restrict__ col, const double* __restrict__ x,
-                                               double* __restrict__ y) {
+                                               double* __restrict__ y, double* __restrict__ P) {
+  __shared__ double xs[BX_COLS];
   __shared__ double part[256];
-  const DevRowBlock b = blocks[blockIdx.x];
-  for (int r0 = 0; r0 < b.R; r0 += 256) {
-    const int R = min(256, b.R - r0);
-    const int ncl = 256 / R;
-    const int t = threadIdx.x;
-    const int r = t % R, cl = t / R;
-    double acc = 0.0;
-    if (cl < ncl) {
-      const double* m = M + b.moff + r0 + r;
-      const int* ci = col + b.coff;
-      for (int c = cl; c < b.C; c += ncl) acc = fma(m[(size_t)b.R * c], x[ci[c]], acc);
+  const DevBItem it = items[blockIdx.x];
+  const int R = it.R, ncl = 256 / R;
+  const int t = threadIdx.x;
+  const int nc = it.c1 - it.c0;
+  const int* ci = col + it.coff + it.c0;
+  for (int c = t; c < nc; c += 256) xs[c] = x[ci[c]];
+  __syncthreads();
+  const int r = t % R, cl = t / R;
+  double a[8];
+#pragma unroll
+  for (int u = 0; u < 8; u++) a[u] = 0.0;
+  if (cl < ncl) {
+    const double* m = M + it.moff + r + (size_t)it.ld * it.c0;
+    const size_t ld = (size_t)it.ld;
+    int c = cl;
+    for (; c + 7 * ncl < nc; c += 8 * ncl) {
+      double mv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) mv[u] = __builtin_nontemporal_load(m + ld * (c + u * ncl));
+#pragma unroll
+      for (int u = 0; u < 8; u++) a[u] = fma(mv[u], xs[c + u * ncl], a[u]);
     }
-    part[t] = (cl < ncl) ? acc : 0.0;
-    __syncthreads();
-    if (t < R) {
-      double s = 0.0;
-      for (int q = 0; q < ncl; q++) s += part[t + R * q];
-      y[b.yoff + r0 + t] = s;
-    }
-    __syncthreads();
+    for (; c < nc; c += ncl) a[0] = fma(__builtin_nontemporal_load(m + ld * c), xs[c], a[0]);
+  }
+  part[t] = (cl < ncl) ? ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7])) : 0.0;
+  __syncthreads();
+  if (t < R) {
+    double s = 0.0;
+    for (int q = 0; q < ncl; q++) s += part[t + R * q];
+    if (it.direct) y[it.out + t] = s;
+    else P[it.out + t] = s;
+  }
+}
+
+// Explicit block product, pass 2: sum the partial results of split blocks in order.
+__global__ __launch_bounds__(256) void k_bspmv_reduce(const DevBRed* __restrict__ reds, const double* __restrict__ P,
+                                                      double* __restrict__ y) {
+  const DevBRed d = reds[blockIdx.x];
+  for (int r = threadIdx.x; r < d.R; r += 256) {
+    double s = 0.0;
+    for (int q = 0; q < d.nparts; q++) s += P[d.p0 + (int64_t)q * d.R + r];
+    y[d.yoff + r] = s;
   }
 }
 
@@ -339,10 +366,10 @@ void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs
   }
 }
 
-void launch_bspmv(const DevRowBlock* blocks, int nblocks, const double* M, const int* col, const double* x, double* y,
-                  hipStream_t s) {
-  if (nblocks <= 0) return;
-  hipLaunchKernelGGL(k_bspmv, dim3(nblocks), dim3(256), 0, s, blocks, M, col, x, y);
+void launch_bspmv(const DevBItem* items, int nitems, const DevBRed* reds, int nreds, double* P, const double* M,
+                  const int* col, const double* x, double* y, hipStream_t s) {
+  if (nitems > 0) hipLaunchKernelGGL(k_bspmv, dim3(nitems), dim3(256), 0, s, items, M, col, x, y, P);
+  if (nreds > 0) hipLaunchKernelGGL(k_bspmv_reduce, dim3(nreds), dim3(256), 0, s, reds, P, y);
 }
 
 void launch_combos(const DevCombo* d_combos, const DevSrc* d_src, int ncombo, const int64_t* d_chunk_start,

@@ -103,7 +103,8 @@ int chm_davidson(chm_ctx* ctx, double* x, int on_device, double rtol, double* ei
 int chm_flops(const chm_ctx* ctx, double* F_H, double* B_H);
 /* stats[16]: terms, combinations, combination doubles, W doubles, chain phase-1 tiles,
  * chain phase-2 tiles, explicit block pairs, chain terms, blocks, V, diagonal terms,
- * explicit matrix doubles, explicit column indices, explicit size threshold, 0, 0. */
+ * explicit matrix doubles, explicit column indices, explicit size threshold,
+ * explicit work items, 0. */
 int chm_plan_stats(const chm_ctx* ctx, int64_t* stats);
 /* The context's HIP stream (hipStream_t) for callers that time or order work on it. */
 int chm_stream(const chm_ctx* ctx, void** stream);
