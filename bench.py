@@ -41,11 +41,12 @@ def read_traffic():
 
 
 def cpu_baseline(L, D, site, seed, irr, T, V, x, y_gpu, budget_s):
-    """The oracle (single-threaded FP64 restatement of the reference) on a bounded sample
-    of output blocks of the same step; also checks the GPU result on that sample."""
+    """The oracle (FP64 restatement of the reference, OpenMP over output blocks like
+    Heff.cpp:54-64) on a bounded sample of output blocks of the same step; also checks
+    the GPU result on that sample."""
     import numpy as np
 
-    from oracle.wrap import OracleDMRG
+    from oracle.wrap import OracleDMRG, threads
 
     o = OracleDMRG(L, 7, irr, L, 0, 0, T, V)
     t0 = time.time()
@@ -71,10 +72,10 @@ def cpu_baseline(L, D, site, seed, irr, T, V, x, y_gpu, budget_s):
     return {
         "value": flops / secs / 1e12,
         "unit": "TFLOP/s",
-        "cores": 1,
+        "cores": threads(),
         "kind": "port",
         "sample": f"oracle Heff x on output blocks 0..{k1 - 1} of {nk} at the same site and data "
-                  f"({flops:.3e} of the step's GEMM flops, {secs:.1f} s, one thread)",
+                  f"({flops:.3e} of the step's GEMM flops, {secs:.1f} s, {threads()} threads)",
     }, err
 
 

@@ -7,14 +7,16 @@
 namespace chm {
 
 // ------------------------------------------------------------- Wigner -----
-static double g_fact[200];
-static bool g_fact_ready = false;
-static void fact_init() {
-  if (g_fact_ready) return;
-  g_fact[0] = 1.0;
-  for (int i = 1; i < 200; i++) g_fact[i] = g_fact[i - 1] * i;
-  g_fact_ready = true;
-}
+struct FactTable {  // filled at static initialisation, before any concurrent reader
+  double v[200];
+  FactTable() {
+    v[0] = 1.0;
+    for (int i = 1; i < 200; i++) v[i] = v[i - 1] * i;
+  }
+};
+static const FactTable g_fact_table;
+static const double* const g_fact = g_fact_table.v;
+static void fact_init() {}
 static bool tri_bad(int a, int b, int c) { return ((a + b + c) % 2) != 0 || c < std::abs(a - b) || c > a + b; }
 static double tri_coef(int a, int b, int c) {  // Delta(abc) with doubled arguments
   return std::sqrt(g_fact[(a + b - c) / 2] * g_fact[(a - b + c) / 2] * g_fact[(-a + b + c) / 2] / g_fact[(a + b + c) / 2 + 1]);

@@ -3,6 +3,8 @@
 // product library never does).
 #include <chrono>
 
+#include <omp.h>
+
 #include "dmrg.hpp"
 
 using namespace orc;
@@ -150,6 +152,9 @@ long orc_synthetic(void* p, int D, int site, unsigned long long seed_l, unsigned
   h->site = site;
   return (long)h->S->size();
 }
+
+// Threads the OpenMP parts of the oracle use (its Heff application).
+int orc_threads() { return omp_get_max_threads(); }
 
 // Only the bookkeeper at D (sector dimensions), for layout comparisons.
 void orc_bookkeeper(void* p, int D) {

@@ -14,14 +14,17 @@ int num_irreps(int group) {  // Irreps.cpp:100-108
 }
 
 // ------------------------------------------------------------------ 6j -----
-static long double g_fact[256];
-static bool g_fact_ready = false;
-static void fact_init() {
-  if (g_fact_ready) return;
-  g_fact[0] = 1.0L;
-  for (int i = 1; i < 256; i++) g_fact[i] = g_fact[i - 1] * (long double)i;
-  g_fact_ready = true;
-}
+// Factorial table, filled once before any thread can read it (static initialisation).
+struct FactTable {
+  long double v[256];
+  FactTable() {
+    v[0] = 1.0L;
+    for (int i = 1; i < 256; i++) v[i] = v[i - 1] * (long double)i;
+  }
+};
+static const FactTable g_fact_table;
+static const long double* const g_fact = g_fact_table.v;
+static void fact_init() {}
 static bool tri_fail(int a, int b, int c) {
   if ((a + b + c) % 2 != 0) return true;
   if (c > a + b || c < std::abs(a - b)) return true;

@@ -55,7 +55,22 @@ void Heff::apply(const double* x_, double* y_) { apply_range(x_, y_, 0, S->nk())
 void Heff::apply_range(const double* x_, double* y_, int k0, int k1) {  // Heff.cpp:43-248
   x = x_;
   y = y_;
-  for (int ik = k0; ik < k1; ik++) {
+  // Output blocks are independent (OpenMP over kappa, as Heff.cpp:54-64); each thread
+  // works on its own copy of the scratch state and adds its flop count at the end.
+  double total = 0.0;
+#pragma omp parallel reduction(+ : total)
+  {
+    Heff local = *this;
+    local.flops = 0.0;
+#pragma omp for schedule(dynamic)
+    for (int ik = k0; ik < k1; ik++) local.apply_block(ik);
+    total += local.flops;
+  }
+  flops += total;
+}
+
+void Heff::apply_block(int ik) {
+  {
     std::fill(y + S->k2i[ik], y + S->k2i[ik + 1], 0.0);
     d1(ik);
     d2loc(ik);

@@ -22,6 +22,7 @@ struct Heff {
   void apply(const double* x, double* y);  // makeHeff
   // Output blocks k0 <= kappa < k1 only (a bounded sample of one application).
   void apply_range(const double* x, double* y, int k0, int k1);
+  void apply_block(int ik);
   void diag(double* d);                    // fillHeffDiag, Heff.cpp:250-315
 
   // --- internals -------------------------------------------------------

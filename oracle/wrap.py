@@ -43,6 +43,7 @@ def lib():
             "orc_step_left": (d, [vp, i, i, d, i]),
             "orc_synthetic": (L_, [vp, i, i, C.c_ulonglong, C.c_ulonglong]),
             "orc_bookkeeper": (None, [vp, i]),
+            "orc_threads": (i, []),
             "orc_site_apply_range": (d, [vp, _dp, _dp, i, i, C.POINTER(C.c_double)]),
             "orc_site": (L_, [vp, i]),
             "orc_site_x": (None, [vp, _dp]),
@@ -194,6 +195,10 @@ class OracleDMRG:
 
     def matvec_stats(self):
         return self.lib.orc_matvec_count(self.h), self.lib.orc_matvec_flops(self.h)
+
+
+def threads():
+    return lib().orc_threads()
 
 
 def wigner6j(*a):
