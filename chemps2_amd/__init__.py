@@ -20,6 +20,7 @@ EXPORTS = [
     "chm_set_sectors_auto", "chm_get_sectors", "chm_tensor_upload", "chm_tensor_fill_synthetic",
     "chm_tensor_clear", "chm_heff_prepare", "chm_heff_veclen", "chm_heff_apply", "chm_heff_diag",
     "chm_davidson", "chm_flops", "chm_plan_stats", "chm_stream", "chm_timing_enable", "chm_timing_read",
+    "chm_mps_upload", "chm_mps_download", "chm_tensor_download", "chm_update_operators",
 ]
 
 _lib = None
@@ -51,6 +52,10 @@ def load():
         "chm_tensor_upload": (i, [vp, i, i, i, i, i, i, i, i, _dp, i64]),
         "chm_tensor_fill_synthetic": (i, [vp, i, i, C.c_uint64]),
         "chm_tensor_clear": (i, [vp, i]),
+        "chm_mps_upload": (i, [vp, i, _dp, i64]),
+        "chm_mps_download": (i, [vp, i, _dp, i64]),
+        "chm_tensor_download": (i, [vp, i, i, i, i, _dp, i64]),
+        "chm_update_operators": (i, [vp, i, i]),
         "chm_heff_prepare": (i, [vp, i]),
         "chm_heff_veclen": (i, [vp, C.POINTER(i64)]),
         "chm_heff_apply": (i, [vp, vp, vp, i]),
@@ -134,6 +139,24 @@ class Context:
 
     def clear(self, cnt):
         self._check(self.lib.chm_tensor_clear(self.h, cnt), "chm_tensor_clear")
+
+    def mps_upload(self, site, data):
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        self._check(self.lib.chm_mps_upload(self.h, int(site), data, data.size), "chm_mps_upload")
+
+    def mps_download(self, site, n):
+        out = np.zeros(int(n))
+        self._check(self.lib.chm_mps_download(self.h, int(site), out, out.size), "chm_mps_download")
+        return out
+
+    def tensor_download(self, cnt, kind, k, m, n):
+        out = np.zeros(int(n))
+        self._check(self.lib.chm_tensor_download(self.h, int(cnt), KINDS[kind] if isinstance(kind, str) else kind,
+                                                 int(k), int(m), out, out.size), "chm_tensor_download")
+        return out
+
+    def update_operators(self, index, moving_right):
+        self._check(self.lib.chm_update_operators(self.h, int(index), int(moving_right)), "chm_update_operators")
 
     def prepare(self, site):
         self._check(self.lib.chm_heff_prepare(self.h, int(site)), "chm_heff_prepare")

@@ -66,12 +66,33 @@ enum : int32_t { BUF_X = 0, BUF_COMBO = 1, BUF_TENSOR0 = 2 };
 // (n, 2s, i) x (n + n_elec, 2s', i ^ n_irrep) with |s - s'| <= two_j.
 struct OpT {
   int boundary = 0, two_j = 0, n_elec = 0, n_irrep = 0;
+  int index = 0;  // == boundary (the reference's TensorOperator::gIndex)
+  bool moving_right = true, prime_last = true, jw_phase = false;
   int32_t buf = -1;  // buffer id of its device copy
   int64_t size = 0;
   std::unordered_map<uint64_t, int64_t> off;  // key (n, s, i, s') -> element offset
+  std::vector<int> Nup, Sup, Iup, Sdn;         // blocks in storage order
+  std::vector<int64_t> k2i;
   void build(const Sectors& sec, int b, int two_j, int n_elec, int n_irrep);
   Ref blk(int n1, int s1, int i1, int n2, int s2, int i2) const;
+  Ref blkref(int ik) const { Ref r; r.buf = buf; r.off = k2i[ik]; return r; }
+  int nk() const { return (int)Nup.size(); }
   static uint64_t key(int n, int s, int i, int s2) { return key3(n, s, i) | ((uint64_t)(s2 & 0xffff) << 40); }
+};
+
+// Block layout of an MPS site tensor (TensorT.cpp:38-104): blocks (NL, 2SL, IL) x
+// (NR, 2SR, IR) with NR - NL in {0, 1, 2}, column-major dL x dR.
+struct TLay {
+  int index = 0;
+  int32_t buf = -1;
+  int64_t size = 0;
+  std::vector<int> NL, SL, IL, NR, SR, IR;
+  std::vector<int64_t> k2i;
+  std::unordered_map<uint64_t, int> lut;
+  void build(const Sectors& sec, int site, int Is);
+  int kappa(int nl, int sl, int il, int nr, int sr, int ir) const;
+  Ref blk(int nl, int sl, int il, int nr, int sr, int ir) const;
+  int nk() const { return (int)NL.size(); }
 };
 
 // All operators of one boundary, indexed like DMRG::allocateTensors

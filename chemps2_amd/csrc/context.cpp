@@ -38,6 +38,8 @@ void Ctx::set_problem(int L_, int group_, const int* irr, int N_, int TwoS_, int
   buf_len.assign(BUF_TENSOR0, 0);
   sets.clear();
   sets.resize(L - 1);
+  mps.assign(L, TLay());
+  mps_buf.assign(L, nullptr);
   site = -1;
   plan.reset();
   have_problem = true;
@@ -125,6 +127,12 @@ void Ctx::clear_set(int cnt) {
     }
   s = OpSet();
   site = -1;
+}
+
+const double* Ctx::resolve_with(const Ref& r, const double* combo_base) const {
+  if (r.buf == BUF_COMBO) return combo_base + r.off;
+  if (r.buf >= BUF_TENSOR0 && r.buf < (int)bufs.size() && bufs[r.buf]) return bufs[r.buf] + r.off;
+  throw Error{ERR_MISSING, "unresolvable block reference"};
 }
 
 const double* Ctx::resolve(const Ref& r) const {
@@ -498,6 +506,208 @@ void Ctx::symm(double* x, bool to_symm) {
   launch_symm_scale(x, d_k2i.p, to_symm ? d_sym.p : d_prog.p, S.nk(), stream);
 }
 
+// ------------------------------------------------------ operator updates -----
+void Ctx::mps_alloc(int st) {
+  if (st < 0 || st >= L) throw Error{ERR_ARG, "site out of range"};
+  TLay& T = mps[st];
+  T.build(sec, st, irreps[st]);
+  if (T.buf >= 0 && bufs[T.buf]) { (void)hipFree(bufs[T.buf]); bufs[T.buf] = nullptr; buf_len[T.buf] = 0; }
+  double* p = nullptr;
+  if (T.size > 0) hip_check(hipMalloc((void**)&p, T.size * sizeof(double)), "hipMalloc MPS");
+  if (T.buf < 0) { T.buf = (int32_t)bufs.size(); bufs.push_back(p); buf_len.push_back(T.size); }
+  else { bufs[T.buf] = p; buf_len[T.buf] = T.size; }
+  mps_buf[st] = p;
+}
+
+OpT* Ctx::find_tensor(int cnt, int kind, int k, int m) {
+  if (cnt < 0 || cnt > L - 2) return nullptr;
+  OpSet& s = sets[cnt];
+  auto g1 = [&](std::vector<OpT*>& v) { return k < (int)v.size() ? v[k] : nullptr; };
+  auto g2 = [&](std::vector<std::vector<OpT*>>& v) { return (k < (int)v.size() && m < (int)v[k].size()) ? v[k][m] : nullptr; };
+  switch (kind) {
+    case KIND_L: return g1(s.L);
+    case KIND_Q: return g1(s.Q);
+    case KIND_X: return s.X;
+    case KIND_F0: return g2(s.F0);
+    case KIND_F1: return g2(s.F1);
+    case KIND_S0: return g2(s.S0);
+    case KIND_S1: return g2(s.S1);
+    case KIND_A: return g2(s.A);
+    case KIND_B: return g2(s.B);
+    case KIND_C: return g2(s.C);
+    case KIND_D: return g2(s.D);
+  }
+  return nullptr;
+}
+
+void Ctx::run_terms(const std::vector<UTerm>& terms, const std::vector<std::pair<Ref, std::pair<int, int>>>& outs,
+                    const double* combo_base) {
+  auto key = [](const Ref& r) { return ((uint64_t)(uint32_t)r.buf << 40) ^ (uint64_t)r.off; };
+  std::unordered_map<uint64_t, int> where;
+  where.reserve(outs.size() * 2);
+  for (size_t q = 0; q < outs.size(); q++) where[key(outs[q].first)] = (int)q;
+  std::vector<std::vector<int>> by_out(outs.size());
+  for (size_t t = 0; t < terms.size(); t++) {
+    auto it = where.find(key(terms[t].out));
+    if (it == where.end()) throw Error{ERR_STATE, "operator update term writes outside the new tensors"};
+    by_out[it->second].push_back((int)t);
+  }
+  std::vector<DevGemmOp> o1, o2;
+  std::vector<DevTile> t1, t2;
+  std::vector<int64_t> woff(terms.size(), -1);
+  int64_t Wsz = 0;
+  for (size_t t = 0; t < terms.size(); t++) {
+    const UTerm& u = terms[t];
+    if (u.kind != 1) continue;
+    DevGemmOp op{};
+    op.A = resolve_with(u.A, combo_base); op.ta = u.ta; op.lda = u.lda;
+    op.B = resolve_with(u.B, combo_base); op.tb = u.tb; op.ldb = u.ldb;
+    op.k = u.k1; op.alpha = u.alpha; op.kind = 0;
+    woff[t] = Wsz;
+    const int opi = (int)o1.size();
+    o1.push_back(op);
+    for (int m0 = 0; m0 < u.m; m0 += 64)
+      for (int n0 = 0; n0 < u.k2; n0 += 64) {
+        DevTile tl{};
+        tl.C = (double*)(intptr_t)Wsz;
+        tl.ldc = u.m; tl.m0 = m0; tl.n0 = n0; tl.m = u.m; tl.n = u.k2; tl.op0 = opi; tl.nop = 1; tl.flags = 0;
+        t1.push_back(tl);
+      }
+    Wsz += ((int64_t)u.m * u.k2 + 7) / 8 * 8;
+  }
+  DBuf<double> Wb;
+  Wb.alloc(std::max<int64_t>(Wsz, 1));
+  for (auto& tl : t1) tl.C = Wb.p + (intptr_t)tl.C;
+  for (size_t q = 0; q < outs.size(); q++) {
+    const int m = outs[q].second.first, n = outs[q].second.second;
+    if (m <= 0 || n <= 0) continue;
+    const int op0 = (int)o2.size();
+    for (int t : by_out[q]) {
+      const UTerm& u = terms[t];
+      DevGemmOp op{};
+      if (u.kind == 1) {
+        op.A = Wb.p + woff[t]; op.ta = 0; op.lda = u.m;
+        op.B = resolve_with(u.C, combo_base); op.tb = u.tc; op.ldb = u.ldc;
+        op.k = u.k2; op.alpha = 1.0; op.kind = 0;
+      } else if (u.kind == 0) {
+        op.A = resolve_with(u.A, combo_base); op.ta = u.ta; op.lda = u.lda;
+        op.B = resolve_with(u.B, combo_base); op.tb = u.tb; op.ldb = u.ldb;
+        op.k = u.k1; op.alpha = u.alpha; op.kind = 0;
+      } else {
+        op.A = resolve_with(u.A, combo_base); op.ta = u.ta; op.lda = u.lda;
+        op.alpha = u.alpha; op.kind = 1;
+      }
+      o2.push_back(op);
+    }
+    const int nop = (int)o2.size() - op0;
+    double* C = const_cast<double*>(resolve_with(outs[q].first, combo_base));
+    for (int m0 = 0; m0 < m; m0 += 64)
+      for (int n0 = 0; n0 < n; n0 += 64) {
+        DevTile tl{};
+        tl.C = C; tl.ldc = m; tl.m0 = m0; tl.n0 = n0; tl.m = m; tl.n = n; tl.op0 = op0; tl.nop = nop; tl.flags = 0;
+        t2.push_back(tl);
+      }
+  }
+  DBuf<DevGemmOp> d1, d2;
+  DBuf<DevTile> dt1, dt2;
+  d1.upload(o1, stream);
+  d2.upload(o2, stream);
+  dt1.upload(t1, stream);
+  dt2.upload(t2, stream);
+  if ((int64_t)t1.size() > INT32_MAX || (int64_t)t2.size() > INT32_MAX) throw Error{ERR_SIZE, "too many update tiles"};
+  launch_tiles(dt1.p, d1.p, (int)t1.size(), nullptr, nullptr, stream);
+  launch_tiles(dt2.p, d2.p, (int)t2.size(), nullptr, nullptr, stream);
+  hip_check(hipGetLastError(), "update kernels");
+  hip_check(hipStreamSynchronize(stream), "update");
+}
+
+// DMRG::updateMovingRight/Left (DMRGoperators.cpp:243-907) for slot `index`.
+void Ctx::update_operators(int index, bool mr) {
+  if (!have_problem) throw Error{ERR_STATE, "set the problem first"};
+  if (index < 0 || index > L - 2) throw Error{ERR_ARG, "slot out of range"};
+  const int tsite = mr ? index : index + 1;
+  if (mps[tsite].buf < 0 || !bufs[mps[tsite].buf]) throw Error{ERR_MISSING, "MPS tensor of site " + std::to_string(tsite) + " not set"};
+  {
+    TLay check;
+    check.build(sec, tsite, irreps[tsite]);
+    if (check.size != mps[tsite].size) throw Error{ERR_STATE, "MPS tensor layout does not match the current sectors"};
+  }
+  OpSet* prev = nullptr;
+  if (mr && index > 0) { validate_set(index - 1, true); prev = &sets[index - 1]; }
+  if (!mr && index < L - 2) { validate_set(index + 1, false); prev = &sets[index + 1]; }
+  // lay out the new slot with the flags of DMRG::allocateTensors
+  clear_set(index);
+  for (const OpShape& sh : opset_shapes(*this, index, mr)) {
+    OpT* t = add_tensor(index, mr, sh.kind, sh.k, sh.m, sh.two_j, sh.n_elec, sh.n_irrep, -1);
+    t->moving_right = mr;
+    t->jw_phase = (sh.kind == KIND_L || sh.kind == KIND_Q);
+    t->prime_last = (sh.kind == KIND_F1 || sh.kind == KIND_D) ? mr : true;
+  }
+  OpSet& o = sets[index];
+  std::vector<int> Qsite;
+  const int nC = mr ? L - 1 - index : index + 1;
+  for (int k = 0; k < nC; k++) Qsite.push_back(mr ? index + 1 + k : index - k);
+  UpdatePlanner up;
+  up.sec = &sec;
+  up.irr = irreps.data();
+  up.mx = mx.data();
+  up.L = L;
+  if (mr) up.moving_right(index, o, prev, &mps[tsite], Qsite);
+  else up.moving_left(index, o, prev, &mps[tsite], Qsite);
+  last_update_flops = up.flops;
+  last_update_terms = (int64_t)(up.terms[1].size() + up.terms[2].size());
+  // combinations first (they read the previous slot only)
+  DBuf<double> arena;
+  arena.alloc(std::max<int64_t>(up.combo_size, 1));
+  if (!up.combos.empty()) {
+    std::vector<DevCombo> dc;
+    std::vector<DevSrc> ds;
+    std::vector<int64_t> chunk_start;
+    int64_t nch = 0;
+    for (const Combo& c : up.combos) {
+      DevCombo d;
+      d.out = arena.p + c.out_off; d.size = c.size; d.src0 = (int)ds.size(); d.nsrc = (int)c.src.size();
+      for (auto& e : c.src) ds.push_back({resolve(e.first), e.second});
+      dc.push_back(d);
+      chunk_start.push_back(nch);
+      nch += (c.size + 1023) / 1024;
+    }
+    DBuf<DevCombo> d_dc;
+    DBuf<DevSrc> d_ds;
+    DBuf<int64_t> d_cs;
+    d_dc.upload(dc, stream);
+    d_ds.upload(ds, stream);
+    d_cs.upload(chunk_start, stream);
+    launch_combos(d_dc.p, d_ds.p, (int)dc.size(), d_cs.p, nch, stream);
+    hip_check(hipGetLastError(), "combination kernel launch");
+    hip_check(hipStreamSynchronize(stream), "combinations");
+  }
+  // outputs of each level
+  std::vector<std::pair<Ref, std::pair<int, int>>> out1, out2;
+  auto add_outs = [&](OpT* t, std::vector<std::pair<Ref, std::pair<int, int>>>& v) {
+    if (!t) return;
+    for (int ik = 0; ik < t->nk(); ik++) {
+      const int du = sec.dim(t->boundary, t->Nup[ik], t->Sup[ik], t->Iup[ik]);
+      const int dd = sec.dim(t->boundary, t->Nup[ik] + t->n_elec, t->Sdn[ik], t->Iup[ik] ^ t->n_irrep);
+      v.push_back({t->blkref(ik), {du, dd}});
+    }
+  };
+  for (OpT* t : o.L) add_outs(t, out1);
+  for (auto& row : o.F0) for (OpT* t : row) add_outs(t, out1);
+  for (auto& row : o.F1) for (OpT* t : row) add_outs(t, out1);
+  for (auto& row : o.S0) for (OpT* t : row) add_outs(t, out1);
+  for (auto& row : o.S1) for (OpT* t : row) add_outs(t, out1);
+  for (OpT* t : o.Q) add_outs(t, out1);
+  add_outs(o.X, out1);
+  for (auto& row : o.A) for (OpT* t : row) add_outs(t, out2);
+  for (auto& row : o.B) for (OpT* t : row) add_outs(t, out2);
+  for (auto& row : o.C) for (OpT* t : row) add_outs(t, out2);
+  for (auto& row : o.D) for (OpT* t : row) add_outs(t, out2);
+  run_terms(up.terms[1], out1, arena.p);
+  run_terms(up.terms[2], out2, arena.p);
+  site = -1;
+}
+
 // ------------------------------------------------------------ Davidson -----
 static void jacobi_eig(int n, const std::vector<double>& A, int lda, std::vector<double>& w, std::vector<double>& Vv) {
   std::vector<double> a((size_t)n * n), v((size_t)n * n, 0.0);
@@ -819,6 +1029,42 @@ int chm_tensor_clear(chm_ctx* ctx, int cnt) {
   return guard(ctx, [&](Ctx& c) {
     if (!c.have_problem || cnt < 0 || cnt > c.L - 2) throw Error{CHM_ERR_ARG, "slot out of range"};
     c.clear_set(cnt);
+  });
+}
+
+int chm_mps_upload(chm_ctx* ctx, int site, const double* data, int64_t n) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!c.have_problem) throw Error{CHM_ERR_STATE, "set the problem first"};
+    if (n > 0 && !data) throw Error{CHM_ERR_ARG, "null pointer"};
+    (void)hipSetDevice(c.device);
+    c.mps_alloc(site);
+    if (c.mps[site].size != n)
+      throw Error{CHM_ERR_SIZE, "MPS size " + std::to_string(n) + " != layout size " + std::to_string(c.mps[site].size)};
+    if (n > 0) chm::hip_check(hipMemcpy(c.mps_buf[site], data, n * sizeof(double), hipMemcpyHostToDevice), "MPS upload");
+  });
+}
+
+int chm_mps_download(chm_ctx* ctx, int site, double* data, int64_t n) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!c.have_problem || site < 0 || site >= c.L || c.mps[site].buf < 0) throw Error{CHM_ERR_STATE, "no MPS tensor at this site"};
+    if (n != c.mps[site].size) throw Error{CHM_ERR_SIZE, "size mismatch"};
+    if (n > 0) chm::hip_check(hipMemcpy(data, c.mps_buf[site], n * sizeof(double), hipMemcpyDeviceToHost), "MPS download");
+  });
+}
+
+int chm_tensor_download(chm_ctx* ctx, int cnt, int kind, int k, int m, double* data, int64_t n) {
+  return guard(ctx, [&](Ctx& c) {
+    chm::OpT* t = c.find_tensor(cnt, kind, k, m);
+    if (!t) throw Error{CHM_ERR_MISSING, "no such tensor"};
+    if (n != t->size) throw Error{CHM_ERR_SIZE, "size " + std::to_string(n) + " != " + std::to_string(t->size)};
+    if (n > 0) chm::hip_check(hipMemcpy(data, c.bufs[t->buf], n * sizeof(double), hipMemcpyDeviceToHost), "tensor download");
+  });
+}
+
+int chm_update_operators(chm_ctx* ctx, int index, int moving_right) {
+  return guard(ctx, [&](Ctx& c) {
+    (void)hipSetDevice(c.device);
+    c.update_operators(index, moving_right != 0);
   });
 }
 

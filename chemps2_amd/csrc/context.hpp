@@ -5,6 +5,7 @@
 
 #include "chm_internal.hpp"
 #include "heff_plan.hpp"
+#include "update_plan.hpp"
 
 namespace chm {
 
@@ -106,8 +107,22 @@ struct Ctx : HostState {
   double t_ms[3] = {0.0, 0.0, 0.0};
   int64_t t_count = 0;
 
+  // MPS site tensors on the device (layouts from the current sectors)
+  std::vector<TLay> mps;
+  std::vector<double*> mps_buf;
+  double last_update_flops = 0.0;
+  int64_t last_update_terms = 0;
+
   ~Ctx();
+  void mps_alloc(int site);
+  OpT* find_tensor(int cnt, int kind, int k, int m);
+  void update_operators(int index, bool mr);
+  // Executes operator-update terms: every block in outs is overwritten with the sum of
+  // its terms (zero if none).
+  void run_terms(const std::vector<UTerm>& terms, const std::vector<std::pair<Ref, std::pair<int, int>>>& outs,
+                 const double* combo_base);
   void set_problem(int L, int group, const int* irr, int N, int TwoS, int Irrep, const double* mx, double econst);
+  const double* resolve_with(const Ref& r, const double* combo_base) const;
   void set_sectors(int b, int n, const int* Ns, const int* Ss, const int* Is, const int* ds);
   OpT* add_tensor(int cnt, bool mr, int kind, int k, int m, int two_j, int n_elec, int n_irrep, int64_t n);
   void clear_set(int cnt);

@@ -44,7 +44,9 @@ __global__ __launch_bounds__(NT) void k_tiles(const DevTile* __restrict__ tiles,
 #pragma unroll
         for (int j = 0; j < 4; j++) {
           const int r = tx + 16 * i, c = ty + 16 * j;
-          if (r < mrem && c < nrem) acc[i][j] += op.alpha * A[(size_t)(t.m0 + r) + (size_t)op.lda * (t.n0 + c)];
+          if (r < mrem && c < nrem)
+            acc[i][j] += op.alpha * (op.ta ? A[(size_t)(t.n0 + c) + (size_t)op.lda * (t.m0 + r)]
+                                           : A[(size_t)(t.m0 + r) + (size_t)op.lda * (t.n0 + c)]);
         }
       continue;
     }

@@ -54,8 +54,10 @@ double wigner9j(int a, int b, int c, int d, int e, int f, int g, int h, int i) {
 
 // ---------------------------------------------------------------- OpT ------
 void OpT::build(const Sectors& sec, int b, int tj, int ne, int nirr) {  // TensorOperator.cpp:29-102
-  boundary = b; two_j = tj; n_elec = ne; n_irrep = nirr;
+  boundary = b; index = b; two_j = tj; n_elec = ne; n_irrep = nirr;
   off.clear();
+  Nup.clear(); Sup.clear(); Iup.clear(); Sdn.clear();
+  k2i.assign(1, 0);
   int64_t pos = 0;
   for (const auto& up : sec.list[b]) {
     const int nd = up.N + ne, id = up.I ^ nirr;
@@ -64,10 +66,55 @@ void OpT::build(const Sectors& sec, int b, int tj, int ne, int nirr) {  // Tenso
       const int dd = sec.dim(b, nd, sd, id);
       if (dd <= 0) continue;
       off[key(up.N, up.S, up.I, sd)] = pos;
+      Nup.push_back(up.N); Sup.push_back(up.S); Iup.push_back(up.I); Sdn.push_back(sd);
       pos += (int64_t)up.d * dd;
+      k2i.push_back(pos);
     }
   }
   size = pos;
+}
+
+// ---------------------------------------------------------------- TLay -----
+static uint64_t key6(int nl, int sl, int il, int nr, int sr, int ir) {
+  return ((uint64_t)(nl & 0xff)) | ((uint64_t)(sl & 0xff) << 8) | ((uint64_t)(il & 0x7) << 16) | ((uint64_t)(nr & 0xff) << 19) |
+         ((uint64_t)(sr & 0xff) << 27) | ((uint64_t)(ir & 0x7) << 35);
+}
+
+void TLay::build(const Sectors& sec, int site, int Is) {  // TensorT.cpp:38-104
+  index = site;
+  NL.clear(); SL.clear(); IL.clear(); NR.clear(); SR.clear(); IR.clear(); lut.clear();
+  k2i.assign(1, 0);
+  for (const auto& l : sec.list[site]) {
+    for (int nr = l.N; nr <= l.N + 2; nr++) {
+      const int tj = (nr == l.N + 1) ? 1 : 0;
+      for (int sr = l.S - tj; sr <= l.S + tj; sr += 2) {
+        if (sr < 0) continue;
+        const int ir = (nr == l.N + 1) ? (l.I ^ Is) : l.I;
+        const int dr = sec.dim(site + 1, nr, sr, ir);
+        if (dr <= 0) continue;
+        lut[key6(l.N, l.S, l.I, nr, sr, ir)] = (int)NL.size();
+        NL.push_back(l.N); SL.push_back(l.S); IL.push_back(l.I);
+        NR.push_back(nr); SR.push_back(sr); IR.push_back(ir);
+        k2i.push_back(k2i.back() + (int64_t)l.d * dr);
+      }
+    }
+  }
+  size = k2i.back();
+}
+
+int TLay::kappa(int nl, int sl, int il, int nr, int sr, int ir) const {
+  if (nl < 0 || sl < 0 || nr < 0 || sr < 0) return -1;
+  auto it = lut.find(key6(nl, sl, il, nr, sr, ir));
+  return it == lut.end() ? -1 : it->second;
+}
+
+Ref TLay::blk(int nl, int sl, int il, int nr, int sr, int ir) const {
+  Ref r;
+  const int k = kappa(nl, sl, il, nr, sr, ir);
+  if (k < 0) return r;
+  r.buf = buf;
+  r.off = k2i[k];
+  return r;
 }
 
 Ref OpT::blk(int n1, int s1, int i1, int n2, int s2, int i2) const {

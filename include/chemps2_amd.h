@@ -82,6 +82,19 @@ int chm_tensor_fill_synthetic(chm_ctx* ctx, int cnt, int moving_right, uint64_t 
 /* Drops the operators of slot cnt (DMRG::deleteTensors). */
 int chm_tensor_clear(chm_ctx* ctx, int cnt);
 
+/* MPS site tensor `site` in the storage of TensorT (TensorT.cpp:38-104; blocks
+ * (NL, 2SL, IL) x (NR, 2SR, IR) for the current sectors of boundaries site and site+1). */
+int chm_mps_upload(chm_ctx* ctx, int site, const double* data, int64_t n);
+int chm_mps_download(chm_ctx* ctx, int site, double* data, int64_t n);
+/* Copy of one operator tensor of slot cnt to the host (the TensorOperator storage). */
+int chm_tensor_download(chm_ctx* ctx, int cnt, int kind, int k, int m, double* data, int64_t n);
+/* Renormalised operators of slot `index` built on the device from the neighbouring slot
+ * and one MPS tensor: DMRG::updateMovingRight(index) (moving_right = 1, uses site
+ * `index`, slot index-1) or updateMovingLeft(index) (site index+1, slot index+1)
+ * (DMRGoperators.cpp:243-907 with TensorL/F0/F1/S0/S1/Q/X and
+ * TensorOperator::update). */
+int chm_update_operators(chm_ctx* ctx, int index, int moving_right);
+
 /* Two-site step at (site, site+1): builds the work lists of Heff (the reference's
  * Heff(denBK, Prob, rtol) + Sobject(site, denBK), Heff.h:50, Sobject.cpp:36-147).
  * Needs slot site-1 built moving right (site > 0) and slot site+1 moving left

@@ -281,6 +281,13 @@ long orc_op(void* p, int cnt, int kind, int k, int m, int* meta, const double** 
   return (long)t->data.size();
 }
 
+// MPS site tensor data (TensorT storage); returns the number of doubles.
+long orc_mps(void* p, int site, const double** data) {
+  TensorT& T = ((Handle*)p)->dm.mps[site];
+  if (data) *data = T.data.data();
+  return (long)T.data.size();
+}
+
 int orc_op_qsite(void* p, int cnt, int k) { return ((Handle*)p)->dm.ops.ops[cnt].Qsite[k]; }
 
 double orc_wigner6j(int a, int b, int c, int d, int e, int f) { return wigner6j(a, b, c, d, e, f); }

@@ -60,6 +60,7 @@ def lib():
             "orc_sectors": (i, [vp, i, C.c_void_p]),
             "orc_op": (L_, [vp, i, i, i, i, C.c_void_p, C.POINTER(C.POINTER(C.c_double))]),
             "orc_op_qsite": (i, [vp, i, i]),
+            "orc_mps": (L_, [vp, i, C.POINTER(C.POINTER(C.c_double))]),
             "orc_wigner6j": (d, [i] * 6),
             "orc_wigner9j": (d, [i] * 9),
             "orc_davidson_dense": (d, [i, _dp, _dp, d, _dp, C.POINTER(C.c_int)]),
@@ -192,6 +193,12 @@ class OracleDMRG:
             return None
         data = np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n > 0 else np.zeros(0)
         return meta, data
+
+    def mps(self, site):
+        """Copy of the MPS tensor of `site` (TensorT storage)."""
+        ptr = C.POINTER(C.c_double)()
+        n = self.lib.orc_mps(self.h, int(site), C.byref(ptr))
+        return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n > 0 else np.zeros(0)
 
     def matvec_stats(self):
         return self.lib.orc_matvec_count(self.h), self.lib.orc_matvec_flops(self.h)

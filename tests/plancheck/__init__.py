@@ -43,6 +43,14 @@ def lib():
             "pc_combos": (None, [vp, _lp, _ip, _lp, _dp]),
             "pc_ndiag": (lg, [vp]),
             "pc_diag": (None, [vp, _ip, _lp, _dp]),
+            "pc_mps": (i, [vp, i, C.POINTER(C.c_long)]),
+            "pc_update": (lg, [vp, i, i]),
+            "pc_update_newt": (None, [vp, _lp]),
+            "pc_update_nterms": (lg, [vp, i]),
+            "pc_update_terms": (None, [vp, i, _ip, _lp, _dp]),
+            "pc_update_ncombos": (lg, [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
+            "pc_update_combos": (None, [vp, _lp, _ip, _lp, _dp]),
+            "pc_update_flops": (d, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -172,3 +180,104 @@ class Plan:
                 dr_ = self.bufs[rb][self.do[2 * q + 1]: self.do[2 * q + 1] + dR * dR:dR + 1]
                 blk += c * np.outer(dl_, dr_)
         return d
+
+
+def _op(M, t, rows, cols, ld):
+    """op(M) as a (rows x cols) view of a column-major block with leading dimension ld."""
+    if t:
+        return M[: cols * ld].reshape(rows, ld).T[:cols, :].T if False else M[: (rows - 1) * ld + cols].copy()
+    return None
+
+
+def _blk(buf, off, rows, cols, ld, trans):
+    """The (rows x cols) operand op(B) where B is stored column-major with leading dimension ld."""
+    if not trans:
+        n = (cols - 1) * ld + rows
+        flat = buf[off: off + n]
+        return np.lib.stride_tricks.as_strided(flat, shape=(rows, cols), strides=(8, 8 * ld))
+    n = (rows - 1) * ld + cols
+    flat = buf[off: off + n]
+    return np.lib.stride_tricks.as_strided(flat, shape=(rows, cols), strides=(8 * ld, 8))
+
+
+class UpdatePlan:
+    """Operator-update work list for slot `index`, executed with numpy on the oracle's data."""
+
+    def __init__(self, o, index, moving_right):
+        l = lib()
+        self.l = l
+        L = o.L
+        self.h = l.pc_create(L, int(o.group), o.irreps(), o.N, o.TwoS, o.Irrep, o.mx())
+        for b in range(L + 1):
+            s = o.sectors(b)
+            l.pc_sectors(self.h, b, len(s), *[np.ascontiguousarray(s[:, c]) for c in range(4)])
+        bufs = {}
+        prev = index - 1 if moving_right else index + 1
+        if 0 <= prev <= L - 2:
+            for kind, k, m, meta, data in oracle_tensors(o, prev):
+                size = C.c_long(0)
+                buf = l.pc_tensor(self.h, prev, 1 if moving_right else 0, KIND_NAMES.index(kind), k, m, int(meta[1]),
+                                  int(meta[2]), int(meta[3]), C.byref(size))
+                assert size.value == len(data)
+                bufs[buf] = data
+        tsite = index if moving_right else index + 1
+        size = C.c_long(0)
+        buf = l.pc_mps(self.h, tsite, C.byref(size))
+        T = o.mps(tsite)
+        assert size.value == len(T), (size.value, len(T))
+        bufs[buf] = T
+        nt = l.pc_update(self.h, index, 1 if moving_right else 0)
+        if nt < 0:
+            raise RuntimeError(l.pc_error(self.h).decode())
+        newt = np.zeros(5 * nt, dtype=np.int64)
+        l.pc_update_newt(self.h, newt)
+        self.newt = newt.reshape(nt, 5)
+        for kind, k, m, b, sz in self.newt:
+            bufs[int(b)] = np.zeros(int(sz))
+        ns, ar = C.c_long(0), C.c_long(0)
+        nc = l.pc_update_ncombos(self.h, C.byref(ns), C.byref(ar))
+        meta = np.zeros(3 * max(nc, 1), dtype=np.int64)
+        sb = np.zeros(max(ns.value, 1), dtype=np.int32)
+        so = np.zeros(max(ns.value, 1), dtype=np.int64)
+        sc = np.zeros(max(ns.value, 1))
+        l.pc_update_combos(self.h, meta, sb, so, sc)
+        arena = np.zeros(max(ar.value, 1))
+        pos = 0
+        for q in range(nc):
+            off, sz, nsrc = meta[3 * q: 3 * q + 3]
+            acc = np.zeros(sz)
+            for e in range(pos, pos + nsrc):
+                acc += sc[e] * bufs[sb[e]][so[e]: so[e] + sz]
+            arena[off: off + sz] = acc
+            pos += nsrc
+        bufs[1] = arena
+        for level in (1, 2):
+            n = l.pc_update_nterms(self.h, level)
+            ints = np.zeros(16 * max(n, 1), dtype=np.int32)
+            offs = np.zeros(4 * max(n, 1), dtype=np.int64)
+            al = np.zeros(max(n, 1))
+            l.pc_update_terms(self.h, level, ints, offs, al)
+            ints = ints.reshape(-1, 16)
+            offs = offs.reshape(-1, 4)
+            for q in range(n):
+                ob, m_, n_, k1, k2, ab, bb, cb, ta, tb, tc, lda, ldb, ldc, kind, _ = ints[q]
+                oo, ao, bo, co = offs[q]
+                if kind == 2:
+                    M = al[q] * _blk(bufs[ab], ao, m_, n_, lda, ta)
+                elif kind == 0:
+                    M = al[q] * (_blk(bufs[ab], ao, m_, k1, lda, ta) @ _blk(bufs[bb], bo, k1, n_, ldb, tb))
+                else:
+                    W = al[q] * (_blk(bufs[ab], ao, m_, k1, lda, ta) @ _blk(bufs[bb], bo, k1, k2, ldb, tb))
+                    M = W @ _blk(bufs[cb], co, k2, n_, ldc, tc)
+                out = bufs[ob]
+                out[oo: oo + m_ * n_] += M.T.ravel()
+        self.bufs = bufs
+        self.flops = l.pc_update_flops(self.h)
+
+    def tensors(self):
+        """{(kind name, k, m): data} of the new slot."""
+        return {(KIND_NAMES[int(kd)], int(k), int(m)): self.bufs[int(b)] for kd, k, m, b, sz in self.newt}
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.l.pc_destroy(self.h)

@@ -1,9 +1,11 @@
 // Host-only access to the chemps2_amd work-list generator, for CPU tests: the tests
 // execute the generated terms with numpy and compare with the oracle's Heff.
+#include <array>
 #include <cstring>
 
 #include "../../chemps2_amd/csrc/context.hpp"
 #include "../../chemps2_amd/csrc/heff_plan.hpp"
+#include "../../chemps2_amd/csrc/update_plan.hpp"
 
 using namespace chm;
 
@@ -15,6 +17,10 @@ struct PC {
   SLayout S;
   std::unique_ptr<HeffPlanner> plan;
   std::string err;
+  std::vector<TLay> mps;
+  int next_buf = 1000000;  // buffer ids of MPS tensors and new operator tensors
+  std::unique_ptr<UpdatePlanner> up;
+  std::vector<std::array<int64_t, 5>> newt;  // kind, k, m, buf, size
 };
 }  // namespace
 
@@ -99,6 +105,120 @@ int pc_synthetic_set(void* h, int cnt, int mr) {
   }
   return n;
 }
+
+// MPS layout of `site`; returns its buffer id and writes the size.
+int pc_mps(void* h, int site, long* size) {
+  PC* p = (PC*)h;
+  if ((int)p->mps.size() < p->c.L) p->mps.resize(p->c.L);
+  TLay& T = p->mps[site];
+  T.build(p->c.sec, site, p->c.irreps[site]);
+  T.buf = p->next_buf++;
+  *size = T.size;
+  return T.buf;
+}
+
+// Work list of updateMovingRight/Left(index); returns the number of new tensors (-1 on error).
+long pc_update(void* h, int index, int mr) {
+  PC* p = (PC*)h;
+  try {
+    HostState& c = p->c;
+    OpSet* prev = nullptr;
+    if (mr && index > 0) { c.validate_set(index - 1, true); prev = &c.sets[index - 1]; }
+    if (!mr && index < c.L - 2) { c.validate_set(index + 1, false); prev = &c.sets[index + 1]; }
+    c.sets[index] = OpSet();
+    OpSet& o = c.sets[index];
+    o.dir = mr ? 1 : 2;
+    p->newt.clear();
+    for (const OpShape& sh : opset_shapes(c, index, mr != 0)) {
+      auto t = std::make_unique<OpT>();
+      t->build(c.sec, index + 1, sh.two_j, sh.n_elec, sh.n_irrep);
+      t->moving_right = mr != 0;
+      t->jw_phase = (sh.kind == KIND_L || sh.kind == KIND_Q);
+      t->prime_last = (sh.kind == KIND_F1 || sh.kind == KIND_D) ? (mr != 0) : true;
+      t->buf = p->next_buf++;
+      OpT* raw = t.get();
+      auto s1 = [&](std::vector<OpT*>& v) -> OpT*& { if ((int)v.size() <= sh.k) v.resize(sh.k + 1, nullptr); return v[sh.k]; };
+      auto s2 = [&](std::vector<std::vector<OpT*>>& v) -> OpT*& {
+        if ((int)v.size() <= sh.k) v.resize(sh.k + 1);
+        if ((int)v[sh.k].size() <= sh.m) v[sh.k].resize(sh.m + 1, nullptr);
+        return v[sh.k][sh.m];
+      };
+      switch (sh.kind) {
+        case KIND_L: s1(o.L) = raw; break;
+        case KIND_Q: s1(o.Q) = raw; break;
+        case KIND_X: o.X = raw; break;
+        case KIND_F0: s2(o.F0) = raw; break;
+        case KIND_F1: s2(o.F1) = raw; break;
+        case KIND_S0: s2(o.S0) = raw; break;
+        case KIND_S1: s2(o.S1) = raw; break;
+        case KIND_A: s2(o.A) = raw; break;
+        case KIND_B: s2(o.B) = raw; break;
+        case KIND_C: s2(o.C) = raw; break;
+        case KIND_D: s2(o.D) = raw; break;
+      }
+      p->newt.push_back({sh.kind, sh.k, sh.m, raw->buf, raw->size});
+      o.owned.push_back(std::move(t));
+    }
+    std::vector<int> Qsite;
+    const int nC = mr ? c.L - 1 - index : index + 1;
+    for (int k = 0; k < nC; k++) Qsite.push_back(mr ? index + 1 + k : index - k);
+    p->up = std::make_unique<UpdatePlanner>();
+    UpdatePlanner& u = *p->up;
+    u.sec = &c.sec; u.irr = c.irreps.data(); u.mx = c.mx.data(); u.L = c.L;
+    const TLay* T = &p->mps[mr ? index : index + 1];
+    if (mr) u.moving_right(index, o, prev, T, Qsite);
+    else u.moving_left(index, o, prev, T, Qsite);
+    return (long)p->newt.size();
+  } catch (const Error& e) {
+    p->err = e.msg;
+    return -1;
+  }
+}
+
+void pc_update_newt(void* h, long* out) {
+  PC* p = (PC*)h;
+  for (size_t q = 0; q < p->newt.size(); q++)
+    for (int j = 0; j < 5; j++) out[5 * q + j] = p->newt[q][j];
+}
+
+long pc_update_nterms(void* h, int level) { return (long)((PC*)h)->up->terms[level].size(); }
+
+// ints[16]: out.buf, m, n, k1, k2, A.buf, B.buf, C.buf, ta, tb, tc, lda, ldb, ldc, kind, 0;
+// offs[4]: out, A, B, C; alpha.
+void pc_update_terms(void* h, int level, int* ints, long* offs, double* alpha) {
+  PC* p = (PC*)h;
+  const auto& v = p->up->terms[level];
+  for (size_t q = 0; q < v.size(); q++) {
+    const UTerm& t = v[q];
+    int* o = ints + 16 * q;
+    o[0] = t.out.buf; o[1] = t.m; o[2] = t.n; o[3] = t.k1; o[4] = t.k2; o[5] = t.A.buf; o[6] = t.B.buf; o[7] = t.C.buf;
+    o[8] = t.ta; o[9] = t.tb; o[10] = t.tc; o[11] = t.lda; o[12] = t.ldb; o[13] = t.ldc; o[14] = t.kind; o[15] = 0;
+    long* f = offs + 4 * q;
+    f[0] = t.out.off; f[1] = t.A.off; f[2] = t.B.off; f[3] = t.C.off;
+    alpha[q] = t.alpha;
+  }
+}
+
+long pc_update_ncombos(void* h, long* nsrc_total, long* arena) {
+  PC* p = (PC*)h;
+  long n = 0;
+  for (auto& c : p->up->combos) n += (long)c.src.size();
+  *nsrc_total = n;
+  *arena = p->up->combo_size;
+  return (long)p->up->combos.size();
+}
+
+void pc_update_combos(void* h, long* meta, int* sbuf, long* soff, double* scoef) {
+  PC* p = (PC*)h;
+  long s = 0;
+  for (size_t q = 0; q < p->up->combos.size(); q++) {
+    const Combo& c = p->up->combos[q];
+    meta[3 * q] = c.out_off; meta[3 * q + 1] = c.size; meta[3 * q + 2] = (long)c.src.size();
+    for (auto& e : c.src) { sbuf[s] = e.first.buf; soff[s] = e.first.off; scoef[s] = e.second; s++; }
+  }
+}
+
+double pc_update_flops(void* h) { return ((PC*)h)->up->flops; }
 
 long pc_plan(void* h, int site) {
   PC* p = (PC*)h;

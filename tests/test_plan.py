@@ -92,3 +92,21 @@ def test_synthetic_plan_matches_oracle(oracle_lib):
     y1, f1, sec = o.apply_range(x, 0, 10)
     k2i = p.k2i
     assert np.array_equal(y1[: k2i[10]], y0[: k2i[10]]) and not y1[k2i[10]:].any()
+
+
+@pytest.mark.parametrize("index,mr", [(0, True), (2, True), (4, True), (6, False), (4, False), (2, False)])
+def test_update_plan_matches_oracle(oracle_lib, index, mr):
+    # operator updates (DMRGoperators.cpp:243-907) through the library's work list
+    from plancheck import UpdatePlan, oracle_tensors
+
+    irr, T, V = random_abinitio(8, 4, group=7)
+    o = oracle_lib.OracleDMRG(8, 7, irr, 8, 0, 0, T, V)
+    o.init(14, 4)
+    if not mr:
+        o.sweep_left_to(max(index - 1, 0), 14, rtol=1e-4)
+    new = UpdatePlan(o, index, mr).tensors()
+    ref = {(kd, k, m): d for kd, k, m, meta, d in oracle_tensors(o, index)}
+    assert set(new) == set(ref)
+    for key, b in ref.items():
+        if b.size:
+            assert np.abs(new[key] - b).max() <= 1e-12 * max(1.0, np.abs(b).max()), key
