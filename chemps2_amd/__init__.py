@@ -21,6 +21,7 @@ EXPORTS = [
     "chm_tensor_clear", "chm_heff_prepare", "chm_heff_veclen", "chm_heff_apply", "chm_heff_diag",
     "chm_davidson", "chm_flops", "chm_plan_stats", "chm_stream", "chm_timing_enable", "chm_timing_read",
     "chm_mps_upload", "chm_mps_download", "chm_tensor_download", "chm_update_operators",
+    "chm_join", "chm_s_get", "chm_s_set", "chm_split", "chm_dmrg_init", "chm_dmrg_solve", "chm_dmrg_stats",
 ]
 
 _lib = None
@@ -56,6 +57,13 @@ def load():
         "chm_mps_download": (i, [vp, i, _dp, i64]),
         "chm_tensor_download": (i, [vp, i, i, i, i, _dp, i64]),
         "chm_update_operators": (i, [vp, i, i]),
+        "chm_join": (i, [vp, i, C.POINTER(i64)]),
+        "chm_s_get": (i, [vp, vp, i64, i]),
+        "chm_s_set": (i, [vp, vp, i64, i]),
+        "chm_split": (i, [vp, i, i, i, i, C.POINTER(d)]),
+        "chm_dmrg_init": (i, [vp, i, C.c_uint64]),
+        "chm_dmrg_solve": (i, [vp, i, _ip, _dp, _ip, _dp, _dp, C.POINTER(d)]),
+        "chm_dmrg_stats": (i, [vp, C.POINTER(d), C.POINTER(i64)]),
         "chm_heff_prepare": (i, [vp, i]),
         "chm_heff_veclen": (i, [vp, C.POINTER(i64)]),
         "chm_heff_apply": (i, [vp, vp, vp, i]),
@@ -157,6 +165,46 @@ class Context:
 
     def update_operators(self, index, moving_right):
         self._check(self.lib.chm_update_operators(self.h, int(index), int(moving_right)), "chm_update_operators")
+
+    def join(self, site):
+        v = C.c_int64(0)
+        self._check(self.lib.chm_join(self.h, int(site), C.byref(v)), "chm_join")
+        self.SV = v.value
+        return v.value
+
+    def s_get(self):
+        out = np.zeros(self.SV)
+        self._check(self.lib.chm_s_get(self.h, _ptr(out), out.size, 0), "chm_s_get")
+        return out
+
+    def s_set(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        self._check(self.lib.chm_s_set(self.h, _ptr(x), x.size, 0), "chm_s_set")
+
+    def split(self, site, D, moving_right, change=True):
+        dw = C.c_double(0)
+        self._check(self.lib.chm_split(self.h, int(site), int(D), int(moving_right), int(change), C.byref(dw)), "chm_split")
+        return dw.value
+
+    def dmrg_init(self, D, seed=1):
+        self._check(self.lib.chm_dmrg_init(self.h, int(D), int(seed)), "chm_dmrg_init")
+
+    def dmrg_solve(self, scheme):
+        """scheme: list of (D, econv, max_sweeps, noise, rtol), as ConvergenceScheme."""
+        D = np.array([s[0] for s in scheme], dtype=np.int32)
+        ec = np.array([s[1] for s in scheme], dtype=np.float64)
+        ms = np.array([s[2] for s in scheme], dtype=np.int32)
+        nz = np.array([s[3] for s in scheme], dtype=np.float64)
+        rt = np.array([s[4] for s in scheme], dtype=np.float64)
+        e = C.c_double(0)
+        self._check(self.lib.chm_dmrg_solve(self.h, len(scheme), D, ec, ms, nz, rt, C.byref(e)), "chm_dmrg_solve")
+        return e.value
+
+    def dmrg_stats(self):
+        t = (C.c_double * 6)()
+        n = C.c_int64(0)
+        self._check(self.lib.chm_dmrg_stats(self.h, t, C.byref(n)), "chm_dmrg_stats")
+        return dict(zip(["join", "prepare", "davidson", "split", "update", "total"], list(t))), n.value
 
     def prepare(self, site):
         self._check(self.lib.chm_heff_prepare(self.h, int(site)), "chm_heff_prepare")

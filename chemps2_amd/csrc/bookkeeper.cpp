@@ -56,6 +56,10 @@ void auto_sectors(HostState& c, int D) {
         }
   }
   c.sec.resize(L);
+  c.fci.assign(L + 1, {});
+  for (int b = 0; b <= L; b++)
+    for (auto& kv : fci[b])
+      if (kv.second > 0) c.fci[b].push_back({std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), (int)kv.second});
   for (int b = 0; b <= L; b++) {
     long tot = 0;
     for (auto& kv : fci[b]) tot += kv.second;

@@ -223,7 +223,23 @@ struct DevDiagBlock {
   int t0, nt;
 };
 
+// dst (rows x cols, leading dimension ldd) = sum_q coef_q * src_q (rows x cols,
+// leading dimension lds) over at most two sources; used to assemble the centre-sector
+// matrices of Sobject::Split and to scatter the SVD factors back into MPS blocks.
+struct DevCopy {
+  double* dst;
+  const double* src0;
+  const double* src1;
+  const double* rowscale;  // optional per-row factor on the sources (null: 1)
+  const double* colscale;  // optional per-column factor (null: 1)
+  double c0, c1;
+  int rows, cols, ldd, lds;
+  int keep_rows, keep_cols;  // elements outside [0, keep) are written as zero
+};
+
 // Launchers (kernels.hip).
+void launch_copies(const DevCopy* items, int n, hipStream_t s);
+void launch_add_noise(double* x, int64_t n, uint64_t seed, double level, hipStream_t s);
 void launch_tiles(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const double* xbase, double* ybase,
                   hipStream_t s);
 void launch_combos(const DevCombo* d_combos, const DevSrc* d_src, int ncombo, const int64_t* d_chunk_start,

@@ -10,8 +10,11 @@
 namespace chm {
 
 void launch_fill_hash(double* p, int64_t n, uint64_t seed, double scale, hipStream_t s);
+void destroy_rocblas(void* h);
 
 Ctx::~Ctx() {
+  destroy_rocblas(rocblas);
+  if (s_buf >= 0) bufs[s_buf] = nullptr;  // owned by Sdev
   for (double* p : bufs)
     if (p) (void)hipFree(p);
   for (auto& e : ev)
@@ -32,10 +35,15 @@ void Ctx::set_problem(int L_, int group_, const int* irr, int N_, int TwoS_, int
   mx.assign(mx_, mx_ + (size_t)L * L * L * L);
   econst = econst_;
   sec.resize(L);
+  fci.clear();
+  if (s_buf >= 0) bufs[s_buf] = nullptr;  // owned by Sdev
   for (double* p : bufs)
     if (p) (void)hipFree(p);
   bufs.assign(BUF_TENSOR0, nullptr);
   buf_len.assign(BUF_TENSOR0, 0);
+  Sdev.release();
+  s_buf = -1;
+  s_site = -1;
   sets.clear();
   sets.resize(L - 1);
   mps.assign(L, TLay());
@@ -790,8 +798,8 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
     return host_scalars(1)[0];
   };
   auto lincomb = [&](const std::vector<const double*>& vs, const std::vector<double>& cs, double* out) {
-    hip_check(hipMemcpy(d_ptrs.p, vs.data(), vs.size() * sizeof(double*), hipMemcpyHostToDevice), "ptrs");
-    hip_check(hipMemcpy(d_coef.p, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice), "coefs");
+    copy_sync(stream, d_ptrs.p, vs.data(), vs.size() * sizeof(double*), hipMemcpyHostToDevice, "ptrs");
+    copy_sync(stream, d_coef.p, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice, "coefs");
     launch_lincomb_vecs(d_ptrs.p, d_coef.p, (int)vs.size(), out, n, stream);
   };
   auto matvec = [&](int i) { apply(vec[i], hv[i]); };
@@ -820,7 +828,7 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
       s ^= s << 13; s ^= s >> 7; s ^= s << 17;
       h[i] = (s >> 11) * (1.0 / 9007199254740992.0);
     }
-    hip_check(hipMemcpy(t, h.data(), n * sizeof(double), hipMemcpyHostToDevice), "random guess");
+    copy_sync(stream, t, h.data(), n * sizeof(double), hipMemcpyHostToDevice, "random guess");
   }
   add_new_vec();
   matvec(num_vec);
@@ -829,7 +837,7 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
     // Davidson.cpp:242-320
     {
       std::vector<const double*> hp(hv.begin(), hv.begin() + num_vec + 1);
-      hip_check(hipMemcpy(d_ptrs.p, hp.data(), hp.size() * sizeof(double*), hipMemcpyHostToDevice), "ptrs");
+      copy_sync(stream, d_ptrs.p, hp.data(), hp.size() * sizeof(double*), hipMemcpyHostToDevice, "ptrs");
       launch_multi_dot(d_ptrs.p, num_vec + 1, vec[num_vec], n, partial.p, scal.p, stream);
       std::vector<double> h = host_scalars(num_vec + 1);
       for (int c = 0; c <= num_vec; c++) { M[c + MAXV * num_vec] = h[c]; M[num_vec + MAXV * c] = h[c]; }
@@ -1005,7 +1013,7 @@ int chm_tensor_upload(chm_ctx* ctx, int cnt, int moving_right, int kind, int k, 
     (void)hipSetDevice(c.device);
     chm::OpT* t = c.add_tensor(cnt, moving_right != 0, kind, k, m, two_j, n_elec, n_irrep, n);
     if (t->size > 0)
-      chm::hip_check(hipMemcpy(c.bufs[t->buf], data, t->size * sizeof(double), hipMemcpyHostToDevice), "tensor upload");
+      chm::copy_sync(c.stream, c.bufs[t->buf], data, t->size * sizeof(double), hipMemcpyHostToDevice, "tensor upload");
   });
 }
 
@@ -1040,7 +1048,7 @@ int chm_mps_upload(chm_ctx* ctx, int site, const double* data, int64_t n) {
     c.mps_alloc(site);
     if (c.mps[site].size != n)
       throw Error{CHM_ERR_SIZE, "MPS size " + std::to_string(n) + " != layout size " + std::to_string(c.mps[site].size)};
-    if (n > 0) chm::hip_check(hipMemcpy(c.mps_buf[site], data, n * sizeof(double), hipMemcpyHostToDevice), "MPS upload");
+    if (n > 0) chm::copy_sync(c.stream, c.mps_buf[site], data, n * sizeof(double), hipMemcpyHostToDevice, "MPS upload");
   });
 }
 
@@ -1048,7 +1056,7 @@ int chm_mps_download(chm_ctx* ctx, int site, double* data, int64_t n) {
   return guard(ctx, [&](Ctx& c) {
     if (!c.have_problem || site < 0 || site >= c.L || c.mps[site].buf < 0) throw Error{CHM_ERR_STATE, "no MPS tensor at this site"};
     if (n != c.mps[site].size) throw Error{CHM_ERR_SIZE, "size mismatch"};
-    if (n > 0) chm::hip_check(hipMemcpy(data, c.mps_buf[site], n * sizeof(double), hipMemcpyDeviceToHost), "MPS download");
+    if (n > 0) chm::copy_sync(c.stream, data, c.mps_buf[site], n * sizeof(double), hipMemcpyDeviceToHost, "MPS download");
   });
 }
 
@@ -1057,7 +1065,7 @@ int chm_tensor_download(chm_ctx* ctx, int cnt, int kind, int k, int m, double* d
     chm::OpT* t = c.find_tensor(cnt, kind, k, m);
     if (!t) throw Error{CHM_ERR_MISSING, "no such tensor"};
     if (n != t->size) throw Error{CHM_ERR_SIZE, "size " + std::to_string(n) + " != " + std::to_string(t->size)};
-    if (n > 0) chm::hip_check(hipMemcpy(data, c.bufs[t->buf], n * sizeof(double), hipMemcpyDeviceToHost), "tensor download");
+    if (n > 0) chm::copy_sync(c.stream, data, c.bufs[t->buf], n * sizeof(double), hipMemcpyDeviceToHost, "tensor download");
   });
 }
 
@@ -1065,6 +1073,65 @@ int chm_update_operators(chm_ctx* ctx, int index, int moving_right) {
   return guard(ctx, [&](Ctx& c) {
     (void)hipSetDevice(c.device);
     c.update_operators(index, moving_right != 0);
+  });
+}
+
+int chm_dmrg_init(chm_ctx* ctx, int D, uint64_t seed) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!c.have_problem) throw Error{CHM_ERR_STATE, "set the problem first"};
+    if (D < 1) throw Error{CHM_ERR_ARG, "D must be positive"};
+    (void)hipSetDevice(c.device);
+    chm::auto_sectors(c, D);
+    c.mps_random(seed);
+    c.presolve();
+    c.n_matvec_total = 0;
+    for (double& t : c.tm) t = 0.0;
+  });
+}
+
+int chm_dmrg_solve(chm_ctx* ctx, int n, const int* D, const double* econv, const int* max_sweeps, const double* noise,
+                   const double* rtol, double* energy) {
+  return guard(ctx, [&](Ctx& c) {
+    if (n < 1 || !D || !econv || !max_sweeps || !noise || !rtol || !energy) throw Error{CHM_ERR_ARG, "bad instructions"};
+    (void)hipSetDevice(c.device);
+    *energy = c.dmrg_solve(n, D, econv, max_sweeps, noise, rtol);
+  });
+}
+
+int chm_dmrg_stats(const chm_ctx* ctx, double* seconds, int64_t* n_matvec) {
+  if (!ctx || !seconds || !n_matvec) return CHM_ERR_ARG;
+  for (int q = 0; q < 6; q++) seconds[q] = ctx->c.tm[q];
+  *n_matvec = ctx->c.n_matvec_total;
+  return CHM_OK;
+}
+
+int chm_join(chm_ctx* ctx, int site, int64_t* V) {
+  return guard(ctx, [&](Ctx& c) {
+    (void)hipSetDevice(c.device);
+    c.join(site);
+    if (V) *V = (int64_t)c.buf_len[c.s_buf];
+  });
+}
+
+int chm_s_get(chm_ctx* ctx, double* x, int64_t n, int on_device) {
+  return guard(ctx, [&](Ctx& c) {
+    if (c.s_site < 0 || n != c.buf_len[c.s_buf]) throw Error{CHM_ERR_STATE, "no two-site object of this size"};
+    chm::copy_sync(c.stream, x, c.Sdev.p, n * sizeof(double), on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, "S get");
+  });
+}
+
+int chm_s_set(chm_ctx* ctx, const double* x, int64_t n, int on_device) {
+  return guard(ctx, [&](Ctx& c) {
+    if (c.s_site < 0 || n != c.buf_len[c.s_buf]) throw Error{CHM_ERR_STATE, "no two-site object of this size"};
+    chm::copy_sync(c.stream, c.Sdev.p, x, n * sizeof(double), on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, "S set");
+  });
+}
+
+int chm_split(chm_ctx* ctx, int site, int D, int moving_right, int change, double* disc_weight) {
+  return guard(ctx, [&](Ctx& c) {
+    (void)hipSetDevice(c.device);
+    const double d = c.split(site, D, moving_right != 0, change != 0);
+    if (disc_weight) *disc_weight = d;
   });
 }
 
@@ -1107,7 +1174,7 @@ int chm_heff_diag(chm_ctx* ctx, double* d, int on_device) {
       c.diag(d);
     } else {
       c.diag(c.dy.p);
-      chm::hip_check(hipMemcpy(d, c.dy.p, c.V * sizeof(double), hipMemcpyDeviceToHost), "diag download");
+      chm::copy_sync(c.stream, d, c.dy.p, c.V * sizeof(double), hipMemcpyDeviceToHost, "diag download");
     }
   });
 }
@@ -1123,9 +1190,9 @@ int chm_davidson(chm_ctx* ctx, double* x, int on_device, double rtol, double* ei
     } else {
       chm::DBuf<double> buf;
       buf.alloc(c.V);
-      chm::hip_check(hipMemcpy(buf.p, x, c.V * sizeof(double), hipMemcpyHostToDevice), "x upload");
+      chm::copy_sync(c.stream, buf.p, x, c.V * sizeof(double), hipMemcpyHostToDevice, "x upload");
       *eigenvalue = c.davidson(buf.p, rtol, n_matvec);
-      chm::hip_check(hipMemcpy(x, buf.p, c.V * sizeof(double), hipMemcpyDeviceToHost), "x download");
+      chm::copy_sync(c.stream, x, buf.p, c.V * sizeof(double), hipMemcpyDeviceToHost, "x download");
     }
   });
 }

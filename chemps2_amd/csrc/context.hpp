@@ -50,6 +50,14 @@ struct DBuf {
   }
 };
 
+// hipMemcpy after the context's stream has drained: the stream does not synchronise
+// with the null stream, so a plain hipMemcpy could overtake kernels still using the
+// destination or the source.
+inline void copy_sync(hipStream_t s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind, const char* what) {
+  hip_check(hipStreamSynchronize(s), what);
+  hip_check(hipMemcpy(dst, src, bytes, kind), what);
+}
+
 // Host-side state: the problem, the sector dimensions and the operator layouts.
 struct HostState {
   // problem (DMRG orbital order)
@@ -60,6 +68,7 @@ struct HostState {
   bool have_problem = false;
   Sectors sec;
   std::vector<OpSet> sets;  // slot cnt -> operator set
+  std::vector<std::vector<Sectors::Sec>> fci;  // [b] FCI dimensions (> 0), when known
   // Throws unless slot cnt holds every operator DMRG::allocateTensors creates there.
   void validate_set(int cnt, bool mr) const;
 };
@@ -112,6 +121,22 @@ struct Ctx : HostState {
   std::vector<double*> mps_buf;
   double last_update_flops = 0.0;
   int64_t last_update_terms = 0;
+
+  // two-site object on the device (program convention) and the sweep driver
+  DBuf<double> Sdev;
+  int32_t s_buf = -1;
+  int s_site = -1;
+  void* rocblas = nullptr;  // rocblas_handle for the SVDs of Split
+  double total_min = 1e8, max_disc_last = 0.0, last_min = 1e8;
+  double tm[6] = {0, 0, 0, 0, 0, 0};  // seconds: join, prepare, davidson, split, update, total
+  int64_t n_matvec_total = 0;
+  uint64_t noise_seed = 1;
+  void join(int site);
+  double split(int site, int D, bool moving_right, bool change);
+  void mps_random(uint64_t seed);
+  void presolve();
+  double solve_site(int index, double rtol, double noise, int D, bool moving_right, bool change);
+  double dmrg_solve(int n, const int* D, const double* econv, const int* max_sweeps, const double* noise, const double* rtol);
 
   ~Ctx();
   void mps_alloc(int site);

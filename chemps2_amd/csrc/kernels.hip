@@ -346,6 +346,35 @@ __global__ __launch_bounds__(256) void k_fill_hash(double* __restrict__ p, int64
   }
 }
 
+// Scaled block copies with up to two sources (one workgroup per item).
+__global__ __launch_bounds__(256) void k_copies(const DevCopy* __restrict__ items) {
+  const DevCopy it = items[blockIdx.x];
+  const int n = it.rows * it.cols;
+  for (int e = threadIdx.x; e < n; e += 256) {
+    const int r = e % it.rows, c = e / it.rows;
+    double v = 0.0;
+    if (r < it.keep_rows && c < it.keep_cols) {
+      const size_t si = (size_t)r + (size_t)it.lds * c;
+      v = it.c0 * it.src0[si];
+      if (it.src1) v += it.c1 * it.src1[si];
+      if (it.rowscale) v *= it.rowscale[r];
+      if (it.colscale) v *= it.colscale[c];
+    }
+    it.dst[(size_t)r + (size_t)it.ldd * c] = v;
+  }
+}
+
+// x += level * (u - 1/2) with u uniform in [0, 1) from a splitmix64 hash of the index.
+__global__ __launch_bounds__(256) void k_add_noise(double* __restrict__ x, int64_t n, uint64_t seed, double level) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    uint64_t z = seed * 0x9E3779B97F4A7C15ULL + (uint64_t)i;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    x[i] += level * ((z >> 11) * (1.0 / 9007199254740992.0) - 0.5);
+  }
+}
+
 inline unsigned grid_for(int64_t n) {
   int64_t g = (n + 255) / 256;
   if (g > 8192) g = 8192;
@@ -430,6 +459,14 @@ void launch_precond(const double* diag, const double* u, double shift, double cu
 
 void launch_fill_hash(double* p, int64_t n, uint64_t seed, double scale, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_hash, dim3(grid_for(n)), dim3(256), 0, s, p, n, seed, scale);
+}
+
+void launch_copies(const DevCopy* items, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_copies, dim3(n), dim3(256), 0, s, items);
+}
+
+void launch_add_noise(double* x, int64_t n, uint64_t seed, double level, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_add_noise, dim3(grid_for(n)), dim3(256), 0, s, x, n, seed, level);
 }
 
 void launch_symm_scale(double* x, const int64_t* d_k2i, const double* d_scale, int nk, hipStream_t s) {

@@ -80,6 +80,24 @@ def mx_table(T, V, N):
     return np.ascontiguousarray(mx.transpose(3, 2, 1, 0)).ravel()
 
 
+def reorder_d2h(irreps, group=7):
+    """DMRG orbital order for D2h: irreps grouped as Ag B1u B3u B2g B2u B3g B1g Au
+    (Problem::SetupReorderD2h, Problem.cpp:57-90). Returns f2 (DMRG -> Hamiltonian index);
+    the identity for other groups."""
+    L = len(irreps)
+    if group != 7:
+        return list(range(L))
+    order = [0, 5, 7, 2, 6, 3, 1, 4]
+    return [h for o in order for h in range(L) if irreps[h] == o]
+
+
+def permute(irreps, T, V, f2):
+    """Irreps, T and V in the orbital order f2 (new index -> old index)."""
+    f2 = np.asarray(f2)
+    irr = np.asarray(irreps)[f2]
+    return irr, np.asarray(T)[np.ix_(f2, f2)], np.asarray(V)[np.ix_(f2, f2, f2, f2)]
+
+
 def read_fcidump(path):
     """Reads an FCIDUMP file (Hamiltonian.cpp:296-423). Returns a dict with L, N, TwoS,
     irreps (XOR labels, D2h subgroup given by ``group``), T, V (physicist), econst."""

@@ -95,6 +95,30 @@ int chm_tensor_download(chm_ctx* ctx, int cnt, int kind, int k, int m, double* d
  * TensorOperator::update). */
 int chm_update_operators(chm_ctx* ctx, int index, int moving_right);
 
+/* Sobject::Join (Sobject.cpp:212-258) of the device MPS tensors of site and site+1
+ * into the context's two-site object (program convention); V receives its length. */
+int chm_join(chm_ctx* ctx, int site, int64_t* V);
+/* Read / overwrite the context's two-site object (program convention). */
+int chm_s_get(chm_ctx* ctx, double* x, int64_t n, int on_device);
+int chm_s_set(chm_ctx* ctx, const double* x, int64_t n, int on_device);
+/* Sobject::Split (Sobject.cpp:260-622) of the two-site object into the MPS tensors of
+ * site and site+1 with at most D states (change = 0 keeps the dimensions); needs the
+ * FCI dimensions of chm_set_sectors_auto. The discarded weight goes to *disc_weight. */
+int chm_split(chm_ctx* ctx, int site, int D, int moving_right, int change, double* disc_weight);
+
+/* The whole DMRG on the device, mirroring CheMPS2::DMRG (DMRG.h:93-103): init =
+ * SyBookkeeper(Prob, D) dimensions, a random left-normalised MPS from `seed`
+ * (TensorT::random + left_normalize) and PreSolve (DMRG.cpp:255-265); solve = the
+ * ConvergenceScheme instructions (D, energy tolerance, maximum sweeps, noise
+ * prefactor, Davidson tolerance) swept as DMRG::Solve (DMRG.cpp:268-417); energy =
+ * the lowest energy met (TotalMinEnergy, with the constant of the Hamiltonian). */
+int chm_dmrg_init(chm_ctx* ctx, int D, uint64_t seed);
+int chm_dmrg_solve(chm_ctx* ctx, int n, const int* D, const double* econv, const int* max_sweeps, const double* noise,
+                   const double* rtol, double* energy);
+/* seconds[6]: join, Heff preparation, Davidson, split, operator updates, whole solve;
+ * n_matvec: Heff applications. */
+int chm_dmrg_stats(const chm_ctx* ctx, double* seconds, int64_t* n_matvec);
+
 /* Two-site step at (site, site+1): builds the work lists of Heff (the reference's
  * Heff(denBK, Prob, rtol) + Sobject(site, denBK), Heff.h:50, Sobject.cpp:36-147).
  * Needs slot site-1 built moving right (site > 0) and slot site+1 moving left

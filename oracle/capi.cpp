@@ -94,6 +94,16 @@ double orc_sweep_left_to(void* p, int stop, int D, double rtol, int change) {
   return E;
 }
 
+// Sobject::Split of the given two-site data (program convention) into the MPS of
+// index, index+1; returns the discarded weight.
+double orc_split_test(void* p, int index, int D, int mr, int change, const double* S_prog) {
+  DMRG& dm = ((Handle*)p)->dm;
+  Sobject S;
+  S.init(index, &dm.bk);
+  std::copy(S_prog, S_prog + S.size(), S.data.begin());
+  return S.split(&dm.mps[index], &dm.mps[index + 1], D, mr != 0, change != 0);
+}
+
 // One left-moving sweep step at `index` (solve, split, update moving left).
 double orc_step_left(void* p, int index, int D, double rtol, int change) {
   DMRG& dm = ((Handle*)p)->dm;

@@ -43,6 +43,7 @@ def lib():
             "orc_step_left": (d, [vp, i, i, d, i]),
             "orc_synthetic": (L_, [vp, i, i, C.c_ulonglong, C.c_ulonglong]),
             "orc_bookkeeper": (None, [vp, i]),
+            "orc_split_test": (d, [vp, i, i, i, i, _dp]),
             "orc_threads": (i, []),
             "orc_site_apply_range": (d, [vp, _dp, _dp, i, i, C.POINTER(C.c_double)]),
             "orc_site": (L_, [vp, i]),
@@ -114,6 +115,10 @@ class OracleDMRG:
         self.index = site
         self.V = int(self.lib.orc_synthetic(self.h, int(D), int(site), int(seed_l), int(seed_r)))
         return self.V
+
+    def split_test(self, index, D, moving_right, change, s_prog):
+        return self.lib.orc_split_test(self.h, int(index), int(D), int(moving_right), int(change),
+                                       np.ascontiguousarray(s_prog, dtype=np.float64))
 
     def bookkeeper(self, D):
         self.lib.orc_bookkeeper(self.h, int(D))

@@ -110,3 +110,16 @@ def test_update_plan_matches_oracle(oracle_lib, index, mr):
     for key, b in ref.items():
         if b.size:
             assert np.abs(new[key] - b).max() <= 1e-12 * max(1.0, np.abs(b).max()), key
+
+
+def test_reorder_d2h_matches_oracle(oracle_lib):
+    from chemps2_amd.problems import permute, reorder_d2h
+
+    irr, T, V = random_abinitio(9, 6, group=7)
+    o = oracle_lib.OracleDMRG(9, 7, irr, 8, 0, 0, T, V, reorder=True)
+    o.bookkeeper(20)
+    irr2, T2, V2 = permute(irr, T, V, reorder_d2h(list(irr)))
+    from chemps2_amd.problems import mx_table
+
+    assert np.array_equal(o.irreps(), irr2)
+    assert np.abs(o.mx() - mx_table(T2, V2, 8)).max() < 1e-14
