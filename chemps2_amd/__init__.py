@@ -21,7 +21,7 @@ EXPORTS = [
     "chm_tensor_clear", "chm_heff_prepare", "chm_heff_veclen", "chm_heff_apply", "chm_heff_diag",
     "chm_davidson", "chm_flops", "chm_plan_stats", "chm_stream", "chm_timing_enable", "chm_timing_read",
     "chm_mps_upload", "chm_mps_download", "chm_tensor_download", "chm_update_operators",
-    "chm_join", "chm_s_get", "chm_s_set", "chm_split", "chm_dmrg_init", "chm_dmrg_solve", "chm_dmrg_stats",
+    "chm_join", "chm_s_get", "chm_s_set", "chm_split", "chm_dmrg_init", "chm_dmrg_solve", "chm_dmrg_sweep", "chm_dmrg_stats",
 ]
 
 _lib = None
@@ -63,6 +63,7 @@ def load():
         "chm_split": (i, [vp, i, i, i, i, C.POINTER(d)]),
         "chm_dmrg_init": (i, [vp, i, C.c_uint64]),
         "chm_dmrg_solve": (i, [vp, i, _ip, _dp, _ip, _dp, _dp, C.POINTER(d)]),
+        "chm_dmrg_sweep": (i, [vp, i, i, d, d, i, C.POINTER(d)]),
         "chm_dmrg_stats": (i, [vp, C.POINTER(d), C.POINTER(i64)]),
         "chm_heff_prepare": (i, [vp, i]),
         "chm_heff_veclen": (i, [vp, C.POINTER(i64)]),
@@ -198,6 +199,13 @@ class Context:
         rt = np.array([s[4] for s in scheme], dtype=np.float64)
         e = C.c_double(0)
         self._check(self.lib.chm_dmrg_solve(self.h, len(scheme), D, ec, ms, nz, rt, C.byref(e)), "chm_dmrg_solve")
+        return e.value
+
+    def dmrg_sweep(self, moving_right, D, rtol=1e-5, noise=0.0, change=True):
+        """One half sweep (DMRG::sweepleft / sweepright); returns the last site's energy."""
+        e = C.c_double(0)
+        self._check(self.lib.chm_dmrg_sweep(self.h, int(moving_right), int(D), float(rtol), float(noise), int(change),
+                                            C.byref(e)), "chm_dmrg_sweep")
         return e.value
 
     def dmrg_stats(self):

@@ -64,12 +64,13 @@ void auto_sectors(HostState& c, int D) {
     long tot = 0;
     for (auto& kv : fci[b]) tot += kv.second;
     const double factor = (b >= 1 && b <= L - 1 && tot > D) ? (1.0 * D) / tot : 1.0;
+    std::vector<Sectors::Sec> v;
     for (auto& kv : fci[b]) {  // std::map order == (N, 2S, I) ascending
       if (kv.second <= 0) continue;
       const int d = (factor < 1.0) ? (int)(std::ceil(factor * kv.second) + 0.1) : (int)kv.second;
-      c.sec.list[b].push_back({std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), d});
-      c.sec.dimmap[b][key3(std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first))] = d;
+      v.push_back({std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), d});
     }
+    c.sec.set(b, std::move(v));
   }
 }
 

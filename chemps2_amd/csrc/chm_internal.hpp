@@ -38,17 +38,32 @@ double wigner9j(int a, int b, int c, int d, int e, int f, int g, int h, int i);
 
 inline uint64_t key3(int n, int s, int i) { return ((uint64_t)(n & 0xffff)) | ((uint64_t)(s & 0xffff) << 16) | ((uint64_t)(i & 0xff) << 32); }
 
+// Dense (N, 2S, I) -> position lookup of one boundary's sector list; immutable once
+// built, shared by the layouts made from it.
+struct SecIndex {
+  int nmax = -1, smax = -1;
+  std::vector<int32_t> pos;
+  int find(int N, int S, int I) const {
+    if (N < 0 || S < 0 || N > nmax || S > smax || I < 0 || I > 7) return -1;
+    return pos[((size_t)N * (smax + 1) + S) * 8 + I];
+  }
+};
+
 // Current virtual dimensions d(b, N, 2S, I) of every boundary b = 0..L
 // (SyBookkeeper::gCurrentDim, SyBookkeeper.cpp:165).
 struct Sectors {
   struct Sec { int N, S, I, d; };
-  std::vector<std::vector<Sec>> list;                        // [b], sorted by (N, 2S, I)
-  std::vector<std::unordered_map<uint64_t, int>> dimmap;     // [b] -> d
-  void resize(int L) { list.assign(L + 1, {}); dimmap.assign(L + 1, {}); }
+  std::vector<std::vector<Sec>> list;                   // [b], sorted by (N, 2S, I), d > 0
+  std::vector<std::shared_ptr<const SecIndex>> idx;     // [b]
+  void resize(int L) {
+    list.assign(L + 1, {});
+    idx.assign(L + 1, std::make_shared<const SecIndex>());
+  }
+  void set(int b, std::vector<Sec> v);  // replaces boundary b (entries with d <= 0 dropped)
   int dim(int b, int N, int S, int I) const {
-    if (b < 0 || b >= (int)dimmap.size() || N < 0 || S < 0) return 0;
-    auto it = dimmap[b].find(key3(N, S, I));
-    return it == dimmap[b].end() ? 0 : it->second;
+    if (b < 0 || b >= (int)idx.size()) return 0;
+    const int p = idx[b]->find(N, S, I);
+    return p < 0 ? 0 : list[b][p].d;
   }
 };
 
@@ -70,14 +85,14 @@ struct OpT {
   bool moving_right = true, prime_last = true, jw_phase = false;
   int32_t buf = -1;  // buffer id of its device copy
   int64_t size = 0;
-  std::unordered_map<uint64_t, int64_t> off;  // key (n, s, i, s') -> element offset
-  std::vector<int> Nup, Sup, Iup, Sdn;         // blocks in storage order
+  std::shared_ptr<const SecIndex> bi;  // sector index of the boundary it was laid out on
+  std::vector<int64_t> offs;           // [up position * (two_j + 1) + (s' - s + two_j) / 2] -> offset or -1
+  std::vector<int> Nup, Sup, Iup, Sdn;  // blocks in storage order
   std::vector<int64_t> k2i;
   void build(const Sectors& sec, int b, int two_j, int n_elec, int n_irrep);
   Ref blk(int n1, int s1, int i1, int n2, int s2, int i2) const;
   Ref blkref(int ik) const { Ref r; r.buf = buf; r.off = k2i[ik]; return r; }
   int nk() const { return (int)Nup.size(); }
-  static uint64_t key(int n, int s, int i, int s2) { return key3(n, s, i) | ((uint64_t)(s2 & 0xffff) << 40); }
 };
 
 // Block layout of an MPS site tensor (TensorT.cpp:38-104): blocks (NL, 2SL, IL) x
@@ -110,6 +125,7 @@ struct OpSet {
 struct SLayout {
   int index = 0;
   std::vector<int> NL, SL, IL, N1, N2, TJ, NR, SR, IR;
+  std::vector<int> dL, dR;  // block dimensions
   std::vector<int64_t> k2i;
   std::unordered_map<uint64_t, int> lut;
   void build(const Sectors& sec, int index, int I1, int I2);

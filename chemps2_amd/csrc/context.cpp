@@ -1,11 +1,14 @@
 // Context, device work-list construction, the Davidson driver and the C ABI.
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
 
 #include "../../include/chemps2_amd.h"
 #include "context.hpp"
+#include "parallel.hpp"
 
 namespace chm {
 
@@ -65,9 +68,7 @@ void Ctx::set_sectors(int b, int n, const int* Ns, const int* Ss, const int* Is,
   std::sort(v.begin(), v.end(), [](const Sectors::Sec& a, const Sectors::Sec& b2) {
     return std::make_tuple(a.N, a.S, a.I) < std::make_tuple(b2.N, b2.S, b2.I);
   });
-  sec.list[b] = v;
-  sec.dimmap[b].clear();
-  for (auto& s : v) sec.dimmap[b][key3(s.N, s.S, s.I)] = s.d;
+  sec.set(b, std::move(v));
   site = -1;
 }
 
@@ -149,7 +150,12 @@ const double* Ctx::resolve(const Ref& r) const {
   throw Error{ERR_MISSING, "unresolvable block reference"};
 }
 
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 void Ctx::prepare(int st) {
+  const double t_start = now_s();
   if (!have_problem) throw Error{ERR_STATE, "set the problem first"};
   if (st < 0 || st > L - 2) throw Error{ERR_ARG, "site out of range"};
   site = -1;
@@ -164,6 +170,7 @@ void Ctx::prepare(int st) {
   plan->init(&sec, &irreps, nirrep, mx.data(), L, &S, Lo, Ro);
   plan->build();
   F_H = plan->flops;
+  const double t_built = now_s();
 
   // ---- x-independent combinations, evaluated now
   combo_arena.alloc(std::max<int64_t>(plan->combo_size, 1));
@@ -196,92 +203,136 @@ void Ctx::prepare(int st) {
     }
   }
 
+  const double t_plan = now_s();
   // ---- split the terms: (kappa, kappa') pairs small enough are assembled into an
   // explicit block-sparse matrix once per site; the rest run as chained GEMM tiles.
-  auto dLof = [&](int k) { return (int)(S.k2i[k + 1] - S.k2i[k]) / std::max(1, sec.dim(st + 2, S.NR[k], S.SR[k], S.IR[k])); };
-  auto dRof = [&](int k) { return sec.dim(st + 2, S.NR[k], S.SR[k], S.IR[k]); };
+  // Passes over the planner's chunks (output blocks [c * CH, (c + 1) * CH)) run on the
+  // host threads; every output position comes from prefix sums over the chunks, so the
+  // lists do not depend on the thread count.
+  auto dLof = [&](int k) { return S.dL[k]; };
+  auto dRof = [&](int k) { return S.dR[k]; };
   auto bsize = [&](int k) { return (int64_t)(S.k2i[k + 1] - S.k2i[k]); };
-  const std::vector<Term>& terms = plan->terms;
-  std::vector<int64_t> order(terms.size());
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
-    return terms[a].out != terms[b].out ? terms[a].out < terms[b].out : terms[a].in < terms[b].in;
+  const int nthr = host_threads();
+  const std::vector<std::vector<Term>>& chunks = plan->chunks;
+  const int64_t nch = (int64_t)chunks.size();
+  const int CH = HeffPlanner::CHUNK;
+  const int nk = S.nk();
+  struct Grp { int out, in; int64_t q0, q1, elems; };
+  constexpr int NTHR = 64;
+  std::vector<std::vector<Grp>> grp(nch);
+  std::vector<std::array<double, NTHR>> hist(nch);
+  std::vector<std::vector<char>> seen_t(nthr, std::vector<char>(bufs.size(), 0));
+  parallel_for(nch, nthr, [&](int64_t c, int tid) {
+    auto& g = grp[c];
+    const auto& tv = chunks[c];
+    std::vector<char>& seen = seen_t[tid];
+    for (int64_t q = 0; q < (int64_t)tv.size(); q++) {
+      const Term& t = tv[q];
+      if (t.Lb.buf >= BUF_TENSOR0) seen[t.Lb.buf] = 1;
+      if (t.Rb.buf >= BUF_TENSOR0) seen[t.Rb.buf] = 1;
+      if (g.empty() || g.back().out != t.out || g.back().in != t.in) g.push_back({t.out, t.in, q, q, bsize(t.out) * bsize(t.in)});
+      g.back().q1 = q + 1;
+    }
+    auto& h = hist[c];
+    h.fill(0.0);
+    for (const Grp& x : g)
+      for (int k = 0; k < NTHR; k++) {
+        if ((explicit_max >> k) < x.elems) break;
+        h[k] += (double)x.elems;
+      }
   });
-  struct HP { int out, in; int64_t q0, q1, elems; };
-  std::vector<HP> hp;
-  for (int64_t i = 0; i < (int64_t)order.size(); i++) {
-    const Term& t = terms[order[i]];
-    if (hp.empty() || hp.back().out != t.out || hp.back().in != t.in) hp.push_back({t.out, t.in, i, i, bsize(t.out) * bsize(t.in)});
-    hp.back().q1 = i + 1;
-  }
-  int64_t thr = explicit_max;
+  int64_t thr = 0;
   {
     size_t freeb = 0, totb = 0;
     hip_check(hipMemGetInfo(&freeb, &totb), "hipMemGetInfo");
-    const double budget = explicit_budget_gb > 0 ? explicit_budget_gb * 1e9 : 0.4 * (double)freeb;
-    while (thr > 0) {
+    const double avail = (double)freeb + 8.0 * (double)Mvals.n;  // the matrix of the last site is reused
+    const double budget = explicit_budget_gb > 0 ? explicit_budget_gb * 1e9 : 0.4 * avail;
+    for (int k = 0; k < NTHR && (explicit_max >> k) > 0; k++) {
       double nnz = 0.0;
-      for (const HP& h : hp) if (h.elems <= thr) nnz += (double)h.elems;
-      if (nnz * 8.0 <= budget) break;
-      thr /= 2;
+      for (int64_t c = 0; c < nch; c++) nnz += hist[c][k];
+      if (nnz * 8.0 <= budget) { thr = explicit_max >> k; break; }
     }
   }
   explicit_threshold = thr;
-  std::vector<DevRowBlock> rb(S.nk());
-  std::vector<int> colidx;
-  std::vector<DevPair> dp;
-  std::vector<DevKTerm> kt;
-  std::vector<char> chain(terms.size(), 0);
-  {
-    int64_t moff = 0;
+  // counts per chunk, then offsets
+  struct Cnt { int64_t pairs = 0, kt = 0, cols = 0, m = 0, chain = 0; };
+  std::vector<Cnt> cnt(nch + 1);
+  parallel_for(nch, nthr, [&](int64_t c, int) {
+    Cnt n;
+    for (const Grp& x : grp[c]) {
+      if (x.elems > thr) { n.chain += x.q1 - x.q0; continue; }
+      n.pairs++;
+      n.kt += x.q1 - x.q0;
+      n.cols += bsize(x.in);
+      n.m += (int64_t)bsize(x.out) * bsize(x.in);
+    }
+    cnt[c + 1] = n;
+  });
+  for (int64_t c = 0; c < nch; c++) {
+    cnt[c + 1].pairs += cnt[c].pairs; cnt[c + 1].kt += cnt[c].kt; cnt[c + 1].cols += cnt[c].cols;
+    cnt[c + 1].m += cnt[c].m; cnt[c + 1].chain += cnt[c].chain;
+  }
+  const Cnt tot = cnt[nch];
+  if (tot.kt > INT32_MAX) throw Error{ERR_SIZE, "too many explicit terms"};
+  stage.reset();
+  stage.reserve(256 * 4 + tot.pairs * sizeof(DevPair) + tot.kt * sizeof(DevKTerm) + std::max<int64_t>(tot.cols, 1) * sizeof(int));
+  DevPair* h_dp = stage.take<DevPair>(tot.pairs);
+  DevKTerm* h_kt = stage.take<DevKTerm>(tot.kt);
+  int* h_col = stage.take<int>(std::max<int64_t>(tot.cols, 1));
+  h_col[0] = 0;
+  std::vector<DevRowBlock> rb(nk);
+  std::vector<const Term*> chain(tot.chain);
+  parallel_for(nch, nthr, [&](int64_t c, int) {
+    int64_t ip = cnt[c].pairs, ik = cnt[c].kt, ic = cnt[c].cols, im = cnt[c].m, ich = cnt[c].chain;
+    const auto& tv = chunks[c];
+    const auto& g = grp[c];
     size_t h = 0;
-    for (int k = 0; k < S.nk(); k++) {
+    for (int k = (int)(c * CH); k < std::min<int>(nk, (int)((c + 1) * CH)); k++) {
       const int R = (int)bsize(k);
-      size_t h0 = h;
-      while (h < hp.size() && hp[h].out == k) h++;
-      int C = 0;
-      for (size_t j = h0; j < h; j++) if (hp[j].elems <= thr) C += (int)bsize(hp[j].in);
-      rb[k].moff = moff;
-      rb[k].coff = (int64_t)colidx.size();
+      rb[k].moff = im;
+      rb[k].coff = ic;
       rb[k].yoff = S.k2i[k];
       rb[k].R = R;
-      rb[k].C = C;
       int cpos = 0;
-      for (size_t j = h0; j < h; j++) {
-        const HP& P = hp[j];
-        if (P.elems > thr) {
-          for (int64_t i = P.q0; i < P.q1; i++) chain[order[i]] = 1;
+      for (; h < g.size() && g[h].out == k; h++) {
+        const Grp& x = g[h];
+        if (x.elems > thr) {
+          for (int64_t q = x.q0; q < x.q1; q++) chain[ich++] = &tv[q];
           continue;
         }
         DevPair pr{};
-        pr.moff = moff + (int64_t)R * cpos;
-        pr.R = R; pr.dL = dLof(k); pr.dR = dRof(k); pr.dLi = dLof(P.in); pr.dRi = dRof(P.in);
-        pr.t0 = (int)kt.size();
-        for (int64_t i = P.q0; i < P.q1; i++) {
-          const Term& t = terms[order[i]];
+        pr.moff = im + (int64_t)R * cpos;
+        pr.R = R; pr.dL = dLof(k); pr.dR = dRof(k); pr.dLi = dLof(x.in); pr.dRi = dRof(x.in);
+        pr.t0 = (int)ik;
+        for (int64_t q = x.q0; q < x.q1; q++) {
+          const Term& t = tv[q];
           DevKTerm e{};
           e.alpha = t.alpha;
           if (t.tl != 'I') { e.A = resolve(t.Lb); e.lda = (t.tl == 'N') ? pr.dL : pr.dLi; e.ta = (t.tl == 'T'); }
           if (t.tr != 'I') { e.B = resolve(t.Rb); e.ldb = (t.tr == 'N') ? pr.dRi : pr.dR; e.tb = (t.tr == 'T'); }
-          kt.push_back(e);
+          h_kt[ik++] = e;
         }
-        pr.nt = (int)kt.size() - pr.t0;
-        dp.push_back(pr);
-        const int64_t base = S.k2i[P.in];
-        for (int c = 0; c < (int)bsize(P.in); c++) colidx.push_back((int)(base + c));
-        cpos += (int)bsize(P.in);
+        pr.nt = (int)(ik - pr.t0);
+        h_dp[ip++] = pr;
+        const int64_t base = S.k2i[x.in];
+        for (int q = 0; q < (int)bsize(x.in); q++) h_col[ic++] = (int)(base + q);
+        cpos += (int)bsize(x.in);
       }
-      moff += (int64_t)R * C;
+      rb[k].C = cpos;
+      im += (int64_t)R * cpos;
     }
-    if ((int64_t)kt.size() > INT32_MAX) throw Error{ERR_SIZE, "too many explicit terms"};
-    Mvals.alloc(std::max<int64_t>(moff, 1));
-    Mcol.upload(colidx.empty() ? std::vector<int>{0} : colidx, stream);
+  });
+  const double t_lists = now_s();
+  {
+    Mvals.alloc(std::max<int64_t>(tot.m, 1));
+    Mcol.alloc(std::max<int64_t>(tot.cols, 1));
+    hip_check(hipMemcpyAsync(Mcol.p, h_col, std::max<int64_t>(tot.cols, 1) * sizeof(int), hipMemcpyHostToDevice, stream), "columns");
     // work items: rows in chunks of at most 256, columns in chunks of about
     // item_elems elements, so every workgroup streams a similar share of the matrix
     std::vector<DevBItem> items;
     std::vector<DevBRed> reds;
     int64_t poff = 0;
-    for (int k = 0; k < S.nk(); k++) {
+    for (int k = 0; k < nk; k++) {
       const DevRowBlock& B = rb[k];
       for (int r0 = 0; r0 < B.R; r0 += 256) {
         const int R = std::min(256, B.R - r0);
@@ -310,33 +361,33 @@ void Ctx::prepare(int st) {
     nbitems = (int)items.size();
     nbreds = (int)reds.size();
     bpart.alloc(std::max<int64_t>(poff, 1));
-    if (!dp.empty()) {
-      DBuf<DevPair> d_dp;
-      DBuf<DevKTerm> d_kt;
-      d_dp.upload(dp, stream);
-      d_kt.upload(kt, stream);
-      for (auto& pr : dp) (void)pr;
-      launch_assemble(d_dp.p, d_kt.p, (int64_t)dp.size(), Mvals.p, stream);
+    if (tot.pairs > 0) {
+      d_pairs.alloc(tot.pairs);
+      d_kterms.alloc(tot.kt);
+      hip_check(hipMemcpyAsync(d_pairs.p, h_dp, tot.pairs * sizeof(DevPair), hipMemcpyHostToDevice, stream), "pairs");
+      hip_check(hipMemcpyAsync(d_kterms.p, h_kt, tot.kt * sizeof(DevKTerm), hipMemcpyHostToDevice, stream), "terms");
+      launch_assemble(d_pairs.p, d_kterms.p, tot.pairs, Mvals.p, stream);
       hip_check(hipGetLastError(), "assembly kernel launch");
-      hip_check(hipStreamSynchronize(stream), "assembly");
     }
-    stats[11] = moff;
-    stats[12] = (int64_t)colidx.size();
+    hip_check(hipStreamSynchronize(stream), "assembly");
+    stats[11] = tot.m;
+    stats[12] = tot.cols;
     stats[13] = thr;
     stats[14] = (int64_t)nbitems;
   }
+  const double t_assemble = now_s();
 
-  // ---- chained GEMM tiles for the remaining terms
+  // ---- chained GEMM tiles for the remaining terms (in (out, in) order)
   std::vector<DevGemmOp> o1, o2;
   std::vector<DevTile> t1, t2;
   int64_t Wsize = 0;
   struct WRec { int64_t off; int rows, cols; };
-  std::vector<WRec> wrec(terms.size(), {-1, 0, 0});
+  std::vector<WRec> wrec(chain.size(), {-1, 0, 0});
   auto rel_x = [&](int k) { return (const double*)(intptr_t)S.k2i[k]; };
   // phase 1: one group per two-operand term
-  for (size_t q = 0; q < terms.size(); q++) {
-    const Term& t = terms[q];
-    if (!chain[q] || t.tl == 'I' || t.tr == 'I') continue;
+  for (size_t q = 0; q < chain.size(); q++) {
+    const Term& t = *chain[q];
+    if (t.tl == 'I' || t.tr == 'I') continue;
     const int dL = dLof(t.out), dR = dRof(t.out), dLi = dLof(t.in), dRi = dRof(t.in);
     DevGemmOp op{};
     WRec w;
@@ -366,15 +417,14 @@ void Ctx::prepare(int st) {
   W.alloc(std::max<int64_t>(Wsize, 1));
   for (auto& tl : t1) tl.C = W.p + (intptr_t)tl.C;
   // phase 2: one group per output block, added to the explicit product
-  std::vector<std::vector<int64_t>> by_out(S.nk());
-  for (size_t q = 0; q < terms.size(); q++)
-    if (chain[q]) by_out[terms[q].out].push_back((int64_t)q);
-  for (int k = 0; k < S.nk(); k++) {
-    if (by_out[k].empty()) continue;
+  for (size_t q0 = 0; q0 < chain.size();) {
+    const int k = chain[q0]->out;
+    size_t q1 = q0;
+    while (q1 < chain.size() && chain[q1]->out == k) q1++;
     const int dL = dLof(k), dR = dRof(k);
     const int op0 = (int)o2.size();
-    for (int64_t q : by_out[k]) {
-      const Term& t = terms[q];
+    for (size_t q = q0; q < q1; q++) {
+      const Term& t = *chain[q];
       const int dLi = dLof(t.in), dRi = dRof(t.in);
       DevGemmOp op{};
       op.alpha = t.alpha;
@@ -407,6 +457,7 @@ void Ctx::prepare(int st) {
         tl.ldc = dL; tl.m0 = m0; tl.n0 = n0; tl.m = dL; tl.n = dR; tl.op0 = op0; tl.nop = nop; tl.flags = 3;
         t2.push_back(tl);
       }
+    q0 = q1;
   }
   ops1.upload(o1, stream);
   ops2.upload(o2, stream);
@@ -414,8 +465,8 @@ void Ctx::prepare(int st) {
   tiles2.upload(t2, stream);
   ntiles1 = (int)t1.size();
   ntiles2 = (int)t2.size();
-  stats[6] = (int64_t)dp.size();
-  stats[7] = (int64_t)std::count(chain.begin(), chain.end(), 1);
+  stats[6] = tot.pairs;
+  stats[7] = (int64_t)chain.size();
 
   // ---- diagonal
   {
@@ -461,14 +512,24 @@ void Ctx::prepare(int st) {
   // ---- algorithmic bytes: the vectors and every operator tensor the step reads
   {
     std::vector<char> seen(bufs.size(), 0);
+    for (const auto& st2 : seen_t)
+      for (size_t b = 0; b < seen.size(); b++) seen[b] |= st2[b];
     auto mark = [&](const Ref& r) { if (r.buf >= BUF_TENSOR0) seen[r.buf] = 1; };
-    for (const Term& t : plan->terms) { mark(t.Lb); mark(t.Rb); }
     for (const Combo& c : plan->combos) for (auto& s : c.src) mark(s.first);
     double bytes = 16.0 * V;
     for (size_t b = 0; b < bufs.size(); b++) if (seen[b]) bytes += 8.0 * buf_len[b];
     B_H = bytes;
   }
-  stats[0] = (int64_t)plan->terms.size();
+  stats[0] = plan->nterms;
+  {
+    const double t_end = now_s();
+    const double ph[6] = {t_built - t_start, t_plan - t_built, t_lists - t_plan, t_assemble - t_lists, t_end - t_assemble,
+                          t_end - t_start};
+    for (int q = 0; q < 6; q++) prep_s[q] = ph[q];
+    if (verbose)
+      std::fprintf(stderr, "prepare site %d: plan %.3f s, combinations %.3f s, lists %.3f s, assembly %.3f s, rest %.3f s, total %.3f s\n",
+                   st, ph[0], ph[1], ph[2], ph[3], ph[4], ph[5]);
+  }
   stats[1] = (int64_t)plan->combos.size();
   stats[2] = plan->combo_size;
   stats[3] = Wsize;
@@ -655,6 +716,7 @@ void Ctx::update_operators(int index, bool mr) {
   std::vector<int> Qsite;
   const int nC = mr ? L - 1 - index : index + 1;
   for (int k = 0; k < nC; k++) Qsite.push_back(mr ? index + 1 + k : index - k);
+  const double t0 = now_s();
   UpdatePlanner up;
   up.sec = &sec;
   up.irr = irreps.data();
@@ -662,6 +724,7 @@ void Ctx::update_operators(int index, bool mr) {
   up.L = L;
   if (mr) up.moving_right(index, o, prev, &mps[tsite], Qsite);
   else up.moving_left(index, o, prev, &mps[tsite], Qsite);
+  const double t1 = now_s();
   last_update_flops = up.flops;
   last_update_terms = (int64_t)(up.terms[1].size() + up.terms[2].size());
   // combinations first (they read the previous slot only)
@@ -714,6 +777,9 @@ void Ctx::update_operators(int index, bool mr) {
   run_terms(up.terms[1], out1, arena.p);
   run_terms(up.terms[2], out2, arena.p);
   site = -1;
+  if (verbose)
+    std::fprintf(stderr, "update slot %d (%s): plan %.3f s (%lld terms, %zu combinations), device %.3f s\n", index,
+                 mr ? "right" : "left", t1 - t0, (long long)last_update_terms, up.combos.size(), now_s() - t1);
 }
 
 // ------------------------------------------------------------ Davidson -----
@@ -948,6 +1014,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_EXPLICIT_MAX")) ctx->c.explicit_max = std::atoll(e);
   if (const char* e = std::getenv("CHM_EXPLICIT_BUDGET_GB")) ctx->c.explicit_budget_gb = std::atof(e);
   if (const char* e = std::getenv("CHM_ITEM_ELEMS")) ctx->c.item_elems = std::max(256LL, std::atoll(e));
+  if (const char* e = std::getenv("CHM_VERBOSE")) ctx->c.verbose = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return CHM_ERR_HIP;
@@ -1081,9 +1148,12 @@ int chm_dmrg_init(chm_ctx* ctx, int D, uint64_t seed) {
     if (!c.have_problem) throw Error{CHM_ERR_STATE, "set the problem first"};
     if (D < 1) throw Error{CHM_ERR_ARG, "D must be positive"};
     (void)hipSetDevice(c.device);
+    const double t0 = chm::now_s();
     chm::auto_sectors(c, D);
     c.mps_random(seed);
+    const double t1 = chm::now_s();
     c.presolve();
+    if (c.verbose) std::fprintf(stderr, "dmrg init: sectors and MPS %.3f s, operators %.3f s\n", t1 - t0, chm::now_s() - t1);
     c.n_matvec_total = 0;
     for (double& t : c.tm) t = 0.0;
   });
@@ -1095,6 +1165,15 @@ int chm_dmrg_solve(chm_ctx* ctx, int n, const int* D, const double* econv, const
     if (n < 1 || !D || !econv || !max_sweeps || !noise || !rtol || !energy) throw Error{CHM_ERR_ARG, "bad instructions"};
     (void)hipSetDevice(c.device);
     *energy = c.dmrg_solve(n, D, econv, max_sweeps, noise, rtol);
+  });
+}
+
+int chm_dmrg_sweep(chm_ctx* ctx, int moving_right, int D, double rtol, double noise, int change, double* energy) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!energy || D < 1 || !(rtol > 0.0)) throw Error{CHM_ERR_ARG, "bad sweep arguments"};
+    if (c.mps.empty() || c.fci.empty()) throw Error{CHM_ERR_STATE, "call chm_dmrg_init first"};
+    (void)hipSetDevice(c.device);
+    *energy = c.sweep(moving_right != 0, D, rtol, noise, change != 0);
   });
 }
 

@@ -50,6 +50,34 @@ struct DBuf {
   }
 };
 
+// Page-locked host staging memory that only grows (hipHostMalloc); sub-allocated in
+// order by take() after reset().
+struct HostStage {
+  char* p = nullptr;
+  size_t n = 0, used = 0;
+  HostStage() = default;
+  HostStage(const HostStage&) = delete;
+  HostStage& operator=(const HostStage&) = delete;
+  ~HostStage() { if (p) (void)hipHostFree(p); }
+  void reserve(size_t bytes) {
+    if (bytes <= n) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    const size_t want = bytes + bytes / 4;
+    hip_check(hipHostMalloc((void**)&p, want, hipHostMallocDefault), "hipHostMalloc staging");
+    n = want;
+  }
+  void reset() { used = 0; }
+  template <class T>
+  T* take(size_t count) {
+    const size_t off = (used + 255) / 256 * 256;
+    if (off + count * sizeof(T) > n) throw Error{ERR_STATE, "staging buffer too small"};
+    used = off + count * sizeof(T);
+    return (T*)(p + off);
+  }
+};
+
 // hipMemcpy after the context's stream has drained: the stream does not synchronise
 // with the null stream, so a plain hipMemcpy could overtake kernels still using the
 // destination or the source.
@@ -104,6 +132,11 @@ struct Ctx : HostState {
   int64_t explicit_threshold = 0;
   DBuf<double> Mvals;
   DBuf<int> Mcol;
+  DBuf<DevPair> d_pairs;
+  DBuf<DevKTerm> d_kterms;
+  HostStage stage;
+  double prep_s[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // seconds of the last prepare, by phase
+  bool verbose = false;                         // CHM_VERBOSE: phase times on stderr
   int64_t item_elems = 32768;        // CHM_ITEM_ELEMS: matrix elements per work item
   DBuf<DevBItem> bitems;
   DBuf<DevBRed> breds;
@@ -132,6 +165,7 @@ struct Ctx : HostState {
   int64_t n_matvec_total = 0;
   uint64_t noise_seed = 1;
   void join(int site);
+  double sweep(bool moving_right, int D, double rtol, double noise_prefactor, bool change);
   double split(int site, int D, bool moving_right, bool change);
   void mps_random(uint64_t seed);
   void presolve();

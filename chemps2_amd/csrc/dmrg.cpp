@@ -290,15 +290,9 @@ double Ctx::split(int st, int D, bool mr, bool change) {
       if (nd[i] != sec.dim(st + 1, cs[i].nm, cs[i].sm, cs[i].im)) update = true;
   }
   if (update) {
-    auto& lst = sec.list[st + 1];
-    auto& mp = sec.dimmap[st + 1];
-    lst.clear();
-    mp.clear();
-    for (size_t i = 0; i < cs.size(); i++)
-      if (nd[i] > 0) {
-        lst.push_back({cs[i].nm, cs[i].sm, cs[i].im, nd[i]});
-        mp[key3(cs[i].nm, cs[i].sm, cs[i].im)] = nd[i];
-      }
+    std::vector<Sectors::Sec> v;
+    for (size_t i = 0; i < cs.size(); i++) v.push_back({cs[i].nm, cs[i].sm, cs[i].im, nd[i]});
+    sec.set(st + 1, std::move(v));
   }
   mps_alloc(st);
   mps_alloc(st + 1);
@@ -376,9 +370,32 @@ double Ctx::solve_site(int index, double rtol, double noise, int D, bool mr, boo
   return E;
 }
 
-double Ctx::dmrg_solve(int n, const int* D, const double* econv, const int* max_sweeps, const double* noise,
-                       const double* rtol) {  // DMRG.cpp:268-417
+// DMRG::sweepleft / sweepright (DMRG.cpp:354-417): energy of the last site.
+double Ctx::sweep(bool mr, int D, double rtol, double noise_prefactor, bool change) {
   const double t_start = now();
+  const double nl = std::fabs(noise_prefactor) * max_disc_last;
+  max_disc_last = 0.0;
+  last_min = 1e8;
+  double E = 0.0;
+  auto step = [&](int index) {
+    E = solve_site(index, rtol, nl, D, mr, change);
+    total_min = std::min(total_min, E);
+    last_min = std::min(last_min, E);
+    const double t = now();
+    update_operators(index, mr);
+    tm[4] += now() - t;
+    if (verbose) std::fprintf(stderr, "  site %d: E = %.12f (%.2f s)\n", index, E, now() - t_start);
+  };
+  if (mr)
+    for (int index = 0; index < L - 2; index++) step(index);
+  else
+    for (int index = L - 2; index > 0; index--) step(index);
+  tm[5] += now() - t_start;
+  return E;
+}
+
+double Ctx::dmrg_solve(int n, const int* D, const double* econv, const int* max_sweeps, const double* noise,
+                       const double* rtol) {  // DMRG.cpp:268-352
   bool change = total_min < 1e8;
   double E = 0.0;
   for (int ins = 0; ins < n; ins++) {
@@ -386,37 +403,12 @@ double Ctx::dmrg_solve(int n, const int* D, const double* econv, const int* max_
     double Eprev = E + 10 * econv[ins];
     while (std::fabs(E - Eprev) > econv[ins] && it < max_sweeps[ins]) {
       Eprev = E;
-      {  // sweep left
-        const double nl = std::fabs(noise[ins]) * max_disc_last;
-        max_disc_last = 0.0;
-        last_min = 1e8;
-        for (int index = L - 2; index > 0; index--) {
-          E = solve_site(index, rtol[ins], nl, D[ins], false, change);
-          total_min = std::min(total_min, E);
-          last_min = std::min(last_min, E);
-          const double t = now();
-          update_operators(index, false);
-          tm[4] += now() - t;
-        }
-      }
+      E = sweep(false, D[ins], rtol[ins], noise[ins], change);
       change = true;
-      {  // sweep right
-        const double nl = std::fabs(noise[ins]) * max_disc_last;
-        max_disc_last = 0.0;
-        last_min = 1e8;
-        for (int index = 0; index < L - 2; index++) {
-          E = solve_site(index, rtol[ins], nl, D[ins], true, change);
-          total_min = std::min(total_min, E);
-          last_min = std::min(last_min, E);
-          const double t = now();
-          update_operators(index, true);
-          tm[4] += now() - t;
-        }
-      }
+      E = sweep(true, D[ins], rtol[ins], noise[ins], change);
       it++;
     }
   }
-  tm[5] += now() - t_start;
   return total_min;
 }
 

@@ -5,8 +5,10 @@
 // sum_l V L_l, sum_b V L_b) become combinations computed once per site.
 #include <algorithm>
 #include <cassert>
+#include <cstring>
 
 #include "heff_plan.hpp"
+#include "parallel.hpp"
 
 namespace chm {
 
@@ -42,26 +44,42 @@ void HeffPlanner::term(int ik, int kin, double alpha, char tl, Ref Lb, char tr, 
   terms.push_back(t);
 }
 
+static inline uint64_t hmix(uint64_t h, uint64_t v) {
+  h ^= v + 0x9E3779B97F4A7C15ULL + (h << 6) + (h >> 2);
+  return h * 0xFF51AFD7ED558CCDULL;
+}
+
+static uint64_t combo_key(const std::vector<std::pair<Ref, double>>& src, int64_t size) {
+  uint64_t h = hmix(0x1234567ULL, (uint64_t)size);
+  for (auto& s : src) {
+    uint64_t c;
+    std::memcpy(&c, &s.second, sizeof(c));
+    h = hmix(hmix(hmix(h, (uint64_t)(uint32_t)s.first.buf), (uint64_t)s.first.off), c);
+  }
+  return h;
+}
+
+static bool same_combo(const Combo& c, const std::vector<std::pair<Ref, double>>& src, int64_t size) {
+  if (c.size != size || c.src.size() != src.size()) return false;
+  for (size_t q = 0; q < src.size(); q++)
+    if (c.src[q].first.buf != src[q].first.buf || c.src[q].first.off != src[q].first.off || c.src[q].second != src[q].second)
+      return false;
+  return true;
+}
+
 Ref HeffPlanner::make_combo(std::vector<std::pair<Ref, double>>&& src, int64_t size) {
   if (src.empty()) { Ref z; z.off = ZERO; return z; }
-  std::string key;
-  key.reserve(src.size() * 24 + 8);
-  key.append((const char*)&size, sizeof(size));
-  for (auto& s : src) {
-    key.append((const char*)&s.first.buf, sizeof(s.first.buf));
-    key.append((const char*)&s.first.off, sizeof(s.first.off));
-    key.append((const char*)&s.second, sizeof(s.second));
-  }
-  auto it = combo_cache.find(key);
-  if (it != combo_cache.end()) return Ref{BUF_COMBO, it->second};
+  const uint64_t h = combo_key(src, size);
+  auto it = combo_cache.find(h);
+  if (it != combo_cache.end() && same_combo(combos[it->second], src, size)) return Ref{BUF_COMBO, it->second};
   Combo c;
-  c.out_off = combo_size;
+  c.out_off = -1;  // set by the merge
   c.size = size;
   c.src = std::move(src);
-  combo_size += (size + 7) / 8 * 8;  // 64-byte aligned blocks
   combos.push_back(std::move(c));
-  combo_cache.emplace(std::move(key), combos.back().out_off);
-  return Ref{BUF_COMBO, combos.back().out_off};
+  combo_hash.push_back(h);
+  if (it == combo_cache.end()) combo_cache.emplace(h, (int)combos.size() - 1);
+  return Ref{BUF_COMBO, (int64_t)combos.size() - 1};
 }
 
 Ref HeffPlanner::lin_comb(const std::vector<OpT*>& ops, const std::vector<double>& co, int n1, int s1, int i1, int n2,
@@ -106,16 +124,89 @@ Ref HeffPlanner::q_plus_l(OpT* Q, std::vector<OpT*>& Ls, int site, bool left, in
   return make_combo(std::move(src), sz);
 }
 
-void HeffPlanner::build() {  // Heff.cpp:43-248
-  for (int ik = 0; ik < S->nk(); ik++) {
-    d1(ik);
-    d2loc(ik);
-    if (!atLeft && !atRight) d2a(ik);
-    d3(ik);
-    d4(ik);
-    if (!atLeft && !atRight) d5(ik);
+void HeffPlanner::build() {  // Heff.cpp:43-248 (the loop over output blocks)
+  const int nk = S->nk();
+  const int CH = CHUNK;
+  const int64_t nch = (nk + CH - 1) / CH;
+  struct Part {
+    std::vector<Term> terms;
+    std::vector<Combo> combos;
+    std::vector<uint64_t> hash;
+    std::vector<DiagTerm> diag;
+    double flops = 0.0;
+  };
+  std::vector<Part> parts(nch);
+  const int nt = (int)std::min<int64_t>(nthreads > 0 ? nthreads : host_threads(), std::max<int64_t>(nch, 1));
+  std::vector<std::unique_ptr<HeffPlanner>> workers(nt);
+  for (auto& w : workers) {
+    w = std::make_unique<HeffPlanner>();
+    w->sec = sec; w->S = S; w->bkv = bkv; w->pv = pv; w->index = index; w->L = L;
+    w->atLeft = atLeft; w->atRight = atRight; w->Lo = Lo; w->Ro = Ro;
   }
-  build_diag();
+  parallel_for(nch, nt, [&](int64_t c, int tid) {
+    HeffPlanner& w = *workers[tid];
+    w.terms.clear(); w.combos.clear(); w.combo_hash.clear(); w.combo_cache.clear(); w.diag.clear();
+    w.flops = 0.0;
+    const int k1 = (int)std::min<int64_t>(nk, (c + 1) * CH);
+    for (int ik = (int)(c * CH); ik < k1; ik++) {
+      w.d1(ik);
+      w.d2loc(ik);
+      if (!atLeft && !atRight) w.d2a(ik);
+      w.d3(ik);
+      w.d4(ik);
+      if (!atLeft && !atRight) w.d5(ik);
+      w.diag_block(ik);
+    }
+    std::stable_sort(w.terms.begin(), w.terms.end(),
+                     [](const Term& a, const Term& b) { return a.out != b.out ? a.out < b.out : a.in < b.in; });
+    Part& p = parts[c];
+    p.terms = std::move(w.terms);
+    p.combos = std::move(w.combos);
+    p.hash = std::move(w.combo_hash);
+    p.diag = std::move(w.diag);
+    p.flops = w.flops;
+    w.terms = {}; w.combos = {}; w.combo_hash = {}; w.diag = {};
+  });
+  // merge in chunk order: global deduplication of the combinations, then the terms
+  terms.clear(); combos.clear(); combo_hash.clear(); combo_cache.clear(); diag.clear();
+  combo_size = 0;
+  flops = 0.0;
+  std::vector<std::vector<int64_t>> remap(nch);
+  std::vector<int64_t> tstart(nch + 1, 0);
+  size_t ndiag = 0;
+  for (int64_t c = 0; c < nch; c++) {
+    Part& p = parts[c];
+    remap[c].resize(p.combos.size());
+    for (size_t q = 0; q < p.combos.size(); q++) {
+      Combo& cb = p.combos[q];
+      auto it = combo_cache.find(p.hash[q]);
+      if (it != combo_cache.end() && same_combo(combos[it->second], cb.src, cb.size)) {
+        remap[c][q] = combos[it->second].out_off;
+        continue;
+      }
+      cb.out_off = combo_size;
+      combo_size += (cb.size + 7) / 8 * 8;  // 64-byte aligned blocks
+      remap[c][q] = cb.out_off;
+      if (it == combo_cache.end()) combo_cache.emplace(p.hash[q], (int)combos.size());
+      combos.push_back(std::move(cb));
+    }
+    flops += p.flops;
+    tstart[c + 1] = tstart[c] + (int64_t)p.terms.size();
+    ndiag += p.diag.size();
+  }
+  combo_cache.clear();
+  nterms = tstart[nch];
+  chunks.assign(nch, {});
+  parallel_for(nch, nt, [&](int64_t c, int) {
+    Part& p = parts[c];
+    for (Term& t : p.terms) {
+      if (t.Lb.buf == BUF_COMBO) t.Lb.off = remap[c][t.Lb.off];
+      if (t.Rb.buf == BUF_COMBO) t.Rb.off = remap[c][t.Rb.off];
+    }
+    chunks[c] = std::move(p.terms);
+  });
+  diag.reserve(ndiag);
+  for (auto& p : parts) diag.insert(diag.end(), p.diag.begin(), p.diag.end());
 }
 
 // ---------------------------------------------------------------- group 1 --
@@ -1616,12 +1707,12 @@ void HeffPlanner::d5(int ik) {
 // ----------------------------------------------------------- diagonal ------
 // Diagonal of Heff (Heff.cpp:250-315, HeffDiagonal.cpp:27-620): the sector-preserving
 // terms, each through the diagonals of its operator blocks.
-void HeffPlanner::build_diag() {
+void HeffPlanner::diag_block(int ik) {  // HeffDiagonal.cpp:27-620 for block ik
   const int i = index, j = index + 1;
   const double sqh = std::sqrt(0.5);
   const bool leftSum = index < L * 0.5;
   auto need = [](Ref r) { if (!r.ok()) throw Error{ERR_MISSING, "missing block for the diagonal"}; return r; };
-  for (int ik = 0; ik < S->nk(); ik++) {
+  {
     const int nl = S->NL[ik], sl = S->SL[ik], il = S->IL[ik], nr = S->NR[ik], sr = S->SR[ik], ir = S->IR[ik];
     const int n1 = S->N1[ik], n2 = S->N2[ik], tj = S->TJ[ik];
     auto add_const = [&](double c) { diag.push_back({ik, 0, c, Ref(), Ref()}); };

@@ -32,22 +32,35 @@ struct HeffPlanner {
   OpSet* Lo = nullptr;
   OpSet* Ro = nullptr;
 
-  // Outputs.
-  std::vector<Term> terms;
+  // Outputs. Chunk c holds the terms of output blocks [c * CHUNK, (c + 1) * CHUNK),
+  // sorted by (out, in).
+  static constexpr int CHUNK = 8;
+  std::vector<std::vector<Term>> chunks;
+  int64_t nterms = 0;
+  std::vector<Term> terms;  // scratch of one chunk while it is generated
   std::vector<Combo> combos;
   int64_t combo_size = 0;
   std::vector<DiagTerm> diag;
   double flops = 0.0;  // F_H: 2mnk over every dgemm_ the reference issues at this site
 
+  int nthreads = 0;  // 0: host_threads()
+
   void init(const Sectors* sec, const std::vector<int>* irreps, int nirrep, const double* mx, int L, const SLayout* S,
             OpSet* Lo, OpSet* Ro);
-  void build();  // throws Error
+  // Terms grouped by output block and sorted by (out, in), combinations, diagonal
+  // terms; output blocks are processed in parallel chunks and merged in order, so the
+  // result does not depend on the thread count. Throws Error.
+  void build();
 
   // --- internals ----------------------------------------------------------
   Ref w2;
-  std::unordered_map<std::string, int64_t> combo_cache;
-  int dl(int ik) const { return bk->dim(index, S->NL[ik], S->SL[ik], S->IL[ik]); }
-  int dr(int ik) const { return bk->dim(index + 2, S->NR[ik], S->SR[ik], S->IR[ik]); }
+  // While a chunk is generated, combination Refs hold the chunk-local combination
+  // index; the merge replaces it by the arena offset.
+  std::unordered_map<uint64_t, int> combo_cache;  // content hash -> local combination
+  std::vector<uint64_t> combo_hash;
+  int dl(int ik) const { return S->dL[ik]; }
+  int dr(int ik) const { return S->dR[ik]; }
+  void diag_block(int ik);
   void term(int ik, int kin, double alpha, char tl, Ref Lb, char tr, Ref Rb, bool xr_first = false);
   Ref make_combo(std::vector<std::pair<Ref, double>>&& src, int64_t size);
   Ref lin_comb(const std::vector<OpT*>& ops, const std::vector<double>& co, int n1, int s1, int i1, int n2, int s2, int i2,
@@ -82,7 +95,6 @@ struct HeffPlanner {
   void d4J(int ik);
   void d4KL(int ik);
   void d5(int ik);
-  void build_diag();
 };
 
 }  // namespace chm

@@ -52,20 +52,41 @@ double wigner9j(int a, int b, int c, int d, int e, int f, int g, int h, int i) {
   return v;
 }
 
+// ------------------------------------------------------------ Sectors ------
+void Sectors::set(int b, std::vector<Sec> v) {
+  std::stable_sort(v.begin(), v.end(), [](const Sec& x, const Sec& y) {
+    return x.N != y.N ? x.N < y.N : (x.S != y.S ? x.S < y.S : x.I < y.I);
+  });
+  std::vector<Sec> keep;
+  auto ix = std::make_shared<SecIndex>();
+  for (const Sec& q : v)
+    if (q.d > 0) {
+      keep.push_back(q);
+      ix->nmax = std::max(ix->nmax, q.N);
+      ix->smax = std::max(ix->smax, q.S);
+    }
+  if (!keep.empty()) ix->pos.assign((size_t)(ix->nmax + 1) * (ix->smax + 1) * 8, -1);
+  for (size_t p = 0; p < keep.size(); p++) ix->pos[((size_t)keep[p].N * (ix->smax + 1) + keep[p].S) * 8 + keep[p].I] = (int32_t)p;
+  list[b] = std::move(keep);
+  idx[b] = std::move(ix);
+}
+
 // ---------------------------------------------------------------- OpT ------
 void OpT::build(const Sectors& sec, int b, int tj, int ne, int nirr) {  // TensorOperator.cpp:29-102
   boundary = b; index = b; two_j = tj; n_elec = ne; n_irrep = nirr;
-  off.clear();
+  bi = sec.idx[b];
+  offs.assign(sec.list[b].size() * (tj + 1), -1);
   Nup.clear(); Sup.clear(); Iup.clear(); Sdn.clear();
   k2i.assign(1, 0);
   int64_t pos = 0;
-  for (const auto& up : sec.list[b]) {
+  for (size_t p = 0; p < sec.list[b].size(); p++) {
+    const auto& up = sec.list[b][p];
     const int nd = up.N + ne, id = up.I ^ nirr;
     for (int sd = up.S - tj; sd <= up.S + tj; sd += 2) {
       if (sd < 0) continue;
       const int dd = sec.dim(b, nd, sd, id);
       if (dd <= 0) continue;
-      off[key(up.N, up.S, up.I, sd)] = pos;
+      offs[p * (tj + 1) + (sd - up.S + tj) / 2] = pos;
       Nup.push_back(up.N); Sup.push_back(up.S); Iup.push_back(up.I); Sdn.push_back(sd);
       pos += (int64_t)up.d * dd;
       k2i.push_back(pos);
@@ -119,11 +140,13 @@ Ref TLay::blk(int nl, int sl, int il, int nr, int sr, int ir) const {
 
 Ref OpT::blk(int n1, int s1, int i1, int n2, int s2, int i2) const {
   Ref r;
-  if ((i1 ^ n_irrep) != i2 || n2 != n1 + n_elec || std::abs(s1 - s2) > two_j) return r;
-  auto it = off.find(key(n1, s1, i1, s2));
-  if (it == off.end()) return r;
+  if ((i1 ^ n_irrep) != i2 || n2 != n1 + n_elec || std::abs(s1 - s2) > two_j || ((s2 - s1 + two_j) & 1)) return r;
+  const int p = bi ? bi->find(n1, s1, i1) : -1;
+  if (p < 0) return r;
+  const int64_t o = offs[(size_t)p * (two_j + 1) + (s2 - s1 + two_j) / 2];
+  if (o < 0) return r;
   r.buf = buf;
-  r.off = it->second;
+  r.off = o;
   return r;
 }
 
@@ -137,6 +160,7 @@ static uint64_t key9(int nl, int sl, int il, int n1, int n2, int tj, int nr, int
 void SLayout::build(const Sectors& sec, int idx, int I1, int I2) {  // Sobject.cpp:36-125
   index = idx;
   NL.clear(); SL.clear(); IL.clear(); N1.clear(); N2.clear(); TJ.clear(); NR.clear(); SR.clear(); IR.clear(); lut.clear();
+  dL.clear(); dR.clear();
   k2i.assign(1, 0);
   for (const auto& l : sec.list[idx]) {
     for (int n1 = 0; n1 <= 2; n1++)
@@ -153,6 +177,7 @@ void SLayout::build(const Sectors& sec, int idx, int I1, int I2) {  // Sobject.c
             lut[key9(l.N, l.S, l.I, n1, n2, tj, nr, sr, ir)] = (int)NL.size();
             NL.push_back(l.N); SL.push_back(l.S); IL.push_back(l.I); N1.push_back(n1); N2.push_back(n2);
             TJ.push_back(tj); NR.push_back(nr); SR.push_back(sr); IR.push_back(ir);
+            dL.push_back(l.d); dR.push_back(dr);
             k2i.push_back(k2i.back() + (int64_t)l.d * dr);
           }
       }

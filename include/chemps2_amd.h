@@ -115,7 +115,12 @@ int chm_split(chm_ctx* ctx, int site, int D, int moving_right, int change, doubl
 int chm_dmrg_init(chm_ctx* ctx, int D, uint64_t seed);
 int chm_dmrg_solve(chm_ctx* ctx, int n, const int* D, const double* econv, const int* max_sweeps, const double* noise,
                    const double* rtol, double* energy);
-/* seconds[6]: join, Heff preparation, Davidson, split, operator updates, whole solve;
+/* One half sweep, DMRG::sweepleft (moving_right = 0) or DMRG::sweepright
+ * (moving_right = 1) of DMRG.cpp:354-417 with one instruction's D, Davidson tolerance
+ * and noise prefactor; change = 0 keeps the virtual dimensions (the reference's first
+ * sweep). *energy: the energy at the last site. */
+int chm_dmrg_sweep(chm_ctx* ctx, int moving_right, int D, double rtol, double noise, int change, double* energy);
+/* seconds[6]: join, Heff preparation, Davidson, split, operator updates, all sweeps;
  * n_matvec: Heff applications. */
 int chm_dmrg_stats(const chm_ctx* ctx, double* seconds, int64_t* n_matvec);
 

@@ -42,12 +42,9 @@ void pc_destroy(void* h) { delete (PC*)h; }
 
 void pc_sectors(void* h, int b, int n, const int* N, const int* S, const int* I, const int* d) {
   PC* p = (PC*)h;
-  p->c.sec.list[b].clear();
-  p->c.sec.dimmap[b].clear();
-  for (int q = 0; q < n; q++) {
-    p->c.sec.list[b].push_back({N[q], S[q], I[q], d[q]});
-    p->c.sec.dimmap[b][key3(N[q], S[q], I[q])] = d[q];
-  }
+  std::vector<Sectors::Sec> v;
+  for (int q = 0; q < n; q++) v.push_back({N[q], S[q], I[q], d[q]});
+  p->c.sec.set(b, std::move(v));
 }
 
 void pc_sectors_auto(void* h, int D) { auto_sectors(((PC*)h)->c, D); }
@@ -230,7 +227,7 @@ long pc_plan(void* h, int site) {
     p->plan->init(&p->c.sec, &p->c.irreps, p->c.nirrep, p->c.mx.data(), p->c.L, &p->S, site > 0 ? &p->c.sets[site - 1] : nullptr,
                   site < p->c.L - 2 ? &p->c.sets[site + 1] : nullptr);
     p->plan->build();
-    return (long)p->plan->terms.size();
+    return (long)p->plan->nterms;
   } catch (const Error& e) {
     p->err = e.msg;
     return -1;
@@ -252,13 +249,15 @@ void pc_k2i(void* h, long* k2i, int* dims) {
 // terms: ints[8] = out, in, tl, tr, xr_first, Lbuf, Rbuf, 0; offs[2]; alpha
 void pc_terms(void* h, int* ints, long* offs, double* alpha) {
   PC* p = (PC*)h;
-  for (size_t q = 0; q < p->plan->terms.size(); q++) {
-    const Term& t = p->plan->terms[q];
+  size_t q = 0;
+  for (const auto& ch : p->plan->chunks)
+  for (const Term& t : ch) {
     int* o = ints + 8 * q;
     o[0] = t.out; o[1] = t.in; o[2] = t.tl; o[3] = t.tr; o[4] = t.xr_first; o[5] = t.Lb.buf; o[6] = t.Rb.buf; o[7] = 0;
     offs[2 * q] = t.Lb.off;
     offs[2 * q + 1] = t.Rb.off;
     alpha[q] = t.alpha;
+    q++;
   }
 }
 
