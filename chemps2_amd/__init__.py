@@ -21,7 +21,7 @@ EXPORTS = [
     "chm_tensor_clear", "chm_heff_prepare", "chm_heff_veclen", "chm_heff_apply", "chm_heff_diag",
     "chm_davidson", "chm_flops", "chm_plan_stats", "chm_stream", "chm_timing_enable", "chm_timing_read",
     "chm_mps_upload", "chm_mps_download", "chm_tensor_download", "chm_update_operators",
-    "chm_join", "chm_s_get", "chm_s_set", "chm_split", "chm_dmrg_init", "chm_dmrg_solve", "chm_dmrg_sweep", "chm_dmrg_stats",
+    "chm_join", "chm_s_get", "chm_s_set", "chm_split", "chm_dmrg_init", "chm_dmrg_solve", "chm_dmrg_sweep", "chm_dmrg_heff_stats", "chm_dmrg_stats",
 ]
 
 _lib = None
@@ -65,6 +65,7 @@ def load():
         "chm_dmrg_solve": (i, [vp, i, _ip, _dp, _ip, _dp, _dp, C.POINTER(d)]),
         "chm_dmrg_sweep": (i, [vp, i, i, d, d, i, C.POINTER(d)]),
         "chm_dmrg_stats": (i, [vp, C.POINTER(d), C.POINTER(i64)]),
+        "chm_dmrg_heff_stats": (i, [vp, C.POINTER(d), C.POINTER(d)]),
         "chm_heff_prepare": (i, [vp, i]),
         "chm_heff_veclen": (i, [vp, C.POINTER(i64)]),
         "chm_heff_apply": (i, [vp, vp, vp, i]),
@@ -208,6 +209,12 @@ class Context:
                                             C.byref(e)), "chm_dmrg_sweep")
         return e.value
 
+    def dmrg_heff_stats(self):
+        """(sum of F_H over the sweeps' Heff applications, their device time in ms)."""
+        f, ms = C.c_double(0), C.c_double(0)
+        self._check(self.lib.chm_dmrg_heff_stats(self.h, C.byref(f), C.byref(ms)), "chm_dmrg_heff_stats")
+        return f.value, ms.value
+
     def dmrg_stats(self):
         t = (C.c_double * 6)()
         n = C.c_int64(0)
@@ -252,7 +259,7 @@ class Context:
         s = (C.c_int64 * 16)()
         self._check(self.lib.chm_plan_stats(self.h, s), "chm_plan_stats")
         keys = ["terms", "combos", "combo_doubles", "W_doubles", "tiles1", "tiles2", "explicit_pairs", "chain_terms",
-                "blocks", "V", "diag_terms", "explicit_doubles", "explicit_cols", "explicit_threshold", "explicit_items", "_15"]
+                "blocks", "V", "diag_terms", "explicit_doubles", "explicit_cols", "explicit_threshold", "explicit_items", "chain_kernel_terms"]
         return dict(zip(keys, list(s)))
 
     def stream(self):

@@ -181,6 +181,30 @@ struct DevTile {  // one 64 x 64 output tile and the products that accumulate in
   int op0, nop;
   int flags;  // bit 0: C relative to y; bit 1: add to C instead of overwriting it
 };
+// Output-stationary chained products for output blocks of at most CHAIN_MAXD rows and
+// columns (and inputs of the same bound): one workgroup per block keeps y_kappa in
+// registers and runs every term of the block; terms sharing an input block kappa'
+// form a group that stages x_kappa' in LDS once.
+constexpr int CHAIN_MAXD = 36;  // 6 LDS arrays of 36^2 doubles fit in 64 KB
+struct DevCBlock {
+  int64_t yoff;
+  int dL, dR;
+  int g0, ng;
+};
+struct DevCGroup {
+  int64_t xoff;
+  int dLi, dRi;
+  int t0, nt;
+};
+struct DevCTerm {  // mode 0: y += a x; 1: a op(A) x; 2: a x op(B); 3: a (op(A) x) op(B); 4: a op(A) (x op(B))
+  const double* A;
+  const double* B;
+  double alpha;
+  int lda, ldb;
+  int ta, tb;
+  int mode;
+  int pad;
+};
 // Explicit block-sparse Heff: output block kappa holds a dense R x C column-major
 // matrix (R = |kappa|) whose column c multiplies x[col[c]].
 struct DevRowBlock {
@@ -251,16 +275,37 @@ struct DevCopy {
   double c0, c1;
   int rows, cols, ldd, lds;
   int keep_rows, keep_cols;  // elements outside [0, keep) are written as zero
+  int trans;                 // write the transpose: dst[c + ldd * r]
+  int pad;
 };
+
+// Thin SVD M = U diag(S) VT of one centre sector of Split by one-sided Jacobi. The
+// working matrix A (m x n, m >= n, column-major) is M, or M^T when M is wide; it is
+// overwritten. Q (n x n) is workspace. U is dLt x n and VT is n x dRt (leading
+// dimensions dLt and n), S descending.
+struct DevSvd {
+  double* A;
+  double* Q;
+  double* U;
+  double* VT;
+  double* S;
+  int m, n;
+  int wide;
+  int pad;
+};
+constexpr int SVD_JACOBI_MAXN = 64;  // largest n of the batched Jacobi kernel
 
 // Launchers (kernels.hip).
 void launch_copies(const DevCopy* items, int n, hipStream_t s);
+void launch_svd_jacobi(const DevSvd* items, int n, hipStream_t s);
 void launch_add_noise(double* x, int64_t n, uint64_t seed, double level, hipStream_t s);
 void launch_tiles(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const double* xbase, double* ybase,
                   hipStream_t s);
 void launch_combos(const DevCombo* d_combos, const DevSrc* d_src, int ncombo, const int64_t* d_chunk_start,
                    int64_t nchunks, hipStream_t s);
 void launch_diag(const DevDiagBlock* blocks, const DevDiag* terms, int nblocks, hipStream_t s);
+void launch_chain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups, const DevCTerm* terms, const double* x,
+                  double* y, int P, hipStream_t s);
 void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs, double* M, hipStream_t s);
 void launch_bspmv(const DevBItem* items, int nitems, const DevBRed* reds, int nreds, double* P, const double* M,
                   const int* col, const double* x, double* y, hipStream_t s);

@@ -22,6 +22,8 @@ Ctx::~Ctx() {
     if (p) (void)hipFree(p);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : mv_ev)
+    if (e) (void)hipEventDestroy(e);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -377,6 +379,70 @@ void Ctx::prepare(int st) {
   }
   const double t_assemble = now_s();
 
+  // ---- chained terms: output blocks whose own and input dimensions are at most
+  // CHAIN_MAXD run in the output-stationary kernel; the rest as GEMM tiles
+  {
+    std::vector<DevCBlock> cb;
+    std::vector<DevCGroup> cg;
+    std::vector<DevCTerm> ct;
+    std::vector<const Term*> rest;
+    int Pmax = 1;
+    for (size_t q0 = 0; q0 < chain.size();) {
+      const int k = chain[q0]->out;
+      size_t q1 = q0;
+      bool small = use_chain_kernel && dLof(k) <= CHAIN_MAXD && dRof(k) <= CHAIN_MAXD;
+      while (q1 < chain.size() && chain[q1]->out == k) {
+        if (dLof(chain[q1]->in) > CHAIN_MAXD || dRof(chain[q1]->in) > CHAIN_MAXD) small = false;
+        q1++;
+      }
+      if (!small) {
+        for (size_t q = q0; q < q1; q++) rest.push_back(chain[q]);
+        q0 = q1;
+        continue;
+      }
+      DevCBlock B{};
+      B.yoff = S.k2i[k]; B.dL = dLof(k); B.dR = dRof(k); B.g0 = (int)cg.size();
+      Pmax = std::max(Pmax, B.dL * B.dR);
+      for (size_t a = q0; a < q1;) {
+        const int in = chain[a]->in;
+        size_t e = a;
+        while (e < q1 && chain[e]->in == in) e++;
+        DevCGroup G{};
+        G.xoff = S.k2i[in]; G.dLi = dLof(in); G.dRi = dRof(in); G.t0 = (int)ct.size();
+        Pmax = std::max({Pmax, G.dLi * G.dRi, B.dL * G.dRi, G.dLi * B.dR, B.dL * G.dLi, G.dRi * B.dR});
+        for (size_t q = a; q < e; q++) {
+          const Term& t = *chain[q];
+          const int dL = B.dL, dR = B.dR, dLi = G.dLi, dRi = G.dRi;
+          DevCTerm T{};
+          T.alpha = t.alpha;
+          if (t.tl != 'I') { T.A = resolve(t.Lb); T.lda = (t.tl == 'N') ? dL : dLi; T.ta = (t.tl == 'T'); }
+          if (t.tr != 'I') { T.B = resolve(t.Rb); T.ldb = (t.tr == 'N') ? dRi : dR; T.tb = (t.tr == 'T'); }
+          if (t.tl == 'I' && t.tr == 'I') T.mode = 0;
+          else if (t.tr == 'I') T.mode = 1;
+          else if (t.tl == 'I') T.mode = 2;
+          else {
+            const double c3 = (double)dL * dRi * dLi + (double)dL * dR * dRi, c4 = (double)dLi * dR * dRi + (double)dL * dR * dLi;
+            T.mode = c3 <= c4 ? 3 : 4;
+          }
+          ct.push_back(T);
+        }
+        G.nt = (int)ct.size() - G.t0;
+        cg.push_back(G);
+        a = e;
+      }
+      B.ng = (int)cg.size() - B.g0;
+      cb.push_back(B);
+      q0 = q1;
+    }
+    if ((int64_t)ct.size() > INT32_MAX || (int64_t)cg.size() > INT32_MAX) throw Error{ERR_SIZE, "too many chained terms"};
+    cblocks.upload(cb, stream);
+    cgroups.upload(cg, stream);
+    cterms.upload(ct, stream);
+    ncblocks = (int)cb.size();
+    chain_P = std::min((Pmax + 31) / 32 * 32, CHAIN_MAXD * CHAIN_MAXD);
+    stats[15] = (int64_t)ct.size();
+    chain.swap(rest);
+  }
   // ---- chained GEMM tiles for the remaining terms (in (out, in) order)
   std::vector<DevGemmOp> o1, o2;
   std::vector<DevTile> t1, t2;
@@ -466,7 +532,7 @@ void Ctx::prepare(int st) {
   ntiles1 = (int)t1.size();
   ntiles2 = (int)t2.size();
   stats[6] = tot.pairs;
-  stats[7] = (int64_t)chain.size();
+  stats[7] = (int64_t)chain.size() + stats[15];
 
   // ---- diagonal
   {
@@ -548,6 +614,7 @@ void Ctx::apply(const double* xp, double* yp) {
   launch_bspmv(bitems.p, nbitems, breds.p, nbreds, bpart.p, Mvals.p, Mcol.p, xp, yp, stream);
   if (timing) hip_check(hipEventRecord(ev[2], stream), "event");
   launch_tiles(tiles2.p, ops2.p, ntiles2, xp, yp, stream);
+  launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, chain_P, stream);
   if (timing) hip_check(hipEventRecord(ev[3], stream), "event");
   hip_check(hipGetLastError(), "Heff kernels");
   if (timing) {
@@ -569,6 +636,17 @@ void Ctx::diag(double* d) {
   launch_diag(dblocks.p, dterms.p, S.nk(), stream);
   hip_check(hipGetLastError(), "diag kernel");
   hip_check(hipStreamSynchronize(stream), "diag");
+}
+
+void Ctx::mv_flush() {
+  if (mv_used == 0) return;
+  hip_check(hipEventSynchronize(mv_ev[mv_used - 1]), "event sync");
+  for (int q = 0; q < mv_used; q += 2) {
+    float a = 0.f;
+    hip_check(hipEventElapsedTime(&a, mv_ev[q], mv_ev[q + 1]), "elapsed");
+    mv_ms += a;
+  }
+  mv_used = 0;
 }
 
 void Ctx::symm(double* x, bool to_symm) {
@@ -611,83 +689,148 @@ OpT* Ctx::find_tensor(int cnt, int kind, int k, int m) {
 
 void Ctx::run_terms(const std::vector<UTerm>& terms, const std::vector<std::pair<Ref, std::pair<int, int>>>& outs,
                     const double* combo_base) {
+  std::vector<USeg> segs(1);
+  segs[0].terms = terms;
+  run_terms(segs, outs, combo_base);
+}
+
+// Executes update terms: every output block is overwritten by the sum of its terms.
+// Three-operand terms first form W = alpha op(A) op(B) (tile phase 1); phase 2 sums,
+// per output block, the two-operand products, W op(C) and the transposed axpys. The
+// lists are built on the host threads; positions come from prefix sums, so they do
+// not depend on the thread count.
+void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<Ref, std::pair<int, int>>>& outs,
+                    const double* combo_base) {
+  const double t_begin = now_s();
+  const int nthr = host_threads();
+  const int64_t nseg = (int64_t)segs.size();
   auto key = [](const Ref& r) { return ((uint64_t)(uint32_t)r.buf << 40) ^ (uint64_t)r.off; };
   std::unordered_map<uint64_t, int> where;
   where.reserve(outs.size() * 2);
   for (size_t q = 0; q < outs.size(); q++) where[key(outs[q].first)] = (int)q;
-  std::vector<std::vector<int>> by_out(outs.size());
-  for (size_t t = 0; t < terms.size(); t++) {
-    auto it = where.find(key(terms[t].out));
-    if (it == where.end()) throw Error{ERR_STATE, "operator update term writes outside the new tensors"};
-    by_out[it->second].push_back((int)t);
-  }
-  std::vector<DevGemmOp> o1, o2;
-  std::vector<DevTile> t1, t2;
-  std::vector<int64_t> woff(terms.size(), -1);
-  int64_t Wsz = 0;
-  for (size_t t = 0; t < terms.size(); t++) {
-    const UTerm& u = terms[t];
-    if (u.kind != 1) continue;
-    DevGemmOp op{};
-    op.A = resolve_with(u.A, combo_base); op.ta = u.ta; op.lda = u.lda;
-    op.B = resolve_with(u.B, combo_base); op.tb = u.tb; op.ldb = u.ldb;
-    op.k = u.k1; op.alpha = u.alpha; op.kind = 0;
-    woff[t] = Wsz;
-    const int opi = (int)o1.size();
-    o1.push_back(op);
-    for (int m0 = 0; m0 < u.m; m0 += 64)
-      for (int n0 = 0; n0 < u.k2; n0 += 64) {
-        DevTile tl{};
-        tl.C = (double*)(intptr_t)Wsz;
-        tl.ldc = u.m; tl.m0 = m0; tl.n0 = n0; tl.m = u.m; tl.n = u.k2; tl.op0 = opi; tl.nop = 1; tl.flags = 0;
-        t1.push_back(tl);
-      }
-    Wsz += ((int64_t)u.m * u.k2 + 7) / 8 * 8;
-  }
+  std::vector<int64_t> s0(nseg + 1, 0);
+  for (int64_t s = 0; s < nseg; s++) s0[s + 1] = s0[s] + (int64_t)segs[s].terms.size();
+  const int64_t n = s0[nseg];
+  // output block of every term; phase-1 counts per segment
+  std::vector<int> qof(n);
+  struct P1 { int64_t ops = 0, tiles = 0, w = 0; };
+  std::vector<P1> p1(nseg + 1);
+  parallel_for(nseg, nthr, [&](int64_t s, int) {
+    P1 c;
+    for (size_t j = 0; j < segs[s].terms.size(); j++) {
+      const UTerm& u = segs[s].terms[j];
+      auto it = where.find(key(u.out));
+      if (it == where.end()) throw Error{ERR_STATE, "operator update term writes outside the new tensors"};
+      qof[s0[s] + j] = it->second;
+      if (u.kind != 1) continue;
+      c.ops++;
+      c.tiles += (int64_t)((u.m + 63) / 64) * ((u.k2 + 63) / 64);
+      c.w += ((int64_t)u.m * u.k2 + 7) / 8 * 8;
+    }
+    p1[s + 1] = c;
+  });
+  for (int64_t s = 0; s < nseg; s++) { p1[s + 1].ops += p1[s].ops; p1[s + 1].tiles += p1[s].tiles; p1[s + 1].w += p1[s].w; }
   DBuf<double> Wb;
-  Wb.alloc(std::max<int64_t>(Wsz, 1));
-  for (auto& tl : t1) tl.C = Wb.p + (intptr_t)tl.C;
-  for (size_t q = 0; q < outs.size(); q++) {
-    const int m = outs[q].second.first, n = outs[q].second.second;
-    if (m <= 0 || n <= 0) continue;
-    const int op0 = (int)o2.size();
-    for (int t : by_out[q]) {
-      const UTerm& u = terms[t];
+  Wb.alloc(std::max<int64_t>(p1[nseg].w, 1));
+  std::vector<DevGemmOp> o1(p1[nseg].ops), o2(n);
+  std::vector<DevTile> t1(p1[nseg].tiles);
+  std::vector<int64_t> woff(n, -1);
+  parallel_for(nseg, nthr, [&](int64_t s, int) {
+    int64_t io = p1[s].ops, it = p1[s].tiles, iw = p1[s].w;
+    const double* cb = combo_base ? combo_base + segs[s].combo_base : nullptr;
+    for (size_t j = 0; j < segs[s].terms.size(); j++) {
+      const UTerm& u = segs[s].terms[j];
+      if (u.kind != 1) continue;
+      DevGemmOp op{};
+      op.A = resolve_with(u.A, cb); op.ta = u.ta; op.lda = u.lda;
+      op.B = resolve_with(u.B, cb); op.tb = u.tb; op.ldb = u.ldb;
+      op.k = u.k1; op.alpha = u.alpha; op.kind = 0;
+      woff[s0[s] + j] = iw;
+      const int opi = (int)io;
+      o1[io++] = op;
+      for (int m0 = 0; m0 < u.m; m0 += 64)
+        for (int n0 = 0; n0 < u.k2; n0 += 64) {
+          DevTile tl{};
+          tl.C = Wb.p + iw;
+          tl.ldc = u.m; tl.m0 = m0; tl.n0 = n0; tl.m = u.m; tl.n = u.k2; tl.op0 = opi; tl.nop = 1; tl.flags = 0;
+          t1[it++] = tl;
+        }
+      iw += ((int64_t)u.m * u.k2 + 7) / 8 * 8;
+    }
+  });
+  // terms grouped by output block, stable in (segment, term) order
+  const int64_t nq = (int64_t)outs.size();
+  std::vector<int64_t> qstart(nq + 1, 0);
+  for (int64_t g = 0; g < n; g++) qstart[qof[g] + 1]++;
+  for (int64_t q = 0; q < nq; q++) qstart[q + 1] += qstart[q];
+  std::vector<int64_t> order(n);
+  {
+    std::vector<int64_t> fill(qstart.begin(), qstart.end() - 1);
+    for (int64_t g = 0; g < n; g++) order[fill[qof[g]]++] = g;
+  }
+  std::vector<int> seg_of(n);
+  for (int64_t s = 0; s < nseg; s++)
+    for (int64_t g = s0[s]; g < s0[s + 1]; g++) seg_of[g] = (int)s;
+  // phase 2: one op per term at its grouped position, then the tiles of every block
+  parallel_for((n + 65535) / 65536, nthr, [&](int64_t c, int) {
+    for (int64_t pos = c * 65536; pos < std::min<int64_t>(n, (c + 1) * 65536); pos++) {
+      const int64_t g = order[pos];
+      const int s = seg_of[g];
+      const UTerm& u = segs[s].terms[g - s0[s]];
+      const double* cb = combo_base ? combo_base + segs[s].combo_base : nullptr;
       DevGemmOp op{};
       if (u.kind == 1) {
-        op.A = Wb.p + woff[t]; op.ta = 0; op.lda = u.m;
-        op.B = resolve_with(u.C, combo_base); op.tb = u.tc; op.ldb = u.ldc;
+        op.A = Wb.p + woff[g]; op.ta = 0; op.lda = u.m;
+        op.B = resolve_with(u.C, cb); op.tb = u.tc; op.ldb = u.ldc;
         op.k = u.k2; op.alpha = 1.0; op.kind = 0;
       } else if (u.kind == 0) {
-        op.A = resolve_with(u.A, combo_base); op.ta = u.ta; op.lda = u.lda;
-        op.B = resolve_with(u.B, combo_base); op.tb = u.tb; op.ldb = u.ldb;
+        op.A = resolve_with(u.A, cb); op.ta = u.ta; op.lda = u.lda;
+        op.B = resolve_with(u.B, cb); op.tb = u.tb; op.ldb = u.ldb;
         op.k = u.k1; op.alpha = u.alpha; op.kind = 0;
       } else {
-        op.A = resolve_with(u.A, combo_base); op.ta = u.ta; op.lda = u.lda;
+        op.A = resolve_with(u.A, cb); op.ta = u.ta; op.lda = u.lda;
         op.alpha = u.alpha; op.kind = 1;
       }
-      o2.push_back(op);
+      o2[pos] = op;
     }
-    const int nop = (int)o2.size() - op0;
-    double* C = const_cast<double*>(resolve_with(outs[q].first, combo_base));
-    for (int m0 = 0; m0 < m; m0 += 64)
-      for (int n0 = 0; n0 < n; n0 += 64) {
-        DevTile tl{};
-        tl.C = C; tl.ldc = m; tl.m0 = m0; tl.n0 = n0; tl.m = m; tl.n = n; tl.op0 = op0; tl.nop = nop; tl.flags = 0;
-        t2.push_back(tl);
-      }
+  });
+  std::vector<int64_t> tstart(nq + 1, 0);
+  for (int64_t q = 0; q < nq; q++) {
+    const int m = outs[q].second.first, nn = outs[q].second.second;
+    tstart[q + 1] = tstart[q] + ((m > 0 && nn > 0) ? (int64_t)((m + 63) / 64) * ((nn + 63) / 64) : 0);
   }
+  std::vector<DevTile> t2(tstart[nq]);
+  parallel_for((nq + 1023) / 1024, nthr, [&](int64_t c, int) {
+    for (int64_t q = c * 1024; q < std::min<int64_t>(nq, (c + 1) * 1024); q++) {
+      const int m = outs[q].second.first, nn = outs[q].second.second;
+      if (m <= 0 || nn <= 0) continue;
+      double* C = const_cast<double*>(resolve_with(outs[q].first, combo_base));
+      int64_t it = tstart[q];
+      for (int m0 = 0; m0 < m; m0 += 64)
+        for (int n0 = 0; n0 < nn; n0 += 64) {
+          DevTile tl{};
+          tl.C = C; tl.ldc = m; tl.m0 = m0; tl.n0 = n0; tl.m = m; tl.n = nn;
+          tl.op0 = (int)qstart[q]; tl.nop = (int)(qstart[q + 1] - qstart[q]); tl.flags = 0;
+          t2[it++] = tl;
+        }
+    }
+  });
+  if ((int64_t)t1.size() > INT32_MAX || (int64_t)t2.size() > INT32_MAX || n > INT32_MAX) throw Error{ERR_SIZE, "too many update tiles"};
+  const double t_lists = now_s();
   DBuf<DevGemmOp> d1, d2;
   DBuf<DevTile> dt1, dt2;
   d1.upload(o1, stream);
   d2.upload(o2, stream);
   dt1.upload(t1, stream);
   dt2.upload(t2, stream);
-  if ((int64_t)t1.size() > INT32_MAX || (int64_t)t2.size() > INT32_MAX) throw Error{ERR_SIZE, "too many update tiles"};
+  const double t_up = now_s();
   launch_tiles(dt1.p, d1.p, (int)t1.size(), nullptr, nullptr, stream);
   launch_tiles(dt2.p, d2.p, (int)t2.size(), nullptr, nullptr, stream);
   hip_check(hipGetLastError(), "update kernels");
   hip_check(hipStreamSynchronize(stream), "update");
+  if (verbose)
+    std::fprintf(stderr, "    terms %lld: lists %.3f s, upload %.3f s, kernels %.3f s (%zu + %zu tiles)\n", (long long)n,
+                 t_lists - t_begin, t_up - t_lists, now_s() - t_up, t1.size(), t2.size());
 }
 
 // DMRG::updateMovingRight/Left (DMRGoperators.cpp:243-907) for slot `index`.
@@ -726,7 +869,7 @@ void Ctx::update_operators(int index, bool mr) {
   else up.moving_left(index, o, prev, &mps[tsite], Qsite);
   const double t1 = now_s();
   last_update_flops = up.flops;
-  last_update_terms = (int64_t)(up.terms[1].size() + up.terms[2].size());
+  last_update_terms = up.nterms(1) + up.nterms(2);
   // combinations first (they read the previous slot only)
   DBuf<double> arena;
   arena.alloc(std::max<int64_t>(up.combo_size, 1));
@@ -774,8 +917,8 @@ void Ctx::update_operators(int index, bool mr) {
   for (auto& row : o.B) for (OpT* t : row) add_outs(t, out2);
   for (auto& row : o.C) for (OpT* t : row) add_outs(t, out2);
   for (auto& row : o.D) for (OpT* t : row) add_outs(t, out2);
-  run_terms(up.terms[1], out1, arena.p);
-  run_terms(up.terms[2], out2, arena.p);
+  run_terms(up.segs[1], out1, arena.p);
+  run_terms(up.segs[2], out2, arena.p);
   site = -1;
   if (verbose)
     std::fprintf(stderr, "update slot %d (%s): plan %.3f s (%lld terms, %zu combinations), device %.3f s\n", index,
@@ -801,6 +944,14 @@ static void jacobi_eig(int n, const std::vector<double>& A, int lda, std::vector
       for (int q = p + 1; q < n; q++) {
         const double apq = a[p + (size_t)n * q];
         if (apq == 0.0) continue;
+        // after a few sweeps an element below the rounding of both diagonal entries is
+        // set to zero (the cyclic Jacobi threshold rule), so the sweeps terminate
+        const double app = std::fabs(a[p + (size_t)n * p]), aqq = std::fabs(a[q + (size_t)n * q]);
+        if (sweep > 3 && app + 100.0 * std::fabs(apq) == app && aqq + 100.0 * std::fabs(apq) == aqq) {
+          a[p + (size_t)n * q] = 0.0;
+          a[q + (size_t)n * p] = 0.0;
+          continue;
+        }
         const double th = (a[q + (size_t)n * q] - a[p + (size_t)n * p]) / (2.0 * apq);
         const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
         const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
@@ -868,7 +1019,22 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
     copy_sync(stream, d_coef.p, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice, "coefs");
     launch_lincomb_vecs(d_ptrs.p, d_coef.p, (int)vs.size(), out, n, stream);
   };
-  auto matvec = [&](int i) { apply(vec[i], hv[i]); };
+  auto matvec = [&](int i) {
+    if (mv_timing) {
+      if (mv_ev.empty()) {
+        mv_ev.assign(512, nullptr);
+        for (auto& e : mv_ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+      }
+      if (mv_used + 2 > (int)mv_ev.size()) mv_flush();
+      hip_check(hipEventRecord(mv_ev[mv_used], stream), "event");
+      apply(vec[i], hv[i]);
+      hip_check(hipEventRecord(mv_ev[mv_used + 1], stream), "event");
+      mv_used += 2;
+      mv_flops += F_H;
+    } else {
+      apply(vec[i], hv[i]);
+    }
+  };
 
   // 'A': guess in the symmetric convention and the diagonal
   hip_check(hipMemcpyAsync(t, x, n * sizeof(double), hipMemcpyDeviceToDevice, stream), "guess");
@@ -967,6 +1133,7 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
     matvec(num_vec);
     nmult++;
   }
+  if (mv_timing) mv_flush();
   // 'C': copy back and convert to the program convention
   hip_check(hipMemcpyAsync(x, u, n * sizeof(double), hipMemcpyDeviceToDevice, stream), "solution");
   symm(x, false);
@@ -1015,6 +1182,8 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_EXPLICIT_BUDGET_GB")) ctx->c.explicit_budget_gb = std::atof(e);
   if (const char* e = std::getenv("CHM_ITEM_ELEMS")) ctx->c.item_elems = std::max(256LL, std::atoll(e));
   if (const char* e = std::getenv("CHM_VERBOSE")) ctx->c.verbose = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CHM_SVD_JACOBI_MAXN")) ctx->c.svd_jacobi_maxn = std::atoi(e);
+  if (const char* e = std::getenv("CHM_CHAIN_KERNEL")) ctx->c.use_chain_kernel = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return CHM_ERR_HIP;
@@ -1156,6 +1325,7 @@ int chm_dmrg_init(chm_ctx* ctx, int D, uint64_t seed) {
     if (c.verbose) std::fprintf(stderr, "dmrg init: sectors and MPS %.3f s, operators %.3f s\n", t1 - t0, chm::now_s() - t1);
     c.n_matvec_total = 0;
     for (double& t : c.tm) t = 0.0;
+    c.mv_flops = c.mv_ms = 0.0;
   });
 }
 
@@ -1175,6 +1345,13 @@ int chm_dmrg_sweep(chm_ctx* ctx, int moving_right, int D, double rtol, double no
     (void)hipSetDevice(c.device);
     *energy = c.sweep(moving_right != 0, D, rtol, noise, change != 0);
   });
+}
+
+int chm_dmrg_heff_stats(const chm_ctx* ctx, double* flops, double* kernel_ms) {
+  if (!ctx || !flops || !kernel_ms) return CHM_ERR_ARG;
+  *flops = ctx->c.mv_flops;
+  *kernel_ms = ctx->c.mv_ms;
+  return CHM_OK;
 }
 
 int chm_dmrg_stats(const chm_ctx* ctx, double* seconds, int64_t* n_matvec) {

@@ -132,6 +132,12 @@ struct Ctx : HostState {
   int64_t explicit_threshold = 0;
   DBuf<double> Mvals;
   DBuf<int> Mcol;
+  bool use_chain_kernel = true;      // CHM_CHAIN_KERNEL=0: every chained term through k_tiles
+  DBuf<DevCBlock> cblocks;
+  DBuf<DevCGroup> cgroups;
+  DBuf<DevCTerm> cterms;
+  int ncblocks = 0;
+  int chain_P = 1;  // doubles per LDS array of the chained kernel
   DBuf<DevPair> d_pairs;
   DBuf<DevKTerm> d_kterms;
   HostStage stage;
@@ -160,8 +166,16 @@ struct Ctx : HostState {
   int32_t s_buf = -1;
   int s_site = -1;
   void* rocblas = nullptr;  // rocblas_handle for the SVDs of Split
+  int svd_jacobi_maxn = SVD_JACOBI_MAXN;  // CHM_SVD_JACOBI_MAXN: larger centre sectors use rocSOLVER
   double total_min = 1e8, max_disc_last = 0.0, last_min = 1e8;
   double tm[6] = {0, 0, 0, 0, 0, 0};  // seconds: join, prepare, davidson, split, update, total
+  // Heff applications inside sweeps: F_H summed over applications and their device time
+  // (events around each application, read back after every Davidson)
+  double mv_flops = 0.0, mv_ms = 0.0;
+  bool mv_timing = false;
+  std::vector<hipEvent_t> mv_ev;
+  int mv_used = 0;
+  void mv_flush();
   int64_t n_matvec_total = 0;
   uint64_t noise_seed = 1;
   void join(int site);
@@ -178,6 +192,8 @@ struct Ctx : HostState {
   void update_operators(int index, bool mr);
   // Executes operator-update terms: every block in outs is overwritten with the sum of
   // its terms (zero if none).
+  void run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<Ref, std::pair<int, int>>>& outs,
+                 const double* combo_base);
   void run_terms(const std::vector<UTerm>& terms, const std::vector<std::pair<Ref, std::pair<int, int>>>& outs,
                  const double* combo_base);
   void set_problem(int L, int group, const int* irr, int N, int TwoS, int Irrep, const double* mx, double econst);

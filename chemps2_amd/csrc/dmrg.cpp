@@ -160,7 +160,12 @@ double Ctx::split(int st, int D, bool mr, bool change) {
   Sj.build(sec, st, irreps[st], irreps[st + 1]);
   const int I1 = irreps[st], I2 = irreps[st + 1];
   struct Mult { int n, s, i, d; };
-  struct Center { int nm, sm, im, dLt = 0, dRt = 0, cd = 0; int64_t moff = 0, uoff = 0, voff = 0, soff = 0; std::vector<Mult> lf, rt; };
+  struct Center {
+    int nm, sm, im, dLt = 0, dRt = 0, cd = 0;
+    bool jac = false;  // batched Jacobi kernel (small sectors) or rocSOLVER
+    int64_t moff = 0, uoff = 0, voff = 0, soff = 0, qoff = 0;
+    std::vector<Mult> lf, rt;
+  };
   std::vector<Center> cs;
   for (const auto& f : fci[st + 1]) {
     Center c;
@@ -186,15 +191,18 @@ double Ctx::split(int st, int D, bool mr, bool change) {
     c.cd = std::min(c.dLt, c.dRt);
     cs.push_back(c);
   }
-  int64_t msz = 0, usz = 0, vsz = 0, ssz = 0;
+  int64_t msz = 0, usz = 0, vsz = 0, ssz = 0, qsz = 0;
   for (auto& c : cs) {
     if (c.cd <= 0) continue;
+    c.jac = c.cd <= std::min(svd_jacobi_maxn, SVD_JACOBI_MAXN) && std::max(c.dLt, c.dRt) <= 4096;
     c.moff = msz; msz += (int64_t)c.dLt * c.dRt;
     c.uoff = usz; usz += (int64_t)c.dLt * c.cd;
     c.voff = vsz; vsz += (int64_t)c.cd * c.dRt;
     c.soff = ssz; ssz += c.cd;
+    if (c.jac) { c.qoff = qsz; qsz += (int64_t)c.cd * c.cd; }
   }
-  DBuf<double> M, U, VT, Sg, E;
+  DBuf<double> M, U, VT, Sg, E, Qw;
+  Qw.alloc(std::max<int64_t>(qsz, 1));
   M.alloc(std::max<int64_t>(msz, 1));
   U.alloc(std::max<int64_t>(usz, 1));
   VT.alloc(std::max<int64_t>(vsz, 1));
@@ -216,8 +224,14 @@ double Ctx::split(int st, int D, bool mr, bool change) {
         const int fase = phase(a.s + b.s + s1 + s2);
         const int tjmin = std::max(std::abs(b.s - a.s), std::abs(s2 - s1)), tjmax = std::min(s1 + s2, a.s + b.s);
         DevCopy it{};
-        it.dst = M.p + c.moff + lo + (int64_t)c.dLt * ro;
-        it.ldd = c.dLt; it.rows = a.d; it.cols = b.d; it.lds = a.d;
+        if (c.jac && c.dLt < c.dRt) {  // the Jacobi kernel works on M^T
+          it.dst = M.p + c.moff + ro + (int64_t)c.dRt * lo;
+          it.ldd = c.dRt; it.trans = 1;
+        } else {
+          it.dst = M.p + c.moff + lo + (int64_t)c.dLt * ro;
+          it.ldd = c.dLt;
+        }
+        it.rows = a.d; it.cols = b.d; it.lds = a.d;
         int nsrc = 0;
         for (int tj = tjmin; tj <= tjmax; tj += 2) {
           const int k = Sj.kappa(a.n, a.s, a.i, c.nm - a.n, b.n - c.nm, tj, b.n, b.s, b.i);
@@ -242,9 +256,21 @@ double Ctx::split(int st, int D, bool mr, bool change) {
     d_items.upload(items, stream);
     launch_copies(d_items.p, (int)items.size(), stream);
     hip_check(hipGetLastError(), "split assembly");
+    std::vector<DevSvd> jobs;
+    for (auto& c : cs)
+      if (c.cd > 0 && c.jac) {
+        DevSvd j{};
+        j.A = M.p + c.moff; j.Q = Qw.p + c.qoff; j.U = U.p + c.uoff; j.VT = VT.p + c.voff; j.S = Sg.p + c.soff;
+        j.m = std::max(c.dLt, c.dRt); j.n = c.cd; j.wide = c.dLt < c.dRt;
+        jobs.push_back(j);
+      }
+    DBuf<DevSvd> d_jobs;
+    d_jobs.upload(jobs, stream);
+    launch_svd_jacobi(d_jobs.p, (int)jobs.size(), stream);
+    hip_check(hipGetLastError(), "split SVD kernel");
     int ci = 0;
     for (auto& c : cs) {
-      if (c.cd > 0) {
+      if (c.cd > 0 && !c.jac) {
         const rocblas_status st2 = rocsolver_dgesvd(hnd, rocblas_svect_singular, rocblas_svect_singular, c.dLt, c.dRt,
                                                    M.p + c.moff, c.dLt, Sg.p + c.soff, U.p + c.uoff, c.dLt,
                                                    VT.p + c.voff, c.cd, E.p + c.soff, rocblas_outofplace, info.p + ci);
@@ -355,10 +381,14 @@ double Ctx::solve_site(int index, double rtol, double noise, int D, bool mr, boo
   double t2 = now();
   tm[1] += t2 - t1;
   int nm = 0;
+  const double mv0 = mv_ms;
   const double E = davidson(Sdev.p, rtol, &nm) + econst;
   n_matvec_total += nm;
   double t3 = now();
   tm[2] += t3 - t2;
+  if (verbose)
+    std::fprintf(stderr, "  site %d: join %.3f s, prepare %.3f s, davidson %.3f s (%d products, %.3f s in Heff kernels)\n", index,
+                 t1 - t0, t2 - t1, t3 - t2, nm, (mv_ms - mv0) * 1e-3);
   s_site = index;  // prepare/davidson keep the two-site object
   if (noise > 0.0) {
     launch_add_noise(Sdev.p, S.size(), noise_seed++, noise, stream);
@@ -373,6 +403,7 @@ double Ctx::solve_site(int index, double rtol, double noise, int D, bool mr, boo
 // DMRG::sweepleft / sweepright (DMRG.cpp:354-417): energy of the last site.
 double Ctx::sweep(bool mr, int D, double rtol, double noise_prefactor, bool change) {
   const double t_start = now();
+  mv_timing = true;
   const double nl = std::fabs(noise_prefactor) * max_disc_last;
   max_disc_last = 0.0;
   last_min = 1e8;
@@ -386,10 +417,17 @@ double Ctx::sweep(bool mr, int D, double rtol, double noise_prefactor, bool chan
     tm[4] += now() - t;
     if (verbose) std::fprintf(stderr, "  site %d: E = %.12f (%.2f s)\n", index, E, now() - t_start);
   };
-  if (mr)
-    for (int index = 0; index < L - 2; index++) step(index);
-  else
-    for (int index = L - 2; index > 0; index--) step(index);
+  try {
+    if (mr)
+      for (int index = 0; index < L - 2; index++) step(index);
+    else
+      for (int index = L - 2; index > 0; index--) step(index);
+  } catch (...) {
+    mv_timing = false;
+    mv_used = 0;
+    throw;
+  }
+  mv_timing = false;
   tm[5] += now() - t_start;
   return E;
 }

@@ -149,6 +149,7 @@ void HeffPlanner::build() {  // Heff.cpp:43-248 (the loop over output blocks)
     w.flops = 0.0;
     const int k1 = (int)std::min<int64_t>(nk, (c + 1) * CH);
     for (int ik = (int)(c * CH); ik < k1; ik++) {
+      const size_t t0 = w.terms.size();
       w.d1(ik);
       w.d2loc(ik);
       if (!atLeft && !atRight) w.d2a(ik);
@@ -156,9 +157,8 @@ void HeffPlanner::build() {  // Heff.cpp:43-248 (the loop over output blocks)
       w.d4(ik);
       if (!atLeft && !atRight) w.d5(ik);
       w.diag_block(ik);
+      std::stable_sort(w.terms.begin() + t0, w.terms.end(), [](const Term& a, const Term& b) { return a.in < b.in; });
     }
-    std::stable_sort(w.terms.begin(), w.terms.end(),
-                     [](const Term& a, const Term& b) { return a.out != b.out ? a.out < b.out : a.in < b.in; });
     Part& p = parts[c];
     p.terms = std::move(w.terms);
     p.combos = std::move(w.combos);

@@ -240,6 +240,119 @@ __global__ __launch_bounds__(256) void k_diag(const DevDiagBlock* __restrict__ b
   }
 }
 
+// Chained products, output-stationary (see DevCBlock). Each thread owns the elements
+// e = tid + 256 q of y_kappa (column-major dL x dR). Per term the operator blocks are
+// copied into LDS (contiguous, coalesced) in alternating buffers, so one barrier
+// separates the copy from the use; two-operand terms add one barrier around their
+// LDS intermediate, formed in the cheaper association order chosen on the host.
+// Dynamic LDS: 6 arrays of P doubles (x, intermediate, 2 x A, 2 x B), P = the
+// largest block product of the launch (at most CHAIN_MAXD^2).
+constexpr int CHAIN_PER = CHAIN_MAXD * CHAIN_MAXD / 256 + 1;
+
+__global__ __launch_bounds__(256) void k_chain(const DevCBlock* __restrict__ blocks, const DevCGroup* __restrict__ groups,
+                                               const DevCTerm* __restrict__ terms, const double* __restrict__ x,
+                                               double* __restrict__ y, int P) {
+  extern __shared__ double lds[];
+  double* xs = lds;
+  double* ts = lds + P;
+  const DevCBlock b = blocks[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int dL = b.dL, dR = b.dR, n = dL * dR;
+  double acc[CHAIN_PER];
+#pragma unroll
+  for (int q = 0; q < CHAIN_PER; q++) acc[q] = 0.0;
+  int buf = 0;
+  for (int g = b.g0; g < b.g0 + b.ng; g++) {
+    const DevCGroup G = groups[g];
+    const int dLi = G.dLi, dRi = G.dRi;
+    __syncthreads();  // every reader of the previous group's x is done
+    for (int e = tid; e < dLi * dRi; e += 256) xs[e] = x[G.xoff + e];
+    for (int t = G.t0; t < G.t0 + G.nt; t++) {
+      const DevCTerm T = terms[t];
+      double* As = lds + (2 + 2 * buf) * P;
+      double* Bs = lds + (3 + 2 * buf) * P;
+      buf ^= 1;
+      if (T.A) for (int e = tid; e < dL * dLi; e += 256) As[e] = T.A[e];
+      if (T.B) for (int e = tid; e < dRi * dR; e += 256) Bs[e] = T.B[e];
+      __syncthreads();
+      const double al = T.alpha;
+      // op(A)(i, k) = As[i * ai + k * ak], op(B)(k, j) = Bs[k * bk + j * bj]
+      const int ai = T.ta ? T.lda : 1, ak = T.ta ? 1 : T.lda;
+      const int bk = T.tb ? T.ldb : 1, bj = T.tb ? 1 : T.ldb;
+      if (T.mode == 3) {  // ts = op(A) x  (dL x dRi), then y += al ts op(B)
+        for (int e = tid; e < dL * dRi; e += 256) {
+          const int i = e % dL, j = e / dL;
+          double s = 0.0;
+          for (int k = 0; k < dLi; k++) s += As[i * ai + k * ak] * xs[k + dLi * j];
+          ts[e] = s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < CHAIN_PER; q++) {
+          const int e = tid + 256 * q;
+          if (e < n) {
+            const int i = e % dL, j = e / dL;
+            double s = 0.0;
+            for (int k = 0; k < dRi; k++) s += ts[i + dL * k] * Bs[k * bk + j * bj];
+            acc[q] += al * s;
+          }
+        }
+      } else if (T.mode == 4) {  // ts = x op(B)  (dLi x dR), then y += al op(A) ts
+        for (int e = tid; e < dLi * dR; e += 256) {
+          const int i = e % dLi, j = e / dLi;
+          double s = 0.0;
+          for (int k = 0; k < dRi; k++) s += xs[i + dLi * k] * Bs[k * bk + j * bj];
+          ts[e] = s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < CHAIN_PER; q++) {
+          const int e = tid + 256 * q;
+          if (e < n) {
+            const int i = e % dL, j = e / dL;
+            double s = 0.0;
+            for (int k = 0; k < dLi; k++) s += As[i * ai + k * ak] * ts[k + dLi * j];
+            acc[q] += al * s;
+          }
+        }
+      } else if (T.mode == 1) {  // y += al op(A) x
+#pragma unroll
+        for (int q = 0; q < CHAIN_PER; q++) {
+          const int e = tid + 256 * q;
+          if (e < n) {
+            const int i = e % dL, j = e / dL;
+            double s = 0.0;
+            for (int k = 0; k < dLi; k++) s += As[i * ai + k * ak] * xs[k + dLi * j];
+            acc[q] += al * s;
+          }
+        }
+      } else if (T.mode == 2) {  // y += al x op(B)
+#pragma unroll
+        for (int q = 0; q < CHAIN_PER; q++) {
+          const int e = tid + 256 * q;
+          if (e < n) {
+            const int i = e % dL, j = e / dL;
+            double s = 0.0;
+            for (int k = 0; k < dRi; k++) s += xs[i + dLi * k] * Bs[k * bk + j * bj];
+            acc[q] += al * s;
+          }
+        }
+      } else {  // y += al x
+#pragma unroll
+        for (int q = 0; q < CHAIN_PER; q++) {
+          const int e = tid + 256 * q;
+          if (e < n) acc[q] += al * xs[e];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < CHAIN_PER; q++) {
+    const int e = tid + 256 * q;
+    if (e < n) y[b.yoff + e] += acc[q];
+  }
+}
+
 // --------------------------------------------------------- vector kernels --
 __global__ __launch_bounds__(256) void k_dot_partial(const double* __restrict__ a, const double* __restrict__ b,
                                                      int64_t n, double* __restrict__ partial) {
@@ -360,7 +473,140 @@ __global__ __launch_bounds__(256) void k_copies(const DevCopy* __restrict__ item
       if (it.rowscale) v *= it.rowscale[r];
       if (it.colscale) v *= it.colscale[c];
     }
-    it.dst[(size_t)r + (size_t)it.ldd * c] = v;
+    if (it.trans) it.dst[(size_t)c + (size_t)it.ldd * r] = v;
+    else it.dst[(size_t)r + (size_t)it.ldd * c] = v;
+  }
+}
+
+// One-sided (Hestenes) Jacobi SVD of the small centre-sector matrices of Split, one
+// workgroup per matrix. Each round rotates a set of disjoint column pairs (round-robin
+// tournament order), every pair on a group of SVD_TPP lanes; sweeps end when no pair
+// has |a_p . a_q| > SVD_TOL |a_p| |a_q|.
+constexpr int SVD_TPP = 16;
+constexpr int SVD_GROUPS = 256 / SVD_TPP;
+constexpr double SVD_TOL = 1e-15;
+constexpr int SVD_MAX_SWEEPS = 80;
+
+__device__ inline double group_sum(double v) {
+  for (int o = SVD_TPP / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, SVD_TPP);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_svd_jacobi(const DevSvd* __restrict__ items) {
+  const DevSvd it = items[blockIdx.x];
+  const int m = it.m, n = it.n;
+  const int tid = threadIdx.x, g = tid / SVD_TPP, l = tid % SVD_TPP;
+  double* A = it.A;
+  double* Q = it.Q;
+  __shared__ int rotated;
+  __shared__ double sig[SVD_JACOBI_MAXN];
+  __shared__ int src[SVD_JACOBI_MAXN];
+  for (int e = tid; e < n * n; e += 256) Q[e] = (e % n == e / n) ? 1.0 : 0.0;
+  __syncthreads();
+  const int n2 = n + (n & 1);
+  for (int sweep = 0; sweep < SVD_MAX_SWEEPS && n > 1; sweep++) {
+    if (tid == 0) rotated = 0;
+    __syncthreads();
+    for (int r = 0; r < n2 - 1; r++) {
+      for (int k0 = 0; k0 < n2 / 2; k0 += SVD_GROUPS) {
+        const int k = k0 + g;
+        int p = -1, q = -1;
+        if (k < n2 / 2) {
+          p = (k == 0) ? 0 : ((k - 1 + r) % (n2 - 1)) + 1;
+          q = ((n2 - 2 - k + r) % (n2 - 1)) + 1;
+          if (p > q) { const int t = p; p = q; q = t; }
+          if (q >= n) p = q = -1;
+        }
+        double al = 0.0, be = 0.0, ga = 0.0;
+        if (p >= 0)
+          for (int i = l; i < m; i += SVD_TPP) {
+            const double x = A[i + (size_t)m * p], y = A[i + (size_t)m * q];
+            al += x * x; be += y * y; ga += x * y;
+          }
+        al = group_sum(al); be = group_sum(be); ga = group_sum(ga);
+        if (p >= 0 && ga != 0.0 && al > 0.0 && be > 0.0 && fabs(ga) > SVD_TOL * sqrt(al * be)) {
+          const double zeta = (be - al) / (2.0 * ga);
+          const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+          for (int i = l; i < m; i += SVD_TPP) {
+            const double x = A[i + (size_t)m * p], y = A[i + (size_t)m * q];
+            A[i + (size_t)m * p] = c * x - s * y;
+            A[i + (size_t)m * q] = s * x + c * y;
+          }
+          for (int i = l; i < n; i += SVD_TPP) {
+            const double x = Q[i + (size_t)n * p], y = Q[i + (size_t)n * q];
+            Q[i + (size_t)n * p] = c * x - s * y;
+            Q[i + (size_t)n * q] = s * x + c * y;
+          }
+          if (l == 0) rotated = 1;
+        }
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+    const int any = rotated;
+    __syncthreads();
+    if (!any) break;
+  }
+  // singular values and their descending order
+  for (int j = g; j < n; j += SVD_GROUPS) {
+    double s2 = 0.0;
+    for (int i = l; i < m; i += SVD_TPP) s2 += A[i + (size_t)m * j] * A[i + (size_t)m * j];
+    s2 = group_sum(s2);
+    if (l == 0) sig[j] = sqrt(s2);
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += 256) {
+    int rank = 0;
+    for (int k = 0; k < n; k++) rank += (sig[k] > sig[j] || (sig[k] == sig[j] && k < j)) ? 1 : 0;
+    src[rank] = j;
+  }
+  __syncthreads();
+  // M = sum_j sig_j u_j v_j^T: u_j = a_j / sig_j and v_j = q_j (tall), or u_j = q_j and
+  // v_j = a_j / sig_j (wide, A = M^T)
+  const int ldU = it.wide ? n : m;
+  for (int t = g; t < n; t += SVD_GROUPS) {
+    const int j = src[t];
+    const double sj = sig[j];
+    const double inv = sj > 1e-250 ? 1.0 / sj : 0.0;
+    if (l == 0) it.S[t] = sj;
+    if (!it.wide) {
+      for (int i = l; i < m; i += SVD_TPP) it.U[i + (size_t)ldU * t] = A[i + (size_t)m * j] * inv;
+      for (int i = l; i < n; i += SVD_TPP) it.VT[t + (size_t)n * i] = Q[i + (size_t)n * j];
+    } else {
+      for (int i = l; i < n; i += SVD_TPP) it.U[i + (size_t)ldU * t] = Q[i + (size_t)n * j];
+      for (int i = l; i < m; i += SVD_TPP) it.VT[t + (size_t)n * i] = A[i + (size_t)m * j] * inv;
+    }
+  }
+  __syncthreads();
+  // vectors of (numerically) zero singular values: complete the m-long side to an
+  // orthonormal set by Gram-Schmidt on unit vectors (one wave, in order; every lane
+  // reads and writes only the elements i = lane (mod 64), lanes meet in shuffles)
+  if (tid < 64) {
+    for (int t = 0; t < n; t++) {
+      if (sig[src[t]] > 1e-250) continue;
+      double* base = it.wide ? it.VT : it.U;
+      const size_t rs = it.wide ? (size_t)n : 1;         // stride between elements
+      const size_t cs = it.wide ? 1 : (size_t)ldU;       // stride between vectors
+      for (int e = 0; e < m; e++) {
+        // v = e_e - sum_{t' < t} w_t' w_t'[e], accepted when |v| > 0.5
+        for (int i = tid; i < m; i += 64) base[i * rs + t * cs] = (i == e) ? 1.0 : 0.0;
+        for (int u = 0; u < t; u++) {
+          double d = 0.0;
+          for (int i = tid; i < m; i += 64) d += base[i * rs + u * cs] * base[i * rs + t * cs];
+          for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o, 64);
+          for (int i = tid; i < m; i += 64) base[i * rs + t * cs] -= d * base[i * rs + u * cs];
+        }
+        double nn = 0.0;
+        for (int i = tid; i < m; i += 64) nn += base[i * rs + t * cs] * base[i * rs + t * cs];
+        for (int o = 32; o > 0; o >>= 1) nn += __shfl_xor(nn, o, 64);
+        if (nn > 0.25) {
+          const double f = 1.0 / sqrt(nn);
+          for (int i = tid; i < m; i += 64) base[i * rs + t * cs] *= f;
+          break;
+        }
+      }
+    }
   }
 }
 
@@ -414,6 +660,13 @@ void launch_diag(const DevDiagBlock* blocks, const DevDiag* terms, int nblocks, 
   hipLaunchKernelGGL(k_diag, dim3(nblocks), dim3(256), 0, s, blocks, terms);
 }
 
+void launch_chain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups, const DevCTerm* terms, const double* x,
+                  double* y, int P, hipStream_t s) {
+  if (nblocks > 0)
+    hipLaunchKernelGGL(k_chain, dim3(nblocks), dim3(256), (size_t)6 * P * sizeof(double), s, blocks, groups, terms, x, y, P);
+}
+
+
 int dot_parts(int64_t n) {
   int64_t g = (n + 255) / 256;
   if (g > 1024) g = 1024;
@@ -463,6 +716,10 @@ void launch_fill_hash(double* p, int64_t n, uint64_t seed, double scale, hipStre
 
 void launch_copies(const DevCopy* items, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_copies, dim3(n), dim3(256), 0, s, items);
+}
+
+void launch_svd_jacobi(const DevSvd* items, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_svd_jacobi, dim3(n), dim3(256), 0, s, items);
 }
 
 void launch_add_noise(double* x, int64_t n, uint64_t seed, double level, hipStream_t s) {

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cassert>
 
+#include "parallel.hpp"
 #include "update_plan.hpp"
 
 namespace chm {
@@ -830,143 +831,206 @@ void UpdatePlanner::X_update(OpT& self, const TLay* T, std::vector<OpT*>& Lprev,
 }
 
 // ------------------------------------------------------- orchestration -----
+void UpdatePlanner::run_tasks() {
+  struct Res {
+    std::vector<UTerm> t[3];
+    std::vector<Combo> combos;
+    int64_t csize = 0;
+    double flops = 0.0;
+  };
+  const int64_t n = (int64_t)tasks.size();
+  std::vector<Res> res(n);
+  const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(nthreads > 0 ? nthreads : host_threads(), n));
+  std::vector<std::unique_ptr<UpdatePlanner>> workers(nt);
+  for (auto& w : workers) {
+    w = std::make_unique<UpdatePlanner>();
+    w->sec = sec; w->irr = irr; w->mx = mx; w->L = L;
+  }
+  parallel_for(n, nt, [&](int64_t i, int tid) {
+    UpdatePlanner& w = *workers[tid];
+    w.terms[1].clear(); w.terms[2].clear(); w.combos.clear();
+    w.combo_size = 0;
+    w.flops = 0.0;
+    w.level = tasks[i].level;
+    tasks[i].fn(w);
+    Res& r = res[i];
+    r.t[1] = std::move(w.terms[1]); r.t[2] = std::move(w.terms[2]);
+    r.combos = std::move(w.combos);
+    r.csize = w.combo_size;
+    r.flops = w.flops;
+    w.terms[1] = {}; w.terms[2] = {}; w.combos = {};
+  });
+  for (int64_t i = 0; i < n; i++) {
+    Res& r = res[i];
+    const int64_t base = combo_size;
+    for (Combo& c : r.combos) {
+      c.out_off += base;
+      combos.push_back(std::move(c));
+    }
+    combo_size += r.csize;
+    flops += r.flops;
+    for (int lev = 1; lev <= 2; lev++)
+      if (!r.t[lev].empty()) segs[lev].push_back({std::move(r.t[lev]), base});
+  }
+  tasks.clear();
+}
+
+void UpdatePlanner::flatten() {
+  for (int lev = 1; lev <= 2; lev++) {
+    terms[lev].clear();
+    for (const USeg& s : segs[lev])
+      for (UTerm t : s.terms) {
+        for (Ref* r : {&t.A, &t.B, &t.C})
+          if (r->buf == BUF_COMBO) r->off += s.combo_base;
+        terms[lev].push_back(t);
+      }
+  }
+}
+
 void UpdatePlanner::moving_right(int index, OpSet& o, OpSet* prev, const TLay* T, const std::vector<int>& Qsite) {  // DMRGoperators.cpp:243-574
-  level = 1;
+  OpSet* O = &o;
+  auto add = [&](int lev, std::function<void(UpdatePlanner&)> f) { tasks.push_back({lev, std::move(f)}); };
   for (int k = 0; k <= index; k++) {
-    if (k == 0) L_create(*o.L[0], T);
-    else update(*o.L[k], prev->L[k - 1], T, T);
+    if (k == 0) add(1, [=](UpdatePlanner& w) { w.L_create(*O->L[0], T); });
+    else add(1, [=](UpdatePlanner& w) { w.update(*O->L[k], prev->L[k - 1], T, T); });
   }
   for (int k = 0; k <= index; k++)
     for (int m = 0; m <= index - k; m++) {
       if (m == 0) {
-        if (k == 0) { F0_makenew(*o.F0[0][0], T); F1_makenew(*o.F1[0][0], T); S0_makenew(*o.S0[0][0], T); }
-        else {
+        if (k == 0) {
+          add(1, [=](UpdatePlanner& w) { w.F0_makenew(*O->F0[0][0], T); });
+          add(1, [=](UpdatePlanner& w) { w.F1_makenew(*O->F1[0][0], T); });
+          add(1, [=](UpdatePlanner& w) { w.S0_makenew(*O->S0[0][0], T); });
+        } else {
           OpT* Lp = prev->L[k - 1];
-          F0_makenew(*o.F0[k][0], Lp, T); F1_makenew(*o.F1[k][0], Lp, T);
-          S0_makenew(*o.S0[k][0], Lp, T); S1_makenew(*o.S1[k][0], Lp, T);
+          add(1, [=](UpdatePlanner& w) { w.F0_makenew(*O->F0[k][0], Lp, T); });
+          add(1, [=](UpdatePlanner& w) { w.F1_makenew(*O->F1[k][0], Lp, T); });
+          add(1, [=](UpdatePlanner& w) { w.S0_makenew(*O->S0[k][0], Lp, T); });
+          add(1, [=](UpdatePlanner& w) { w.S1_makenew(*O->S1[k][0], Lp, T); });
         }
       } else {
-        update(*o.F0[k][m], prev->F0[k][m - 1], T, T);
-        update(*o.F1[k][m], prev->F1[k][m - 1], T, T);
-        update(*o.S0[k][m], prev->S0[k][m - 1], T, T);
-        if (k > 0) update(*o.S1[k][m], prev->S1[k][m - 1], T, T);
+        add(1, [=](UpdatePlanner& w) { w.update(*O->F0[k][m], prev->F0[k][m - 1], T, T); });
+        add(1, [=](UpdatePlanner& w) { w.update(*O->F1[k][m], prev->F1[k][m - 1], T, T); });
+        add(1, [=](UpdatePlanner& w) { w.update(*O->S0[k][m], prev->S0[k][m - 1], T, T); });
+        if (k > 0) add(1, [=](UpdatePlanner& w) { w.update(*O->S1[k][m], prev->S1[k][m - 1], T, T); });
       }
     }
-  level = 2;
   const int k2 = L - 1 - index;
   for (int k = 0; k < k2; k++)
-    for (int m = 0; m < k2 - k; m++) {
-      const int s1 = index + 1 + m, s2 = index + 1 + k + m;
-      const int ip = irrep(s1) ^ irrep(s2);
-      if (index == 0) {
-        // new tensors start at zero
-      } else {
-        update(*o.A[k][m], prev->A[k][m + 1], T, T);
-        if (k > 0) update(*o.B[k][m], prev->B[k][m + 1], T, T);
-        update(*o.C[k][m], prev->C[k][m + 1], T, T);
-        update(*o.D[k][m], prev->D[k][m + 1], T, T);
-      }
-      for (int num = 0; num <= index; num++) {
-        if (ip != o.S0[num][0]->n_irrep) continue;
-        double a = gmx(index - num, index, s1, s2);
-        if (k == 0 && num == 0) a *= 0.5;
-        if (k > 0 && num > 0) a += gmx(index - num, index, s2, s1);
-        axpy(*o.A[k][m], a, o.S0[num][0]);
-        if (num > 0 && k > 0) axpy(*o.B[k][m], gmx(index - num, index, s1, s2) - gmx(index - num, index, s2, s1), o.S1[num][0]);
-        axpy(*o.C[k][m], 2 * gmx(index - num, s1, index, s2) - gmx(index - num, s1, s2, index), o.F0[num][0]);
-        axpy(*o.D[k][m], -gmx(index - num, s1, s2, index), o.F1[num][0]);
-        if (num > 0) {
-          axpy_transpose_CD(*o.C[k][m], 2 * gmx(index - num, s2, index, s1) - gmx(index - num, s2, s1, index), o.F0[num][0]);
-          axpy_transpose_CD(*o.D[k][m], -gmx(index - num, s2, s1, index), o.F1[num][0]);
+    for (int m = 0; m < k2 - k; m++)
+      add(2, [=](UpdatePlanner& w) {
+        const int s1 = index + 1 + m, s2 = index + 1 + k + m;
+        const int ip = w.irrep(s1) ^ w.irrep(s2);
+        if (index != 0) {  // at index 0 the new tensors start at zero
+          w.update(*O->A[k][m], prev->A[k][m + 1], T, T);
+          if (k > 0) w.update(*O->B[k][m], prev->B[k][m + 1], T, T);
+          w.update(*O->C[k][m], prev->C[k][m + 1], T, T);
+          w.update(*O->D[k][m], prev->D[k][m + 1], T, T);
         }
-      }
-    }
-  level = 1;
+        for (int num = 0; num <= index; num++) {
+          if (ip != O->S0[num][0]->n_irrep) continue;
+          double a = w.gmx(index - num, index, s1, s2);
+          if (k == 0 && num == 0) a *= 0.5;
+          if (k > 0 && num > 0) a += w.gmx(index - num, index, s2, s1);
+          w.axpy(*O->A[k][m], a, O->S0[num][0]);
+          if (num > 0 && k > 0) w.axpy(*O->B[k][m], w.gmx(index - num, index, s1, s2) - w.gmx(index - num, index, s2, s1), O->S1[num][0]);
+          w.axpy(*O->C[k][m], 2 * w.gmx(index - num, s1, index, s2) - w.gmx(index - num, s1, s2, index), O->F0[num][0]);
+          w.axpy(*O->D[k][m], -w.gmx(index - num, s1, s2, index), O->F1[num][0]);
+          if (num > 0) {
+            w.axpy_transpose_CD(*O->C[k][m], 2 * w.gmx(index - num, s2, index, s1) - w.gmx(index - num, s2, s1, index), O->F0[num][0]);
+            w.axpy_transpose_CD(*O->D[k][m], -w.gmx(index - num, s2, s1, index), O->F1[num][0]);
+          }
+        }
+      });
   for (int k = 0; k < k2; k++) {
-    OpT& q = *o.Q[k];
     const int site = Qsite[k];
-    if (index == 0) { Q_add_simple(q, site, T); }
-    else {
-      update(q, prev->Q[k + 1], T, T);
-      Q_add_simple(q, site, T);
-      Q_add_L(q, site, prev->L, T);
-      Q_add_AB(q, prev->A[k + 1][0], prev->B[k + 1][0], T);
-      Q_add_CD(q, prev->C[k + 1][0], prev->D[k + 1][0], T);
-    }
+    add(1, [=](UpdatePlanner& w) {
+      OpT& q = *O->Q[k];
+      if (index == 0) { w.Q_add_simple(q, site, T); return; }
+      w.update(q, prev->Q[k + 1], T, T);
+      w.Q_add_simple(q, site, T);
+      w.Q_add_L(q, site, prev->L, T);
+      w.Q_add_AB(q, prev->A[k + 1][0], prev->B[k + 1][0], T);
+      w.Q_add_CD(q, prev->C[k + 1][0], prev->D[k + 1][0], T);
+    });
   }
-  if (index == 0) X_update(*o.X, T);
-  else X_update(*o.X, T, prev->L, prev->X, prev->Q[0], prev->A[0][0], prev->C[0][0], prev->D[0][0]);
+  if (index == 0) add(1, [=](UpdatePlanner& w) { w.X_update(*O->X, T); });
+  else add(1, [=](UpdatePlanner& w) { w.X_update(*O->X, T, prev->L, prev->X, prev->Q[0], prev->A[0][0], prev->C[0][0], prev->D[0][0]); });
+  run_tasks();
 }
 
 void UpdatePlanner::moving_left(int index, OpSet& o, OpSet* prev, const TLay* T, const std::vector<int>& Qsite) {  // DMRGoperators.cpp:576-907
-  level = 1;
+  OpSet* O = &o;
+  auto add = [&](int lev, std::function<void(UpdatePlanner&)> f) { tasks.push_back({lev, std::move(f)}); };
   const int k1 = L - 1 - index;
   for (int k = 0; k < k1; k++) {
-    if (k == 0) L_create(*o.L[0], T);
-    else update(*o.L[k], prev->L[k - 1], T, T);
+    if (k == 0) add(1, [=](UpdatePlanner& w) { w.L_create(*O->L[0], T); });
+    else add(1, [=](UpdatePlanner& w) { w.update(*O->L[k], prev->L[k - 1], T, T); });
   }
   for (int k = 0; k < k1; k++)
     for (int m = 0; m < k1 - k; m++) {
       if (m == 0) {
-        if (k == 0) { F0_makenew(*o.F0[0][0], T); F1_makenew(*o.F1[0][0], T); S0_makenew(*o.S0[0][0], T); }
-        else {
+        if (k == 0) {
+          add(1, [=](UpdatePlanner& w) { w.F0_makenew(*O->F0[0][0], T); });
+          add(1, [=](UpdatePlanner& w) { w.F1_makenew(*O->F1[0][0], T); });
+          add(1, [=](UpdatePlanner& w) { w.S0_makenew(*O->S0[0][0], T); });
+        } else {
           OpT* Lp = prev->L[k - 1];
-          F0_makenew(*o.F0[k][0], Lp, T); F1_makenew(*o.F1[k][0], Lp, T);
-          S0_makenew(*o.S0[k][0], Lp, T); S1_makenew(*o.S1[k][0], Lp, T);
+          add(1, [=](UpdatePlanner& w) { w.F0_makenew(*O->F0[k][0], Lp, T); });
+          add(1, [=](UpdatePlanner& w) { w.F1_makenew(*O->F1[k][0], Lp, T); });
+          add(1, [=](UpdatePlanner& w) { w.S0_makenew(*O->S0[k][0], Lp, T); });
+          add(1, [=](UpdatePlanner& w) { w.S1_makenew(*O->S1[k][0], Lp, T); });
         }
       } else {
-        update(*o.F0[k][m], prev->F0[k][m - 1], T, T);
-        update(*o.F1[k][m], prev->F1[k][m - 1], T, T);
-        update(*o.S0[k][m], prev->S0[k][m - 1], T, T);
-        if (k > 0) update(*o.S1[k][m], prev->S1[k][m - 1], T, T);
+        add(1, [=](UpdatePlanner& w) { w.update(*O->F0[k][m], prev->F0[k][m - 1], T, T); });
+        add(1, [=](UpdatePlanner& w) { w.update(*O->F1[k][m], prev->F1[k][m - 1], T, T); });
+        add(1, [=](UpdatePlanner& w) { w.update(*O->S0[k][m], prev->S0[k][m - 1], T, T); });
+        if (k > 0) add(1, [=](UpdatePlanner& w) { w.update(*O->S1[k][m], prev->S1[k][m - 1], T, T); });
       }
     }
-  level = 2;
   const int k2 = index + 1;
   for (int k = 0; k < k2; k++)
-    for (int m = 0; m < k2 - k; m++) {
-      const int s1 = index - m - k, s2 = index - m;
-      const int ip = irrep(s1) ^ irrep(s2);
-      if (index == L - 2) {
-        // new tensors start at zero
-      } else {
-        update(*o.A[k][m], prev->A[k][m + 1], T, T);
-        if (k > 0) update(*o.B[k][m], prev->B[k][m + 1], T, T);
-        update(*o.C[k][m], prev->C[k][m + 1], T, T);
-        update(*o.D[k][m], prev->D[k][m + 1], T, T);
-      }
-      for (int num = 0; num < L - index - 1; num++) {
-        if (ip != o.S0[num][0]->n_irrep) continue;
-        const int r0 = index + 1, rn = index + 1 + num;
-        double a = gmx(s1, s2, r0, rn);
-        if (k == 0 && num == 0) a *= 0.5;
-        if (k > 0 && num > 0) a += gmx(s1, s2, rn, r0);
-        axpy(*o.A[k][m], a, o.S0[num][0]);
-        if (num > 0 && k > 0) axpy(*o.B[k][m], gmx(s1, s2, r0, rn) - gmx(s1, s2, rn, r0), o.S1[num][0]);
-        axpy(*o.C[k][m], 2 * gmx(s1, r0, s2, rn) - gmx(s1, r0, rn, s2), o.F0[num][0]);
-        axpy(*o.D[k][m], -gmx(s1, r0, rn, s2), o.F1[num][0]);
-        if (num > 0) {
-          axpy_transpose_CD(*o.C[k][m], 2 * gmx(s1, rn, s2, r0) - gmx(s1, rn, r0, s2), o.F0[num][0]);
-          axpy_transpose_CD(*o.D[k][m], -gmx(s1, rn, r0, s2), o.F1[num][0]);
+    for (int m = 0; m < k2 - k; m++)
+      add(2, [=](UpdatePlanner& w) {
+        const int s1 = index - m - k, s2 = index - m;
+        const int ip = w.irrep(s1) ^ w.irrep(s2);
+        if (index != L - 2) {  // at index L-2 the new tensors start at zero
+          w.update(*O->A[k][m], prev->A[k][m + 1], T, T);
+          if (k > 0) w.update(*O->B[k][m], prev->B[k][m + 1], T, T);
+          w.update(*O->C[k][m], prev->C[k][m + 1], T, T);
+          w.update(*O->D[k][m], prev->D[k][m + 1], T, T);
         }
-      }
-    }
-  level = 1;
+        for (int num = 0; num < L - index - 1; num++) {
+          if (ip != O->S0[num][0]->n_irrep) continue;
+          const int r0 = index + 1, rn = index + 1 + num;
+          double a = w.gmx(s1, s2, r0, rn);
+          if (k == 0 && num == 0) a *= 0.5;
+          if (k > 0 && num > 0) a += w.gmx(s1, s2, rn, r0);
+          w.axpy(*O->A[k][m], a, O->S0[num][0]);
+          if (num > 0 && k > 0) w.axpy(*O->B[k][m], w.gmx(s1, s2, r0, rn) - w.gmx(s1, s2, rn, r0), O->S1[num][0]);
+          w.axpy(*O->C[k][m], 2 * w.gmx(s1, r0, s2, rn) - w.gmx(s1, r0, rn, s2), O->F0[num][0]);
+          w.axpy(*O->D[k][m], -w.gmx(s1, r0, rn, s2), O->F1[num][0]);
+          if (num > 0) {
+            w.axpy_transpose_CD(*O->C[k][m], 2 * w.gmx(s1, rn, s2, r0) - w.gmx(s1, rn, r0, s2), O->F0[num][0]);
+            w.axpy_transpose_CD(*O->D[k][m], -w.gmx(s1, rn, r0, s2), O->F1[num][0]);
+          }
+        }
+      });
   for (int k = 0; k < k2; k++) {
-    OpT& q = *o.Q[k];
     const int site = Qsite[k];
-    if (index == L - 2) { Q_add_simple(q, site, T); }
-    else {
-      update(q, prev->Q[k + 1], T, T);
-      Q_add_simple(q, site, T);
-      Q_add_L(q, site, prev->L, T);
-      Q_add_AB(q, prev->A[k + 1][0], prev->B[k + 1][0], T);
-      Q_add_CD(q, prev->C[k + 1][0], prev->D[k + 1][0], T);
-    }
+    add(1, [=](UpdatePlanner& w) {
+      OpT& q = *O->Q[k];
+      if (index == L - 2) { w.Q_add_simple(q, site, T); return; }
+      w.update(q, prev->Q[k + 1], T, T);
+      w.Q_add_simple(q, site, T);
+      w.Q_add_L(q, site, prev->L, T);
+      w.Q_add_AB(q, prev->A[k + 1][0], prev->B[k + 1][0], T);
+      w.Q_add_CD(q, prev->C[k + 1][0], prev->D[k + 1][0], T);
+    });
   }
-  if (index == L - 2) X_update(*o.X, T);
-  else X_update(*o.X, T, prev->L, prev->X, prev->Q[0], prev->A[0][0], prev->C[0][0], prev->D[0][0]);
+  if (index == L - 2) add(1, [=](UpdatePlanner& w) { w.X_update(*O->X, T); });
+  else add(1, [=](UpdatePlanner& w) { w.X_update(*O->X, T, prev->L, prev->X, prev->Q[0], prev->A[0][0], prev->C[0][0], prev->D[0][0]); });
+  run_tasks();
 }
-
 
 }  // namespace chm

@@ -1,6 +1,8 @@
 // Work-list generator for the renormalised-operator updates of one boundary
 // (DMRGoperators.cpp:243-907 with TensorL/F0/F1/S0/S1/Q/X and TensorOperator::update).
 #pragma once
+#include <functional>
+
 #include "chm_internal.hpp"
 
 namespace chm {
@@ -17,16 +19,39 @@ struct UTerm {
   int kind;
 };
 
+// Terms of one task; their combination Refs are offsets relative to combo_base.
+struct USeg {
+  std::vector<UTerm> terms;
+  int64_t combo_base = 0;
+};
+
 struct UpdatePlanner {
   const Sectors* sec = nullptr;
   const int* irr = nullptr;
   const double* mx = nullptr;
   int L = 0;
+  int nthreads = 0;               // 0: host_threads()
   int level = 1;                  // 1: L, F, S, Q, X; 2: A, B, C, D (read level-1 results)
-  std::vector<UTerm> terms[3];    // by level
+  std::vector<UTerm> terms[3];    // by level, while one task runs (and after flatten())
   std::vector<Combo> combos;      // x-independent operator sums, evaluated first
   int64_t combo_size = 0;
   double flops = 0.0;
+  // moving_right / moving_left queue one task per output tensor (or per complementary
+  // operator with its sums) and run them on the host threads; the results are
+  // segments in queue order.
+  struct Task {
+    int level;
+    std::function<void(UpdatePlanner&)> fn;
+  };
+  std::vector<Task> tasks;
+  std::vector<USeg> segs[3];
+  int64_t nterms(int lev) const {
+    int64_t n = 0;
+    for (const USeg& s : segs[lev]) n += (int64_t)s.terms.size();
+    return n;
+  }
+  void run_tasks();
+  void flatten();  // segments -> terms[1], terms[2] with arena offsets
 
   int irrep(int site) const { return irr[site]; }
   double gmx(int a, int b, int c, int d) const { return mx[a + (size_t)L * (b + (size_t)L * (c + (size_t)L * d))]; }

@@ -120,6 +120,10 @@ int chm_dmrg_solve(chm_ctx* ctx, int n, const int* D, const double* econv, const
  * and noise prefactor; change = 0 keeps the virtual dimensions (the reference's first
  * sweep). *energy: the energy at the last site. */
 int chm_dmrg_sweep(chm_ctx* ctx, int moving_right, int D, double rtol, double noise, int change, double* energy);
+/* Heff applications inside the sweeps since chm_dmrg_init: the sum of F_H over them
+ * and their device time in ms (HIP events around each application). No counterpart
+ * in the reference: it is the sweep-averaged Heff.x rate of the benchmark. */
+int chm_dmrg_heff_stats(const chm_ctx* ctx, double* flops, double* kernel_ms);
 /* seconds[6]: join, Heff preparation, Davidson, split, operator updates, all sweeps;
  * n_matvec: Heff applications. */
 int chm_dmrg_stats(const chm_ctx* ctx, double* seconds, int64_t* n_matvec);

@@ -165,6 +165,7 @@ long pc_update(void* h, int index, int mr) {
     const TLay* T = &p->mps[mr ? index : index + 1];
     if (mr) u.moving_right(index, o, prev, T, Qsite);
     else u.moving_left(index, o, prev, T, Qsite);
+    u.flatten();
     return (long)p->newt.size();
   } catch (const Error& e) {
     p->err = e.msg;

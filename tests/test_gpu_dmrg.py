@@ -138,3 +138,32 @@ def test_reference_test3_ch4_on_gpu():
                     d["econst"])
     ef = fci_energy(L, irr, N, 0, 0, d["T"], d["V"], d["econst"])
     assert abs(e - ef) < 1e-8
+
+
+@pytest.mark.parametrize("mr,D", [(True, 10), (False, 1000)])
+def test_split_svd_paths_agree(oracle_lib, monkeypatch, mr, D):
+    # batched Jacobi kernel (default) and rocSOLVER (CHM_SVD_JACOBI_MAXN=0) give the
+    # same truncation, discarded weight and re-joined two-site object
+    print(f"case svd paths mr {mr} D {D}", flush=True)
+    irr, T, V = random_abinitio(8, 5, group=7)
+    o = oracle_lib.OracleDMRG(8, 7, irr, 8, 0, 0, T, V)
+    o.init(20, 4)
+    o.sweep_left_to(3, 20, rtol=1e-4)
+    o.site(3)
+    s_prog = symm_to_prog(o, o.davidson(rtol=1e-8)[1])
+    out = []
+    for maxn in ("64", "0"):
+        monkeypatch.setenv("CHM_SVD_JACOBI_MAXN", maxn)
+        c = chemps2_amd.Context(0)
+        load_state(c, o, 20)
+        c.join(3)
+        c.s_set(s_prog)
+        dw = c.split(3, D, mr, True)
+        sec = c.get_sectors(4)
+        c.join(3)
+        out.append((dw, sec, c.s_get()))
+        c.close()
+    (dw0, sec0, s0), (dw1, sec1, s1) = out
+    assert np.array_equal(sec0, sec1)
+    assert abs(dw0 - dw1) <= 1e-12 * max(1.0, dw1)
+    assert np.abs(s0 - s1).max() <= 1e-11 * np.abs(s1).max()

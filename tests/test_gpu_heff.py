@@ -89,10 +89,12 @@ def test_davidson(ctx, oracle_lib, index):
     assert abs(np.linalg.norm(xs) - 1.0) <= 1e-10
 
 
-@pytest.mark.parametrize("explicit_max", ["0", "64", "100000000"])
-def test_explicit_threshold_paths(oracle_lib, explicit_max, monkeypatch):
-    # all chained GEMMs, a mix, and everything assembled explicitly
+@pytest.mark.parametrize("explicit_max,chain_kernel", [("0", "1"), ("0", "0"), ("64", "1"), ("100000000", "1")])
+def test_explicit_threshold_paths(oracle_lib, explicit_max, chain_kernel, monkeypatch):
+    # all chained (output-stationary kernel for blocks up to 64, GEMM tiles above; or
+    # GEMM tiles only), a mix, and everything assembled explicitly
     monkeypatch.setenv("CHM_EXPLICIT_MAX", explicit_max)
+    monkeypatch.setenv("CHM_CHAIN_KERNEL", chain_kernel)
     c = chemps2_amd.Context(0)
     T, V = hubbard(10, U=4.0)
     o = oracle_lib.OracleDMRG(10, 0, [0] * 10, 10, 0, 0, T, V)
@@ -106,6 +108,25 @@ def test_explicit_threshold_paths(oracle_lib, explicit_max, monkeypatch):
         assert st["explicit_pairs"] == 0
     if explicit_max == "100000000":
         assert st["chain_terms"] == 0
+    if chain_kernel == "0":
+        assert st["chain_kernel_terms"] == 0
+    c.close()
+
+
+@pytest.mark.parametrize("site", [0, 3, 6])
+def test_chain_kernel_d2h(oracle_lib, site, monkeypatch):
+    # every term through the output-stationary chained kernel (small D2h blocks: all
+    # four term modes, transposed and plain operator blocks)
+    monkeypatch.setenv("CHM_EXPLICIT_MAX", "0")
+    c = chemps2_amd.Context(0)
+    irr, T, V = random_abinitio(8, 11, group=7)
+    o = oracle_lib.OracleDMRG(8, 7, irr, 8, 0, 0, T, V)
+    o.init(24, 2)
+    o.sweep_left_to(site, 24, rtol=1e-4)
+    o.site(site)
+    check_apply_diag(c, o, site, seed=site + 5)
+    st = c.plan_stats()
+    assert st["explicit_pairs"] == 0 and st["chain_kernel_terms"] == st["terms"]
     c.close()
 
 

@@ -176,3 +176,20 @@ def test_every_site_exact_after_convergence(oracle_lib):
         assert abs(e - ef) < 1e-9, index
         if index > 0:
             o.step_left(index, 5000, rtol=1e-10)
+
+
+@pytest.mark.parametrize("mr,D", [(True, 6), (False, 6), (True, 12)])
+def test_split_discarded_weight_is_lost_norm(oracle_lib, mr, D):
+    # The centre-sector recoupling is orthogonal, so the discarded weight of
+    # Sobject::Split (Sobject.cpp:477-486) must equal the fraction of the squared norm
+    # (symmetric convention) that the re-joined, truncated two-site object has lost.
+    from helpers import symm_to_prog
+
+    o = _site_setup(oracle_lib, index=3, D=16)
+    e, x, nm = o.davidson(rtol=1e-10)
+    n0 = float(x @ x)
+    dw = o.split_test(3, D, mr, True, symm_to_prog(o, x))
+    assert dw > 1e-9  # something was cut
+    o.site(3)
+    x2 = o.site_x()
+    assert abs((1.0 - float(x2 @ x2) / n0) - dw) < 1e-13
