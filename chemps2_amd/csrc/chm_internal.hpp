@@ -67,6 +67,42 @@ struct Sectors {
   }
 };
 
+// Open-addressing hash map from 64-bit keys (never ~0) to int, built once from a list.
+struct FlatMap {
+  std::vector<uint64_t> keys;
+  std::vector<int> vals;
+  uint64_t mask = 0;
+  static uint64_t mixh(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xFF51AFD7ED558CCDULL;
+    k ^= k >> 33;
+    return k;
+  }
+  void build(const std::vector<std::pair<uint64_t, int>>& kv) {
+    size_t cap = 16;
+    while (cap < 2 * kv.size()) cap <<= 1;
+    keys.assign(cap, ~0ULL);
+    vals.assign(cap, -1);
+    mask = cap - 1;
+    for (const auto& e : kv) {
+      size_t i = mixh(e.first) & mask;
+      while (keys[i] != ~0ULL && keys[i] != e.first) i = (i + 1) & mask;
+      keys[i] = e.first;
+      vals[i] = e.second;
+    }
+  }
+  int get(uint64_t k) const {
+    if (keys.empty()) return -1;
+    size_t i = mixh(k) & mask;
+    while (keys[i] != ~0ULL) {
+      if (keys[i] == k) return vals[i];
+      i = (i + 1) & mask;
+    }
+    return -1;
+  }
+  void clear() { keys.clear(); vals.clear(); mask = 0; }
+};
+
 // Where a dense block lives: buffer id (an operator tensor, the combination arena,
 // the Heff input vector x) and the element offset inside it.
 struct Ref {
@@ -103,7 +139,7 @@ struct TLay {
   int64_t size = 0;
   std::vector<int> NL, SL, IL, NR, SR, IR;
   std::vector<int64_t> k2i;
-  std::unordered_map<uint64_t, int> lut;
+  FlatMap lut;
   void build(const Sectors& sec, int site, int Is);
   int kappa(int nl, int sl, int il, int nr, int sr, int ir) const;
   Ref blk(int nl, int sl, int il, int nr, int sr, int ir) const;
@@ -127,7 +163,7 @@ struct SLayout {
   std::vector<int> NL, SL, IL, N1, N2, TJ, NR, SR, IR;
   std::vector<int> dL, dR;  // block dimensions
   std::vector<int64_t> k2i;
-  std::unordered_map<uint64_t, int> lut;
+  FlatMap lut;
   void build(const Sectors& sec, int index, int I1, int I2);
   int kappa(int nl, int sl, int il, int n1, int n2, int tj, int nr, int sr, int ir) const;
   int nk() const { return (int)NL.size(); }

@@ -310,8 +310,8 @@ void Ctx::prepare(int st) {
           const Term& t = tv[q];
           DevKTerm e{};
           e.alpha = t.alpha;
-          if (t.tl != 'I') { e.A = resolve(t.Lb); e.lda = (t.tl == 'N') ? pr.dL : pr.dLi; e.ta = (t.tl == 'T'); }
-          if (t.tr != 'I') { e.B = resolve(t.Rb); e.ldb = (t.tr == 'N') ? pr.dRi : pr.dR; e.tb = (t.tr == 'T'); }
+          if (t.tl != 'I') { e.A = resolve(plan->ref(t, t.Lb)); e.lda = (t.tl == 'N') ? pr.dL : pr.dLi; e.ta = (t.tl == 'T'); }
+          if (t.tr != 'I') { e.B = resolve(plan->ref(t, t.Rb)); e.ldb = (t.tr == 'N') ? pr.dRi : pr.dR; e.tb = (t.tr == 'T'); }
           h_kt[ik++] = e;
         }
         pr.nt = (int)(ik - pr.t0);
@@ -415,8 +415,8 @@ void Ctx::prepare(int st) {
           const int dL = B.dL, dR = B.dR, dLi = G.dLi, dRi = G.dRi;
           DevCTerm T{};
           T.alpha = t.alpha;
-          if (t.tl != 'I') { T.A = resolve(t.Lb); T.lda = (t.tl == 'N') ? dL : dLi; T.ta = (t.tl == 'T'); }
-          if (t.tr != 'I') { T.B = resolve(t.Rb); T.ldb = (t.tr == 'N') ? dRi : dR; T.tb = (t.tr == 'T'); }
+          if (t.tl != 'I') { T.A = resolve(plan->ref(t, t.Lb)); T.lda = (t.tl == 'N') ? dL : dLi; T.ta = (t.tl == 'T'); }
+          if (t.tr != 'I') { T.B = resolve(plan->ref(t, t.Rb)); T.ldb = (t.tr == 'N') ? dRi : dR; T.tb = (t.tr == 'T'); }
           if (t.tl == 'I' && t.tr == 'I') T.mode = 0;
           else if (t.tr == 'I') T.mode = 1;
           else if (t.tl == 'I') T.mode = 2;
@@ -459,12 +459,12 @@ void Ctx::prepare(int st) {
     WRec w;
     if (!t.xr_first) {
       w = {Wsize, dL, dRi};
-      op.A = resolve(t.Lb); op.lda = (t.tl == 'N') ? dL : dLi; op.ta = (t.tl == 'T');
+      op.A = resolve(plan->ref(t, t.Lb)); op.lda = (t.tl == 'N') ? dL : dLi; op.ta = (t.tl == 'T');
       op.B = rel_x(t.in); op.ldb = dLi; op.tb = 0; op.k = dLi; op.flags = 2;
     } else {
       w = {Wsize, dLi, dR};
       op.A = rel_x(t.in); op.lda = dLi; op.ta = 0;
-      op.B = resolve(t.Rb); op.ldb = (t.tr == 'N') ? dRi : dR; op.tb = (t.tr == 'T'); op.k = dRi; op.flags = 1;
+      op.B = resolve(plan->ref(t, t.Rb)); op.ldb = (t.tr == 'N') ? dRi : dR; op.tb = (t.tr == 'T'); op.k = dRi; op.flags = 1;
     }
     op.alpha = t.alpha;
     op.kind = 0;
@@ -497,20 +497,20 @@ void Ctx::prepare(int st) {
       if (t.tl == 'I' && t.tr == 'I') {
         op.kind = 1; op.A = rel_x(t.in); op.lda = dL; op.flags = 1;
       } else if (t.tr == 'I') {
-        op.A = resolve(t.Lb); op.lda = (t.tl == 'N') ? dL : dLi; op.ta = (t.tl == 'T');
+        op.A = resolve(plan->ref(t, t.Lb)); op.lda = (t.tl == 'N') ? dL : dLi; op.ta = (t.tl == 'T');
         op.B = rel_x(t.in); op.ldb = dLi; op.k = dLi; op.flags = 2;
       } else if (t.tl == 'I') {
         op.A = rel_x(t.in); op.lda = dLi; op.ta = 0;
-        op.B = resolve(t.Rb); op.ldb = (t.tr == 'N') ? dRi : dR; op.tb = (t.tr == 'T'); op.k = dRi; op.flags = 1;
+        op.B = resolve(plan->ref(t, t.Rb)); op.ldb = (t.tr == 'N') ? dRi : dR; op.tb = (t.tr == 'T'); op.k = dRi; op.flags = 1;
       } else if (!t.xr_first) {
         const WRec& w = wrec[q];
         op.alpha = 1.0;
         op.A = W.p + w.off; op.lda = w.rows; op.ta = 0;
-        op.B = resolve(t.Rb); op.ldb = (t.tr == 'N') ? dRi : dR; op.tb = (t.tr == 'T'); op.k = dRi;
+        op.B = resolve(plan->ref(t, t.Rb)); op.ldb = (t.tr == 'N') ? dRi : dR; op.tb = (t.tr == 'T'); op.k = dRi;
       } else {
         const WRec& w = wrec[q];
         op.alpha = 1.0;
-        op.A = resolve(t.Lb); op.lda = (t.tl == 'N') ? dL : dLi; op.ta = (t.tl == 'T');
+        op.A = resolve(plan->ref(t, t.Lb)); op.lda = (t.tl == 'N') ? dL : dLi; op.ta = (t.tl == 'T');
         op.B = W.p + w.off; op.ldb = w.rows; op.tb = 0; op.k = dLi;
       }
       o2.push_back(op);
@@ -705,9 +705,12 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
   const int nthr = host_threads();
   const int64_t nseg = (int64_t)segs.size();
   auto key = [](const Ref& r) { return ((uint64_t)(uint32_t)r.buf << 40) ^ (uint64_t)r.off; };
-  std::unordered_map<uint64_t, int> where;
-  where.reserve(outs.size() * 2);
-  for (size_t q = 0; q < outs.size(); q++) where[key(outs[q].first)] = (int)q;
+  FlatMap where;
+  {
+    std::vector<std::pair<uint64_t, int>> kv(outs.size());
+    for (size_t q = 0; q < outs.size(); q++) kv[q] = {key(outs[q].first), (int)q};
+    where.build(kv);
+  }
   std::vector<int64_t> s0(nseg + 1, 0);
   for (int64_t s = 0; s < nseg; s++) s0[s + 1] = s0[s] + (int64_t)segs[s].terms.size();
   const int64_t n = s0[nseg];
@@ -719,9 +722,9 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
     P1 c;
     for (size_t j = 0; j < segs[s].terms.size(); j++) {
       const UTerm& u = segs[s].terms[j];
-      auto it = where.find(key(u.out));
-      if (it == where.end()) throw Error{ERR_STATE, "operator update term writes outside the new tensors"};
-      qof[s0[s] + j] = it->second;
+      const int qq = where.get(key(u.out));
+      if (qq < 0) throw Error{ERR_STATE, "operator update term writes outside the new tensors"};
+      qof[s0[s] + j] = qq;
       if (u.kind != 1) continue;
       c.ops++;
       c.tiles += (int64_t)((u.m + 63) / 64) * ((u.k2 + 63) / 64);

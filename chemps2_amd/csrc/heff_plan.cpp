@@ -4,6 +4,7 @@
 // operator sums the reference rebuilds inside every matrix-vector product (Q + sum V L,
 // sum_l V L_l, sum_b V L_b) become combinations computed once per site.
 #include <algorithm>
+#include <array>
 #include <cassert>
 #include <cstring>
 
@@ -167,44 +168,90 @@ void HeffPlanner::build() {  // Heff.cpp:43-248 (the loop over output blocks)
     p.flops = w.flops;
     w.terms = {}; w.combos = {}; w.combo_hash = {}; w.diag = {};
   });
-  // merge in chunk order: global deduplication of the combinations, then the terms
+  // merge: global deduplication of the combinations in NSH shards (by hash bits), each
+  // shard walking the chunks in order; arena offsets follow shard order, then first
+  // appearance, so the result does not depend on the thread count. The terms keep
+  // chunk-local combination indices; ref() maps them through remap.
   terms.clear(); combos.clear(); combo_hash.clear(); combo_cache.clear(); diag.clear();
   combo_size = 0;
   flops = 0.0;
-  std::vector<std::vector<int64_t>> remap(nch);
-  std::vector<int64_t> tstart(nch + 1, 0);
-  size_t ndiag = 0;
-  for (int64_t c = 0; c < nch; c++) {
-    Part& p = parts[c];
-    remap[c].resize(p.combos.size());
-    for (size_t q = 0; q < p.combos.size(); q++) {
-      Combo& cb = p.combos[q];
-      auto it = combo_cache.find(p.hash[q]);
-      if (it != combo_cache.end() && same_combo(combos[it->second], cb.src, cb.size)) {
-        remap[c][q] = combos[it->second].out_off;
-        continue;
+  constexpr int NSH = 64;
+  auto shard_of = [](uint64_t h) { return (int)(h >> 58); };
+  std::vector<std::vector<int>> by_shard(nch);        // local indices sorted by shard
+  std::vector<std::array<int, NSH + 1>> sh_start(nch);
+  parallel_for(nch, nt, [&](int64_t c, int) {
+    const Part& p = parts[c];
+    auto& st = sh_start[c];
+    st.fill(0);
+    for (uint64_t h : p.hash) st[shard_of(h) + 1]++;
+    for (int q = 0; q < NSH; q++) st[q + 1] += st[q];
+    std::array<int, NSH> fill;
+    for (int q = 0; q < NSH; q++) fill[q] = st[q];
+    by_shard[c].resize(p.hash.size());
+    for (size_t q = 0; q < p.hash.size(); q++) by_shard[c][fill[shard_of(p.hash[q])]++] = (int)q;
+  });
+  struct Shard {
+    std::vector<std::pair<int64_t, int>> reps;  // (chunk, local index) of every distinct combination
+    std::vector<int64_t> size;                   // arena offset within the shard after the pass
+    int64_t total = 0;
+  };
+  std::vector<Shard> shards(NSH);
+  std::vector<std::vector<int64_t>> local(nch);  // local index -> (shard << 40) | index within shard
+  for (int64_t c = 0; c < nch; c++) local[c].resize(parts[c].hash.size());
+  parallel_for(NSH, nt, [&](int64_t s, int) {
+    Shard& sh = shards[s];
+    std::unordered_map<uint64_t, int> cache;
+    for (int64_t c = 0; c < nch; c++) {
+      const Part& p = parts[c];
+      for (int e = sh_start[c][s]; e < sh_start[c][s + 1]; e++) {
+        const int q = by_shard[c][e];
+        const Combo& cb = p.combos[q];
+        auto it = cache.find(p.hash[q]);
+        if (it != cache.end()) {
+          const auto& r = sh.reps[it->second];
+          if (same_combo(parts[r.first].combos[r.second], cb.src, cb.size)) {
+            local[c][q] = ((int64_t)s << 40) | it->second;
+            continue;
+          }
+        }
+        const int id = (int)sh.reps.size();
+        sh.reps.push_back({c, q});
+        if (it == cache.end()) cache.emplace(p.hash[q], id);
+        local[c][q] = ((int64_t)s << 40) | id;
       }
-      cb.out_off = combo_size;
-      combo_size += (cb.size + 7) / 8 * 8;  // 64-byte aligned blocks
-      remap[c][q] = cb.out_off;
-      if (it == combo_cache.end()) combo_cache.emplace(p.hash[q], (int)combos.size());
+    }
+    sh.size.resize(sh.reps.size());
+    for (size_t i = 0; i < sh.reps.size(); i++) {
+      sh.size[i] = sh.total;
+      sh.total += (parts[sh.reps[i].first].combos[sh.reps[i].second].size + 7) / 8 * 8;  // 64-byte aligned
+    }
+  });
+  std::vector<int64_t> sbase(NSH + 1, 0);
+  for (int s = 0; s < NSH; s++) sbase[s + 1] = sbase[s] + shards[s].total;
+  combo_size = sbase[NSH];
+  for (int s = 0; s < NSH; s++)
+    for (size_t i = 0; i < shards[s].reps.size(); i++) {
+      Combo& cb = parts[shards[s].reps[i].first].combos[shards[s].reps[i].second];
+      cb.out_off = sbase[s] + shards[s].size[i];
       combos.push_back(std::move(cb));
     }
-    flops += p.flops;
-    tstart[c + 1] = tstart[c] + (int64_t)p.terms.size();
-    ndiag += p.diag.size();
-  }
-  combo_cache.clear();
-  nterms = tstart[nch];
-  chunks.assign(nch, {});
+  remap.assign(nch, {});
   parallel_for(nch, nt, [&](int64_t c, int) {
-    Part& p = parts[c];
-    for (Term& t : p.terms) {
-      if (t.Lb.buf == BUF_COMBO) t.Lb.off = remap[c][t.Lb.off];
-      if (t.Rb.buf == BUF_COMBO) t.Rb.off = remap[c][t.Rb.off];
+    remap[c].resize(local[c].size());
+    for (size_t q = 0; q < local[c].size(); q++) {
+      const int s = (int)(local[c][q] >> 40), i = (int)(local[c][q] & ((1LL << 40) - 1));
+      remap[c][q] = sbase[s] + shards[s].size[i];
     }
-    chunks[c] = std::move(p.terms);
   });
+  nterms = 0;
+  size_t ndiag = 0;
+  chunks.assign(nch, {});
+  for (int64_t c = 0; c < nch; c++) {
+    flops += parts[c].flops;
+    nterms += (int64_t)parts[c].terms.size();
+    ndiag += parts[c].diag.size();
+    chunks[c] = std::move(parts[c].terms);
+  }
   diag.reserve(ndiag);
   for (auto& p : parts) diag.insert(diag.end(), p.diag.begin(), p.diag.end());
 }

@@ -36,7 +36,13 @@ struct HeffPlanner {
   // sorted by (out, in).
   static constexpr int CHUNK = 8;
   std::vector<std::vector<Term>> chunks;
+  std::vector<std::vector<int64_t>> remap;  // chunk-local combination index -> arena offset
   int64_t nterms = 0;
+  // A term's operator block with its combination reference mapped to the arena.
+  Ref ref(const Term& t, Ref r) const {
+    if (r.buf == BUF_COMBO) r.off = remap[t.out / CHUNK][r.off];
+    return r;
+  }
   std::vector<Term> terms;  // scratch of one chunk while it is generated
   std::vector<Combo> combos;
   int64_t combo_size = 0;

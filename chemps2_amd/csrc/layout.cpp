@@ -104,6 +104,7 @@ static uint64_t key6(int nl, int sl, int il, int nr, int sr, int ir) {
 void TLay::build(const Sectors& sec, int site, int Is) {  // TensorT.cpp:38-104
   index = site;
   NL.clear(); SL.clear(); IL.clear(); NR.clear(); SR.clear(); IR.clear(); lut.clear();
+  std::vector<std::pair<uint64_t, int>> kv;
   k2i.assign(1, 0);
   for (const auto& l : sec.list[site]) {
     for (int nr = l.N; nr <= l.N + 2; nr++) {
@@ -113,7 +114,7 @@ void TLay::build(const Sectors& sec, int site, int Is) {  // TensorT.cpp:38-104
         const int ir = (nr == l.N + 1) ? (l.I ^ Is) : l.I;
         const int dr = sec.dim(site + 1, nr, sr, ir);
         if (dr <= 0) continue;
-        lut[key6(l.N, l.S, l.I, nr, sr, ir)] = (int)NL.size();
+        kv.push_back({key6(l.N, l.S, l.I, nr, sr, ir), (int)NL.size()});
         NL.push_back(l.N); SL.push_back(l.S); IL.push_back(l.I);
         NR.push_back(nr); SR.push_back(sr); IR.push_back(ir);
         k2i.push_back(k2i.back() + (int64_t)l.d * dr);
@@ -121,12 +122,12 @@ void TLay::build(const Sectors& sec, int site, int Is) {  // TensorT.cpp:38-104
     }
   }
   size = k2i.back();
+  lut.build(kv);
 }
 
 int TLay::kappa(int nl, int sl, int il, int nr, int sr, int ir) const {
   if (nl < 0 || sl < 0 || nr < 0 || sr < 0) return -1;
-  auto it = lut.find(key6(nl, sl, il, nr, sr, ir));
-  return it == lut.end() ? -1 : it->second;
+  return lut.get(key6(nl, sl, il, nr, sr, ir));
 }
 
 Ref TLay::blk(int nl, int sl, int il, int nr, int sr, int ir) const {
@@ -161,6 +162,7 @@ void SLayout::build(const Sectors& sec, int idx, int I1, int I2) {  // Sobject.c
   index = idx;
   NL.clear(); SL.clear(); IL.clear(); N1.clear(); N2.clear(); TJ.clear(); NR.clear(); SR.clear(); IR.clear(); lut.clear();
   dL.clear(); dR.clear();
+  std::vector<std::pair<uint64_t, int>> kv;
   k2i.assign(1, 0);
   for (const auto& l : sec.list[idx]) {
     for (int n1 = 0; n1 <= 2; n1++)
@@ -174,7 +176,7 @@ void SLayout::build(const Sectors& sec, int idx, int I1, int I2) {  // Sobject.c
             if (sr < 0) continue;
             const int dr = sec.dim(idx + 2, nr, sr, ir);
             if (dr <= 0) continue;
-            lut[key9(l.N, l.S, l.I, n1, n2, tj, nr, sr, ir)] = (int)NL.size();
+            kv.push_back({key9(l.N, l.S, l.I, n1, n2, tj, nr, sr, ir), (int)NL.size()});
             NL.push_back(l.N); SL.push_back(l.S); IL.push_back(l.I); N1.push_back(n1); N2.push_back(n2);
             TJ.push_back(tj); NR.push_back(nr); SR.push_back(sr); IR.push_back(ir);
             dL.push_back(l.d); dR.push_back(dr);
@@ -182,12 +184,12 @@ void SLayout::build(const Sectors& sec, int idx, int I1, int I2) {  // Sobject.c
           }
       }
   }
+  lut.build(kv);
 }
 
 int SLayout::kappa(int nl, int sl, int il, int n1, int n2, int tj, int nr, int sr, int ir) const {
   if (n1 < 0 || n1 > 2 || n2 < 0 || n2 > 2 || tj < 0 || tj > 2 || sl < 0 || sr < 0 || nl < 0 || nr < 0) return -1;
-  auto it = lut.find(key9(nl, sl, il, n1, n2, tj, nr, sr, ir));
-  return it == lut.end() ? -1 : it->second;
+  return lut.get(key9(nl, sl, il, n1, n2, tj, nr, sr, ir));
 }
 
 }  // namespace chm

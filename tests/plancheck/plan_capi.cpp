@@ -255,8 +255,8 @@ void pc_terms(void* h, int* ints, long* offs, double* alpha) {
   for (const Term& t : ch) {
     int* o = ints + 8 * q;
     o[0] = t.out; o[1] = t.in; o[2] = t.tl; o[3] = t.tr; o[4] = t.xr_first; o[5] = t.Lb.buf; o[6] = t.Rb.buf; o[7] = 0;
-    offs[2 * q] = t.Lb.off;
-    offs[2 * q + 1] = t.Rb.off;
+    offs[2 * q] = p->plan->ref(t, t.Lb).off;
+    offs[2 * q + 1] = p->plan->ref(t, t.Rb).off;
     alpha[q] = t.alpha;
     q++;
   }
