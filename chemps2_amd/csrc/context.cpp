@@ -16,6 +16,7 @@ void launch_fill_hash(double* p, int64_t n, uint64_t seed, double scale, hipStre
 void destroy_rocblas(void* h);
 
 Ctx::~Ctx() {
+  reap_join();
   destroy_rocblas(rocblas);
   if (s_buf >= 0) bufs[s_buf] = nullptr;  // owned by Sdev
   for (double* p : bufs)
@@ -54,6 +55,7 @@ void Ctx::set_problem(int L_, int group_, const int* irr, int N_, int TwoS_, int
   mps.assign(L, TLay());
   mps_buf.assign(L, nullptr);
   site = -1;
+  reap_join();
   plan.reset();
   have_problem = true;
 }
@@ -170,6 +172,7 @@ void Ctx::prepare(int st) {
   if (V <= 0) throw Error{ERR_STATE, "empty two-site object"};
   plan = std::make_unique<HeffPlanner>();
   plan->init(&sec, &irreps, nirrep, mx.data(), L, &S, Lo, Ro);
+  plan->pool = std::move(term_pool);
   plan->build();
   F_H = plan->flops;
   const double t_built = now_s();
@@ -177,21 +180,29 @@ void Ctx::prepare(int st) {
   // ---- x-independent combinations, evaluated now
   combo_arena.alloc(std::max<int64_t>(plan->combo_size, 1));
   {
-    std::vector<DevCombo> dc;
-    std::vector<DevSrc> ds;
-    std::vector<int64_t> chunk_start;
+    const auto& cv = plan->combos;
+    const int64_t ncb = (int64_t)cv.size();
+    std::vector<DevCombo> dc(ncb);
+    std::vector<int64_t> chunk_start(ncb), s_start(ncb + 1, 0);
     int64_t nch = 0;
-    for (const Combo& c : plan->combos) {
-      DevCombo d;
-      d.out = combo_arena.p + c.out_off;
-      d.size = c.size;
-      d.src0 = (int)ds.size();
-      d.nsrc = (int)c.src.size();
-      for (auto& s : c.src) ds.push_back({resolve(s.first), s.second});
-      dc.push_back(d);
-      chunk_start.push_back(nch);
-      nch += (c.size + 1023) / 1024;
+    for (int64_t q = 0; q < ncb; q++) {
+      chunk_start[q] = nch;
+      nch += (cv[q].size + 1023) / 1024;
+      s_start[q + 1] = s_start[q] + (int64_t)cv[q].src.size();
     }
+    if (s_start[ncb] > INT32_MAX) throw Error{ERR_SIZE, "too many combination sources"};
+    std::vector<DevSrc> ds(s_start[ncb]);
+    parallel_for((ncb + 4095) / 4096, host_threads(), [&](int64_t c, int) {
+      for (int64_t q = c * 4096; q < std::min<int64_t>(ncb, (c + 1) * 4096); q++) {
+        DevCombo& d = dc[q];
+        d.out = combo_arena.p + cv[q].out_off;
+        d.size = cv[q].size;
+        d.src0 = (int)s_start[q];
+        d.nsrc = (int)cv[q].src.size();
+        int64_t e = s_start[q];
+        for (auto& s : cv[q].src) ds[e++] = {resolve(s.first), s.second};
+      }
+    });
     if (!dc.empty()) {
       DBuf<DevCombo> d_dc;
       DBuf<DevSrc> d_ds;
@@ -219,13 +230,15 @@ void Ctx::prepare(int st) {
   const int64_t nch = (int64_t)chunks.size();
   const int CH = HeffPlanner::CHUNK;
   const int nk = S.nk();
-  struct Grp { int out, in; int64_t q0, q1, elems; };
+  using Grp = PGrp;
   constexpr int NTHR = 64;
-  std::vector<std::vector<Grp>> grp(nch);
+  if ((int64_t)p_grp.size() < nch) p_grp.resize(nch);
+  std::vector<std::vector<Grp>>& grp = p_grp;
   std::vector<std::array<double, NTHR>> hist(nch);
   std::vector<std::vector<char>> seen_t(nthr, std::vector<char>(bufs.size(), 0));
   parallel_for(nch, nthr, [&](int64_t c, int tid) {
     auto& g = grp[c];
+    g.clear();
     const auto& tv = chunks[c];
     std::vector<char>& seen = seen_t[tid];
     for (int64_t q = 0; q < (int64_t)tv.size(); q++) {
@@ -593,11 +606,13 @@ void Ctx::prepare(int st) {
                           t_end - t_start};
     for (int q = 0; q < 6; q++) prep_s[q] = ph[q];
     if (verbose)
-      std::fprintf(stderr, "prepare site %d: plan %.3f s, combinations %.3f s, lists %.3f s, assembly %.3f s, rest %.3f s, total %.3f s\n",
-                   st, ph[0], ph[1], ph[2], ph[3], ph[4], ph[5]);
+      std::fprintf(stderr, "prepare site %d: plan %.3f s (generate %.3f, merge %.3f, finish %.3f), combinations %.3f s, lists %.3f s, assembly %.3f s, rest %.3f s, total %.3f s\n",
+                   st, ph[0], plan->phase_s[0], plan->phase_s[1], plan->phase_s[2], ph[1], ph[2], ph[3], ph[4], ph[5]);
   }
   stats[1] = (int64_t)plan->combos.size();
   stats[2] = plan->combo_size;
+  term_pool = std::move(plan->chunks);  // their capacity serves the next site
+  reap(std::move(plan));
   stats[3] = Wsize;
   stats[4] = ntiles1;
   stats[5] = ntiles2;
@@ -704,6 +719,7 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
   const double t_begin = now_s();
   const int nthr = host_threads();
   const int64_t nseg = (int64_t)segs.size();
+  const int64_t nq = (int64_t)outs.size();
   auto key = [](const Ref& r) { return ((uint64_t)(uint32_t)r.buf << 40) ^ (uint64_t)r.off; };
   FlatMap where;
   {
@@ -714,8 +730,16 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
   std::vector<int64_t> s0(nseg + 1, 0);
   for (int64_t s = 0; s < nseg; s++) s0[s + 1] = s0[s] + (int64_t)segs[s].terms.size();
   const int64_t n = s0[nseg];
+  if (n > INT32_MAX) throw Error{ERR_SIZE, "too many update terms"};
+  u_qof.resize(n);
+  u_seg.resize(n);
+  u_woff.resize(n);
+  u_order.resize(n);
+  int* qof = u_qof.data();
+  int* seg_of = u_seg.data();
+  int64_t* woff = u_woff.data();
+  int64_t* order = u_order.data();
   // output block of every term; phase-1 counts per segment
-  std::vector<int> qof(n);
   struct P1 { int64_t ops = 0, tiles = 0, w = 0; };
   std::vector<P1> p1(nseg + 1);
   parallel_for(nseg, nthr, [&](int64_t s, int) {
@@ -725,6 +749,7 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
       const int qq = where.get(key(u.out));
       if (qq < 0) throw Error{ERR_STATE, "operator update term writes outside the new tensors"};
       qof[s0[s] + j] = qq;
+      seg_of[s0[s] + j] = (int)s;
       if (u.kind != 1) continue;
       c.ops++;
       c.tiles += (int64_t)((u.m + 63) / 64) * ((u.k2 + 63) / 64);
@@ -733,11 +758,22 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
     p1[s + 1] = c;
   });
   for (int64_t s = 0; s < nseg; s++) { p1[s + 1].ops += p1[s].ops; p1[s + 1].tiles += p1[s].tiles; p1[s + 1].w += p1[s].w; }
-  DBuf<double> Wb;
-  Wb.alloc(std::max<int64_t>(p1[nseg].w, 1));
-  std::vector<DevGemmOp> o1(p1[nseg].ops), o2(n);
-  std::vector<DevTile> t1(p1[nseg].tiles);
-  std::vector<int64_t> woff(n, -1);
+  // phase-2 tiles per output block
+  std::vector<int64_t> tstart(nq + 1, 0);
+  for (int64_t q = 0; q < nq; q++) {
+    const int m = outs[q].second.first, nn = outs[q].second.second;
+    tstart[q + 1] = tstart[q] + ((m > 0 && nn > 0) ? (int64_t)((m + 63) / 64) * ((nn + 63) / 64) : 0);
+  }
+  const int64_t n1 = p1[nseg].ops, nt1 = p1[nseg].tiles, nt2 = tstart[nq];
+  if (nt1 > INT32_MAX || nt2 > INT32_MAX) throw Error{ERR_SIZE, "too many update tiles"};
+  ustage.reset();
+  ustage.reserve(256 * 4 + (n1 + n) * sizeof(DevGemmOp) + (nt1 + nt2) * sizeof(DevTile));
+  DevGemmOp* o1 = ustage.take<DevGemmOp>(n1);
+  DevGemmOp* o2 = ustage.take<DevGemmOp>(n);
+  DevTile* t1 = ustage.take<DevTile>(nt1);
+  DevTile* t2 = ustage.take<DevTile>(nt2);
+  u_W.alloc(std::max<int64_t>(p1[nseg].w, 1));
+  double* Wp = u_W.p;
   parallel_for(nseg, nthr, [&](int64_t s, int) {
     int64_t io = p1[s].ops, it = p1[s].tiles, iw = p1[s].w;
     const double* cb = combo_base ? combo_base + segs[s].combo_base : nullptr;
@@ -754,26 +790,41 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
       for (int m0 = 0; m0 < u.m; m0 += 64)
         for (int n0 = 0; n0 < u.k2; n0 += 64) {
           DevTile tl{};
-          tl.C = Wb.p + iw;
+          tl.C = Wp + iw;
           tl.ldc = u.m; tl.m0 = m0; tl.n0 = n0; tl.m = u.m; tl.n = u.k2; tl.op0 = opi; tl.nop = 1; tl.flags = 0;
           t1[it++] = tl;
         }
       iw += ((int64_t)u.m * u.k2 + 7) / 8 * 8;
     }
   });
-  // terms grouped by output block, stable in (segment, term) order
-  const int64_t nq = (int64_t)outs.size();
+  // terms grouped by output block, stable in (segment, term) order: a counting sort
+  // with one histogram per range of terms
   std::vector<int64_t> qstart(nq + 1, 0);
-  for (int64_t g = 0; g < n; g++) qstart[qof[g] + 1]++;
-  for (int64_t q = 0; q < nq; q++) qstart[q + 1] += qstart[q];
-  std::vector<int64_t> order(n);
   {
-    std::vector<int64_t> fill(qstart.begin(), qstart.end() - 1);
-    for (int64_t g = 0; g < n; g++) order[fill[qof[g]]++] = g;
+    const int R = (int)std::max<int64_t>(1, std::min<int64_t>(nthr, (n + 65535) / 65536));
+    u_hist.assign((size_t)R * nq, 0);
+    int64_t* hist = u_hist.data();
+    auto range = [&](int r) { return std::make_pair(n * r / R, n * (r + 1) / R); };
+    parallel_for(R, nthr, [&](int64_t r, int) {
+      const auto rg = range((int)r);
+      for (int64_t g = rg.first; g < rg.second; g++) hist[r * nq + qof[g]]++;
+    });
+    // qstart[q] = terms of blocks before q; hist[r][q] becomes the fill position of range r
+    int64_t pos = 0;
+    for (int64_t q = 0; q < nq; q++) {
+      qstart[q] = pos;
+      for (int r = 0; r < R; r++) {
+        const int64_t c = hist[r * nq + q];
+        hist[r * nq + q] = pos;
+        pos += c;
+      }
+    }
+    qstart[nq] = pos;
+    parallel_for(R, nthr, [&](int64_t r, int) {
+      const auto rg = range((int)r);
+      for (int64_t g = rg.first; g < rg.second; g++) order[hist[r * nq + qof[g]]++] = g;
+    });
   }
-  std::vector<int> seg_of(n);
-  for (int64_t s = 0; s < nseg; s++)
-    for (int64_t g = s0[s]; g < s0[s + 1]; g++) seg_of[g] = (int)s;
   // phase 2: one op per term at its grouped position, then the tiles of every block
   parallel_for((n + 65535) / 65536, nthr, [&](int64_t c, int) {
     for (int64_t pos = c * 65536; pos < std::min<int64_t>(n, (c + 1) * 65536); pos++) {
@@ -783,7 +834,7 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
       const double* cb = combo_base ? combo_base + segs[s].combo_base : nullptr;
       DevGemmOp op{};
       if (u.kind == 1) {
-        op.A = Wb.p + woff[g]; op.ta = 0; op.lda = u.m;
+        op.A = Wp + woff[g]; op.ta = 0; op.lda = u.m;
         op.B = resolve_with(u.C, cb); op.tb = u.tc; op.ldb = u.ldc;
         op.k = u.k2; op.alpha = 1.0; op.kind = 0;
       } else if (u.kind == 0) {
@@ -797,12 +848,6 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
       o2[pos] = op;
     }
   });
-  std::vector<int64_t> tstart(nq + 1, 0);
-  for (int64_t q = 0; q < nq; q++) {
-    const int m = outs[q].second.first, nn = outs[q].second.second;
-    tstart[q + 1] = tstart[q] + ((m > 0 && nn > 0) ? (int64_t)((m + 63) / 64) * ((nn + 63) / 64) : 0);
-  }
-  std::vector<DevTile> t2(tstart[nq]);
   parallel_for((nq + 1023) / 1024, nthr, [&](int64_t c, int) {
     for (int64_t q = c * 1024; q < std::min<int64_t>(nq, (c + 1) * 1024); q++) {
       const int m = outs[q].second.first, nn = outs[q].second.second;
@@ -818,22 +863,23 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
         }
     }
   });
-  if ((int64_t)t1.size() > INT32_MAX || (int64_t)t2.size() > INT32_MAX || n > INT32_MAX) throw Error{ERR_SIZE, "too many update tiles"};
   const double t_lists = now_s();
-  DBuf<DevGemmOp> d1, d2;
-  DBuf<DevTile> dt1, dt2;
-  d1.upload(o1, stream);
-  d2.upload(o2, stream);
-  dt1.upload(t1, stream);
-  dt2.upload(t2, stream);
+  u_d1.alloc(std::max<int64_t>(n1, 1));
+  u_d2.alloc(std::max<int64_t>(n, 1));
+  u_dt1.alloc(std::max<int64_t>(nt1, 1));
+  u_dt2.alloc(std::max<int64_t>(nt2, 1));
+  if (n1) hip_check(hipMemcpyAsync(u_d1.p, o1, n1 * sizeof(DevGemmOp), hipMemcpyHostToDevice, stream), "ops");
+  if (n) hip_check(hipMemcpyAsync(u_d2.p, o2, n * sizeof(DevGemmOp), hipMemcpyHostToDevice, stream), "ops");
+  if (nt1) hip_check(hipMemcpyAsync(u_dt1.p, t1, nt1 * sizeof(DevTile), hipMemcpyHostToDevice, stream), "tiles");
+  if (nt2) hip_check(hipMemcpyAsync(u_dt2.p, t2, nt2 * sizeof(DevTile), hipMemcpyHostToDevice, stream), "tiles");
   const double t_up = now_s();
-  launch_tiles(dt1.p, d1.p, (int)t1.size(), nullptr, nullptr, stream);
-  launch_tiles(dt2.p, d2.p, (int)t2.size(), nullptr, nullptr, stream);
+  launch_tiles(u_dt1.p, u_d1.p, (int)nt1, nullptr, nullptr, stream);
+  launch_tiles(u_dt2.p, u_d2.p, (int)nt2, nullptr, nullptr, stream);
   hip_check(hipGetLastError(), "update kernels");
   hip_check(hipStreamSynchronize(stream), "update");
   if (verbose)
-    std::fprintf(stderr, "    terms %lld: lists %.3f s, upload %.3f s, kernels %.3f s (%zu + %zu tiles)\n", (long long)n,
-                 t_lists - t_begin, t_up - t_lists, now_s() - t_up, t1.size(), t2.size());
+    std::fprintf(stderr, "    terms %lld: lists %.3f s, upload %.3f s, kernels %.3f s (%lld + %lld tiles)\n", (long long)n,
+                 t_lists - t_begin, t_up - t_lists, now_s() - t_up, (long long)nt1, (long long)nt2);
 }
 
 // DMRG::updateMovingRight/Left (DMRGoperators.cpp:243-907) for slot `index`.
@@ -863,7 +909,8 @@ void Ctx::update_operators(int index, bool mr) {
   const int nC = mr ? L - 1 - index : index + 1;
   for (int k = 0; k < nC; k++) Qsite.push_back(mr ? index + 1 + k : index - k);
   const double t0 = now_s();
-  UpdatePlanner up;
+  auto upp = std::make_unique<UpdatePlanner>();
+  UpdatePlanner& up = *upp;
   up.sec = &sec;
   up.irr = irreps.data();
   up.mx = mx.data();
@@ -922,10 +969,12 @@ void Ctx::update_operators(int index, bool mr) {
   for (auto& row : o.D) for (OpT* t : row) add_outs(t, out2);
   run_terms(up.segs[1], out1, arena.p);
   run_terms(up.segs[2], out2, arena.p);
+  const size_t ncombos = up.combos.size();
+  reap(std::move(upp));
   site = -1;
   if (verbose)
     std::fprintf(stderr, "update slot %d (%s): plan %.3f s (%lld terms, %zu combinations), device %.3f s\n", index,
-                 mr ? "right" : "left", t1 - t0, (long long)last_update_terms, up.combos.size(), now_s() - t1);
+                 mr ? "right" : "left", t1 - t0, (long long)last_update_terms, ncombos, now_s() - t1);
 }
 
 // ------------------------------------------------------------ Davidson -----

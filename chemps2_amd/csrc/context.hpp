@@ -1,6 +1,7 @@
 // Library context: problem data, sector dimensions, the device operator pool and the
 // per-site Heff work lists.
 #pragma once
+#include <thread>
 #include <tuple>
 
 #include "chm_internal.hpp"
@@ -114,6 +115,17 @@ struct Ctx : HostState {
   int site = -1;
   SLayout S;
   std::unique_ptr<HeffPlanner> plan;
+  // Work lists are released on a background thread while the GPU runs the step (tens
+  // of millions of records); the previous release is joined first.
+  std::thread reaper;
+  template <class T>
+  void reap(std::unique_ptr<T> p) {
+    if (reaper.joinable()) reaper.join();
+    reaper = std::thread([q = std::move(p)]() mutable { q.reset(); });
+  }
+  void reap_join() {
+    if (reaper.joinable()) reaper.join();
+  }
   double F_H = 0.0, B_H = 0.0;
   int64_t V = 0;
   DBuf<double> combo_arena, W, dx, dy, partial, scal;
@@ -141,6 +153,17 @@ struct Ctx : HostState {
   DBuf<DevPair> d_pairs;
   DBuf<DevKTerm> d_kterms;
   HostStage stage;
+  // Heff preparation: pair groups and term buffers kept across sites
+  struct PGrp { int out, in; int64_t q0, q1, elems; };
+  std::vector<std::vector<PGrp>> p_grp;
+  std::vector<std::vector<Term>> term_pool;
+  // update executor: staging and scratch kept across calls (no page faults per call)
+  HostStage ustage;
+  std::vector<int> u_qof, u_seg;
+  std::vector<int64_t> u_woff, u_order, u_hist;
+  DBuf<DevGemmOp> u_d1, u_d2;
+  DBuf<DevTile> u_dt1, u_dt2;
+  DBuf<double> u_W;
   double prep_s[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // seconds of the last prepare, by phase
   bool verbose = false;                         // CHM_VERBOSE: phase times on stderr
   int64_t item_elems = 32768;        // CHM_ITEM_ELEMS: matrix elements per work item

@@ -8,6 +8,8 @@
 #include <cassert>
 #include <cstring>
 
+#include <chrono>
+
 #include "heff_plan.hpp"
 #include "parallel.hpp"
 
@@ -125,7 +127,10 @@ Ref HeffPlanner::q_plus_l(OpT* Q, std::vector<OpT*>& Ls, int site, bool left, in
   return make_combo(std::move(src), sz);
 }
 
+static double wall() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
 void HeffPlanner::build() {  // Heff.cpp:43-248 (the loop over output blocks)
+  const double w0 = wall();
   const int nk = S->nk();
   const int CH = CHUNK;
   const int64_t nch = (nk + CH - 1) / CH;
@@ -146,6 +151,7 @@ void HeffPlanner::build() {  // Heff.cpp:43-248 (the loop over output blocks)
   }
   parallel_for(nch, nt, [&](int64_t c, int tid) {
     HeffPlanner& w = *workers[tid];
+    if (c < (int64_t)pool.size()) w.terms = std::move(pool[c]);
     w.terms.clear(); w.combos.clear(); w.combo_hash.clear(); w.combo_cache.clear(); w.diag.clear();
     w.flops = 0.0;
     const int k1 = (int)std::min<int64_t>(nk, (c + 1) * CH);
@@ -172,6 +178,7 @@ void HeffPlanner::build() {  // Heff.cpp:43-248 (the loop over output blocks)
   // shard walking the chunks in order; arena offsets follow shard order, then first
   // appearance, so the result does not depend on the thread count. The terms keep
   // chunk-local combination indices; ref() maps them through remap.
+  const double w1 = wall();
   terms.clear(); combos.clear(); combo_hash.clear(); combo_cache.clear(); diag.clear();
   combo_size = 0;
   flops = 0.0;
@@ -243,6 +250,7 @@ void HeffPlanner::build() {  // Heff.cpp:43-248 (the loop over output blocks)
       remap[c][q] = sbase[s] + shards[s].size[i];
     }
   });
+  const double w2 = wall();
   nterms = 0;
   size_t ndiag = 0;
   chunks.assign(nch, {});
@@ -254,6 +262,10 @@ void HeffPlanner::build() {  // Heff.cpp:43-248 (the loop over output blocks)
   }
   diag.reserve(ndiag);
   for (auto& p : parts) diag.insert(diag.end(), p.diag.begin(), p.diag.end());
+  parts.clear();
+  phase_s[0] = w1 - w0;
+  phase_s[1] = w2 - w1;
+  phase_s[2] = wall() - w2;
 }
 
 // ---------------------------------------------------------------- group 1 --

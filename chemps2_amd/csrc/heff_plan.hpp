@@ -44,12 +44,14 @@ struct HeffPlanner {
     return r;
   }
   std::vector<Term> terms;  // scratch of one chunk while it is generated
+  std::vector<std::vector<Term>> pool;  // buffers to reuse for the chunks (optional)
   std::vector<Combo> combos;
   int64_t combo_size = 0;
   std::vector<DiagTerm> diag;
   double flops = 0.0;  // F_H: 2mnk over every dgemm_ the reference issues at this site
 
   int nthreads = 0;  // 0: host_threads()
+  double phase_s[3] = {0, 0, 0};  // seconds: generation, combination merge, rest of build()
 
   void init(const Sectors* sec, const std::vector<int>* irreps, int nirrep, const double* mx, int L, const SLayout* S,
             OpSet* Lo, OpSet* Ro);
