@@ -221,7 +221,8 @@ struct DevTile {  // one 64 x 64 output tile and the products that accumulate in
 // columns (and inputs of the same bound): one workgroup per block keeps y_kappa in
 // registers and runs every term of the block; terms sharing an input block kappa'
 // form a group that stages x_kappa' in LDS once.
-constexpr int CHAIN_MAXD = 36;  // 6 LDS arrays of 36^2 doubles fit in 64 KB
+constexpr int CHAIN_MAXD = 64;  // larger blocks run as GEMM tiles
+constexpr int CHAIN_SMALLP = 1296;  // block products up to 36^2: 6 LDS arrays (62 KB), double-buffered
 struct DevCBlock {
   int64_t yoff;
   int dL, dR;
@@ -329,7 +330,7 @@ struct DevSvd {
   int wide;
   int pad;
 };
-constexpr int SVD_JACOBI_MAXN = 64;  // largest n of the batched Jacobi kernel
+constexpr int SVD_JACOBI_MAXN = 256;  // largest n of the batched Jacobi kernel
 
 // Launchers (kernels.hip).
 void launch_copies(const DevCopy* items, int n, hipStream_t s);
@@ -340,8 +341,11 @@ void launch_tiles(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const 
 void launch_combos(const DevCombo* d_combos, const DevSrc* d_src, int ncombo, const int64_t* d_chunk_start,
                    int64_t nchunks, hipStream_t s);
 void launch_diag(const DevDiagBlock* blocks, const DevDiag* terms, int nblocks, hipStream_t s);
+// small: block products <= CHAIN_SMALLP (62 KB of LDS); large: up to CHAIN_MAXD^2 with
+// 4 single-buffered LDS arrays (up to 128 KB, allowed by chain_kernel_setup)
 void launch_chain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups, const DevCTerm* terms, const double* x,
-                  double* y, int P, hipStream_t s);
+                  double* y, int P, bool large, hipStream_t s);
+bool chain_kernel_setup();  // false: the large variant cannot get its LDS on this device
 void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs, double* M, hipStream_t s);
 void launch_bspmv(const DevBItem* items, int nitems, const DevBRed* reds, int nreds, double* P, const double* M,
                   const int* col, const double* x, double* y, hipStream_t s);

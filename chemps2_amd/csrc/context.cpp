@@ -393,19 +393,21 @@ void Ctx::prepare(int st) {
   const double t_assemble = now_s();
 
   // ---- chained terms: output blocks whose own and input dimensions are at most
-  // CHAIN_MAXD run in the output-stationary kernel; the rest as GEMM tiles
+  // CHAIN_MAXD run in the output-stationary kernel (two launches: block products up to
+  // CHAIN_SMALLP, and larger ones when the device grants the LDS); the rest as GEMM tiles
   {
-    std::vector<DevCBlock> cb;
+    std::vector<DevCBlock> cb, cbl;
     std::vector<DevCGroup> cg;
     std::vector<DevCTerm> ct;
     std::vector<const Term*> rest;
-    int Pmax = 1;
+    int Pmax = 1, PmaxL = 1;
+    const int maxd = chain_large_ok ? CHAIN_MAXD : 36;
     for (size_t q0 = 0; q0 < chain.size();) {
       const int k = chain[q0]->out;
       size_t q1 = q0;
-      bool small = use_chain_kernel && dLof(k) <= CHAIN_MAXD && dRof(k) <= CHAIN_MAXD;
+      bool small = use_chain_kernel && dLof(k) <= maxd && dRof(k) <= maxd;
       while (q1 < chain.size() && chain[q1]->out == k) {
-        if (dLof(chain[q1]->in) > CHAIN_MAXD || dRof(chain[q1]->in) > CHAIN_MAXD) small = false;
+        if (dLof(chain[q1]->in) > maxd || dRof(chain[q1]->in) > maxd) small = false;
         q1++;
       }
       if (!small) {
@@ -415,14 +417,14 @@ void Ctx::prepare(int st) {
       }
       DevCBlock B{};
       B.yoff = S.k2i[k]; B.dL = dLof(k); B.dR = dRof(k); B.g0 = (int)cg.size();
-      Pmax = std::max(Pmax, B.dL * B.dR);
+      int Pk = B.dL * B.dR;
       for (size_t a = q0; a < q1;) {
         const int in = chain[a]->in;
         size_t e = a;
         while (e < q1 && chain[e]->in == in) e++;
         DevCGroup G{};
         G.xoff = S.k2i[in]; G.dLi = dLof(in); G.dRi = dRof(in); G.t0 = (int)ct.size();
-        Pmax = std::max({Pmax, G.dLi * G.dRi, B.dL * G.dRi, G.dLi * B.dR, B.dL * G.dLi, G.dRi * B.dR});
+        Pk = std::max({Pk, G.dLi * G.dRi, B.dL * G.dRi, G.dLi * B.dR, B.dL * G.dLi, G.dRi * B.dR});
         for (size_t q = a; q < e; q++) {
           const Term& t = *chain[q];
           const int dL = B.dL, dR = B.dR, dLi = G.dLi, dRi = G.dRi;
@@ -444,7 +446,8 @@ void Ctx::prepare(int st) {
         a = e;
       }
       B.ng = (int)cg.size() - B.g0;
-      cb.push_back(B);
+      if (Pk <= CHAIN_SMALLP) { cb.push_back(B); Pmax = std::max(Pmax, Pk); }
+      else { cbl.push_back(B); PmaxL = std::max(PmaxL, Pk); }
       q0 = q1;
     }
     if ((int64_t)ct.size() > INT32_MAX || (int64_t)cg.size() > INT32_MAX) throw Error{ERR_SIZE, "too many chained terms"};
@@ -452,7 +455,10 @@ void Ctx::prepare(int st) {
     cgroups.upload(cg, stream);
     cterms.upload(ct, stream);
     ncblocks = (int)cb.size();
-    chain_P = std::min((Pmax + 31) / 32 * 32, CHAIN_MAXD * CHAIN_MAXD);
+    chain_P = std::min((Pmax + 31) / 32 * 32, CHAIN_SMALLP);
+    cblocks_l.upload(cbl, stream);
+    ncblocks_l = (int)cbl.size();
+    chain_P_l = std::min((PmaxL + 31) / 32 * 32, CHAIN_MAXD * CHAIN_MAXD);
     stats[15] = (int64_t)ct.size();
     chain.swap(rest);
   }
@@ -606,8 +612,14 @@ void Ctx::prepare(int st) {
                           t_end - t_start};
     for (int q = 0; q < 6; q++) prep_s[q] = ph[q];
     if (verbose)
-      std::fprintf(stderr, "prepare site %d: plan %.3f s (generate %.3f, merge %.3f, finish %.3f), combinations %.3f s, lists %.3f s, assembly %.3f s, rest %.3f s, total %.3f s\n",
-                   st, ph[0], plan->phase_s[0], plan->phase_s[1], plan->phase_s[2], ph[1], ph[2], ph[3], ph[4], ph[5]);
+    {
+      int maxd = 0;
+      for (int k = 0; k < S.nk(); k++) maxd = std::max({maxd, S.dL[k], S.dR[k]});
+      std::fprintf(stderr, "prepare site %d: plan %.3f s (generate %.3f, merge %.3f, finish %.3f), combinations %.3f s, lists %.3f s, assembly %.3f s, rest %.3f s, total %.3f s; "
+                   "max block dim %d, terms explicit %lld / chain kernel %lld / tiles %lld\n",
+                   st, ph[0], plan->phase_s[0], plan->phase_s[1], plan->phase_s[2], ph[1], ph[2], ph[3], ph[4], ph[5], maxd,
+                   (long long)(plan->nterms - stats[7]), (long long)stats[15], (long long)(stats[7] - stats[15]));
+    }
   }
   stats[1] = (int64_t)plan->combos.size();
   stats[2] = plan->combo_size;
@@ -629,7 +641,8 @@ void Ctx::apply(const double* xp, double* yp) {
   launch_bspmv(bitems.p, nbitems, breds.p, nbreds, bpart.p, Mvals.p, Mcol.p, xp, yp, stream);
   if (timing) hip_check(hipEventRecord(ev[2], stream), "event");
   launch_tiles(tiles2.p, ops2.p, ntiles2, xp, yp, stream);
-  launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, chain_P, stream);
+  launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, chain_P, false, stream);
+  launch_chain(cblocks_l.p, ncblocks_l, cgroups.p, cterms.p, xp, yp, chain_P_l, true, stream);
   if (timing) hip_check(hipEventRecord(ev[3], stream), "event");
   hip_check(hipGetLastError(), "Heff kernels");
   if (timing) {
@@ -1240,6 +1253,8 @@ int chm_ctx_create(int device, chm_ctx** out) {
     delete ctx;
     return CHM_ERR_HIP;
   }
+  ctx->c.chain_large_ok = chm::chain_kernel_setup();
+  (void)hipGetLastError();  // a refused attribute leaves no error behind
   for (auto& e : ctx->c.ev)
     if (hipEventCreate(&e) != hipSuccess) { delete ctx; return CHM_ERR_HIP; }
   *out = ctx;

@@ -150,6 +150,10 @@ struct Ctx : HostState {
   DBuf<DevCTerm> cterms;
   int ncblocks = 0;
   int chain_P = 1;  // doubles per LDS array of the chained kernel
+  DBuf<DevCBlock> cblocks_l;  // blocks of the large variant (same groups and terms)
+  int ncblocks_l = 0;
+  int chain_P_l = 1;
+  bool chain_large_ok = false;  // the device grants the large variant its LDS
   DBuf<DevPair> d_pairs;
   DBuf<DevKTerm> d_kterms;
   HostStage stage;

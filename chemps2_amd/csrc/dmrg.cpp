@@ -147,6 +147,7 @@ void Ctx::join(int st) {
 // and the new MPS tensors (U, and sqrt((2SM+1)/(2SR+1)) V^T, with the singular values
 // on the side the sweep moves to).
 double Ctx::split(int st, int D, bool mr, bool change) {
+  const double t0 = now();
   if (s_site != st) throw Error{ERR_STATE, "split: the two-site object is not at this site"};
   if ((int)fci.size() != L + 1) throw Error{ERR_STATE, "split needs the FCI dimensions (chm_set_sectors_auto)"};
   if (!rocblas) {
@@ -280,6 +281,7 @@ double Ctx::split(int st, int D, bool mr, bool change) {
     }
     hip_check(hipStreamSynchronize(stream), "split SVD");
   }
+  const double t1 = now();
   std::vector<double> sv(std::max<int64_t>(ssz, 1));
   copy_sync(stream, sv.data(), Sg.p, sv.size() * sizeof(double), hipMemcpyDeviceToHost, "singular values");
   {
@@ -360,6 +362,13 @@ double Ctx::split(int st, int D, bool mr, bool change) {
     launch_copies(d_items.p, (int)items.size(), stream);
     hip_check(hipGetLastError(), "split scatter");
     hip_check(hipStreamSynchronize(stream), "split");
+  }
+  if (verbose) {
+    int nj = 0, nr = 0, maxcd = 0;
+    for (auto& c : cs)
+      if (c.cd > 0) { (c.jac ? nj : nr)++; maxcd = std::max(maxcd, c.cd); }
+    std::fprintf(stderr, "  split %d: %zu centre sectors (%d Jacobi, %d rocSOLVER, largest %d): SVD %.3f s, rest %.3f s\n", st,
+                 cs.size(), nj, nr, maxcd, t1 - t0, now() - t1);
   }
   s_site = -1;
   site = -1;

@@ -134,13 +134,6 @@ void HeffPlanner::build() {  // Heff.cpp:43-248 (the loop over output blocks)
   const int nk = S->nk();
   const int CH = CHUNK;
   const int64_t nch = (nk + CH - 1) / CH;
-  struct Part {
-    std::vector<Term> terms;
-    std::vector<Combo> combos;
-    std::vector<uint64_t> hash;
-    std::vector<DiagTerm> diag;
-    double flops = 0.0;
-  };
   std::vector<Part> parts(nch);
   const int nt = (int)std::min<int64_t>(nthreads > 0 ? nthreads : host_threads(), std::max<int64_t>(nch, 1));
   std::vector<std::unique_ptr<HeffPlanner>> workers(nt);
@@ -262,7 +255,7 @@ void HeffPlanner::build() {  // Heff.cpp:43-248 (the loop over output blocks)
   }
   diag.reserve(ndiag);
   for (auto& p : parts) diag.insert(diag.end(), p.diag.begin(), p.diag.end());
-  parts.clear();
+  leftover = std::move(parts);
   phase_s[0] = w1 - w0;
   phase_s[1] = w2 - w1;
   phase_s[2] = wall() - w2;

@@ -45,6 +45,14 @@ struct HeffPlanner {
   }
   std::vector<Term> terms;  // scratch of one chunk while it is generated
   std::vector<std::vector<Term>> pool;  // buffers to reuse for the chunks (optional)
+  struct Part {  // one chunk's output before the merge
+    std::vector<Term> terms;
+    std::vector<Combo> combos;
+    std::vector<uint64_t> hash;
+    std::vector<DiagTerm> diag;
+    double flops = 0.0;
+  };
+  std::vector<Part> leftover;  // released with the planner (duplicate combinations)
   std::vector<Combo> combos;
   int64_t combo_size = 0;
   std::vector<DiagTerm> diag;

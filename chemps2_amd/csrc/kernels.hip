@@ -242,13 +242,12 @@ __global__ __launch_bounds__(256) void k_diag(const DevDiagBlock* __restrict__ b
 
 // Chained products, output-stationary (see DevCBlock). Each thread owns the elements
 // e = tid + 256 q of y_kappa (column-major dL x dR). Per term the operator blocks are
-// copied into LDS (contiguous, coalesced) in alternating buffers, so one barrier
-// separates the copy from the use; two-operand terms add one barrier around their
-// LDS intermediate, formed in the cheaper association order chosen on the host.
-// Dynamic LDS: 6 arrays of P doubles (x, intermediate, 2 x A, 2 x B), P = the
-// largest block product of the launch (at most CHAIN_MAXD^2).
-constexpr int CHAIN_PER = CHAIN_MAXD * CHAIN_MAXD / 256 + 1;
-
+// copied into LDS (contiguous, coalesced); two-operand terms form their intermediate in
+// LDS in the cheaper association order chosen on the host. DB (double-buffered): the
+// copies alternate between two buffers, so one barrier separates copy and use; without
+// it a barrier also precedes every copy. Dynamic LDS: 6 (DB) or 4 arrays of P doubles
+// (x, intermediate, A and B copies), P = the largest block product of the launch.
+template <int PER, bool DB>
 __global__ __launch_bounds__(256) void k_chain(const DevCBlock* __restrict__ blocks, const DevCGroup* __restrict__ groups,
                                                const DevCTerm* __restrict__ terms, const double* __restrict__ x,
                                                double* __restrict__ y, int P) {
@@ -258,9 +257,9 @@ __global__ __launch_bounds__(256) void k_chain(const DevCBlock* __restrict__ blo
   const DevCBlock b = blocks[blockIdx.x];
   const int tid = threadIdx.x;
   const int dL = b.dL, dR = b.dR, n = dL * dR;
-  double acc[CHAIN_PER];
+  double acc[PER];
 #pragma unroll
-  for (int q = 0; q < CHAIN_PER; q++) acc[q] = 0.0;
+  for (int q = 0; q < PER; q++) acc[q] = 0.0;
   int buf = 0;
   for (int g = b.g0; g < b.g0 + b.ng; g++) {
     const DevCGroup G = groups[g];
@@ -271,7 +270,8 @@ __global__ __launch_bounds__(256) void k_chain(const DevCBlock* __restrict__ blo
       const DevCTerm T = terms[t];
       double* As = lds + (2 + 2 * buf) * P;
       double* Bs = lds + (3 + 2 * buf) * P;
-      buf ^= 1;
+      if (DB) buf ^= 1;
+      else __syncthreads();  // every reader of the previous term's copies is done
       if (T.A) for (int e = tid; e < dL * dLi; e += 256) As[e] = T.A[e];
       if (T.B) for (int e = tid; e < dRi * dR; e += 256) Bs[e] = T.B[e];
       __syncthreads();
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(256) void k_chain(const DevCBlock* __restrict__ blo
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < CHAIN_PER; q++) {
+        for (int q = 0; q < PER; q++) {
           const int e = tid + 256 * q;
           if (e < n) {
             const int i = e % dL, j = e / dL;
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(256) void k_chain(const DevCBlock* __restrict__ blo
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < CHAIN_PER; q++) {
+        for (int q = 0; q < PER; q++) {
           const int e = tid + 256 * q;
           if (e < n) {
             const int i = e % dL, j = e / dL;
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(256) void k_chain(const DevCBlock* __restrict__ blo
         }
       } else if (T.mode == 1) {  // y += al op(A) x
 #pragma unroll
-        for (int q = 0; q < CHAIN_PER; q++) {
+        for (int q = 0; q < PER; q++) {
           const int e = tid + 256 * q;
           if (e < n) {
             const int i = e % dL, j = e / dL;
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(256) void k_chain(const DevCBlock* __restrict__ blo
         }
       } else if (T.mode == 2) {  // y += al x op(B)
 #pragma unroll
-        for (int q = 0; q < CHAIN_PER; q++) {
+        for (int q = 0; q < PER; q++) {
           const int e = tid + 256 * q;
           if (e < n) {
             const int i = e % dL, j = e / dL;
@@ -339,7 +339,7 @@ __global__ __launch_bounds__(256) void k_chain(const DevCBlock* __restrict__ blo
         }
       } else {  // y += al x
 #pragma unroll
-        for (int q = 0; q < CHAIN_PER; q++) {
+        for (int q = 0; q < PER; q++) {
           const int e = tid + 256 * q;
           if (e < n) acc[q] += al * xs[e];
         }
@@ -347,11 +347,13 @@ __global__ __launch_bounds__(256) void k_chain(const DevCBlock* __restrict__ blo
     }
   }
 #pragma unroll
-  for (int q = 0; q < CHAIN_PER; q++) {
+  for (int q = 0; q < PER; q++) {
     const int e = tid + 256 * q;
     if (e < n) y[b.yoff + e] += acc[q];
   }
 }
+constexpr int CHAIN_PER_S = (CHAIN_SMALLP + 255) / 256;
+constexpr int CHAIN_PER_L = (CHAIN_MAXD * CHAIN_MAXD + 255) / 256;
 
 // --------------------------------------------------------- vector kernels --
 __global__ __launch_bounds__(256) void k_dot_partial(const double* __restrict__ a, const double* __restrict__ b,
@@ -661,9 +663,19 @@ void launch_diag(const DevDiagBlock* blocks, const DevDiag* terms, int nblocks, 
 }
 
 void launch_chain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups, const DevCTerm* terms, const double* x,
-                  double* y, int P, hipStream_t s) {
-  if (nblocks > 0)
-    hipLaunchKernelGGL(k_chain, dim3(nblocks), dim3(256), (size_t)6 * P * sizeof(double), s, blocks, groups, terms, x, y, P);
+                  double* y, int P, bool large, hipStream_t s) {
+  if (nblocks <= 0) return;
+  if (large)
+    hipLaunchKernelGGL((k_chain<CHAIN_PER_L, false>), dim3(nblocks), dim3(256), (size_t)4 * P * sizeof(double), s, blocks,
+                       groups, terms, x, y, P);
+  else
+    hipLaunchKernelGGL((k_chain<CHAIN_PER_S, true>), dim3(nblocks), dim3(256), (size_t)6 * P * sizeof(double), s, blocks,
+                       groups, terms, x, y, P);
+}
+
+bool chain_kernel_setup() {
+  return hipFuncSetAttribute((const void*)k_chain<CHAIN_PER_L, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             4 * CHAIN_MAXD * CHAIN_MAXD * (int)sizeof(double)) == hipSuccess;
 }
 
 
