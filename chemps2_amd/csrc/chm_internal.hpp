@@ -223,10 +223,14 @@ struct DevTile {  // one 64 x 64 output tile and the products that accumulate in
 // form a group that stages x_kappa' in LDS once.
 constexpr int CHAIN_MAXD = 64;  // larger blocks run as GEMM tiles
 constexpr int CHAIN_SMALLP = 1296;  // block products up to 36^2: 6 LDS arrays (62 KB), double-buffered
-struct DevCBlock {
-  int64_t yoff;
+constexpr int CHAIN_GROUP_TERMS = 32;  // terms per input group record
+constexpr int CHAIN_ITEM_TERMS = 64;   // terms per work item (load balance across workgroups)
+struct DevCBlock {  // one work item: input groups [g0, g0 + ng) of one output block
+  int64_t out;        // y offset (direct) or partial offset
   int dL, dR;
   int g0, ng;
+  int direct;
+  int pad;
 };
 struct DevCGroup {
   int64_t xoff;
@@ -344,7 +348,8 @@ void launch_diag(const DevDiagBlock* blocks, const DevDiag* terms, int nblocks, 
 // small: block products <= CHAIN_SMALLP (62 KB of LDS); large: up to CHAIN_MAXD^2 with
 // 4 single-buffered LDS arrays (up to 128 KB, allowed by chain_kernel_setup)
 void launch_chain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups, const DevCTerm* terms, const double* x,
-                  double* y, int P, bool large, hipStream_t s);
+                  double* y, double* part, int P, bool large, hipStream_t s);
+void launch_chain_reduce(const DevBRed* reds, int nreds, const double* part, double* y, hipStream_t s);
 bool chain_kernel_setup();  // false: the large variant cannot get its LDS on this device
 void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs, double* M, hipStream_t s);
 void launch_bspmv(const DevBItem* items, int nitems, const DevBRed* reds, int nreds, double* P, const double* M,

@@ -399,6 +399,8 @@ void Ctx::prepare(int st) {
     std::vector<DevCBlock> cb, cbl;
     std::vector<DevCGroup> cg;
     std::vector<DevCTerm> ct;
+    std::vector<DevBRed> cr;
+    int64_t cpart_n = 0;
     std::vector<const Term*> rest;
     int Pmax = 1, PmaxL = 1;
     const int maxd = chain_large_ok ? CHAIN_MAXD : 36;
@@ -416,7 +418,8 @@ void Ctx::prepare(int st) {
         continue;
       }
       DevCBlock B{};
-      B.yoff = S.k2i[k]; B.dL = dLof(k); B.dR = dRof(k); B.g0 = (int)cg.size();
+      B.dL = dLof(k); B.dR = dRof(k);
+      const int g_first = (int)cg.size();
       int Pk = B.dL * B.dR;
       for (size_t a = q0; a < q1;) {
         const int in = chain[a]->in;
@@ -441,13 +444,41 @@ void Ctx::prepare(int st) {
           }
           ct.push_back(T);
         }
-        G.nt = (int)ct.size() - G.t0;
-        cg.push_back(G);
+        // groups of at most CHAIN_GROUP_TERMS terms (x is staged once per group)
+        const int t_end = (int)ct.size();
+        for (int t0 = G.t0; t0 < t_end; t0 += CHAIN_GROUP_TERMS) {
+          DevCGroup Gs = G;
+          Gs.t0 = t0;
+          Gs.nt = std::min(CHAIN_GROUP_TERMS, t_end - t0);
+          cg.push_back(Gs);
+        }
         a = e;
       }
-      B.ng = (int)cg.size() - B.g0;
-      if (Pk <= CHAIN_SMALLP) { cb.push_back(B); Pmax = std::max(Pmax, Pk); }
-      else { cbl.push_back(B); PmaxL = std::max(PmaxL, Pk); }
+      // work items of about CHAIN_ITEM_TERMS terms; a block with several items sums
+      // their partials in k_chain_reduce
+      std::vector<DevCBlock> items;
+      for (int g = g_first; g < (int)cg.size();) {
+        DevCBlock it = B;
+        it.g0 = g;
+        int nt = 0;
+        while (g < (int)cg.size() && (nt == 0 || nt + cg[g].nt <= CHAIN_ITEM_TERMS)) nt += cg[g++].nt;
+        it.ng = g - it.g0;
+        items.push_back(it);
+      }
+      const int64_t nel = (int64_t)B.dL * B.dR;
+      if (items.size() == 1) {
+        items[0].direct = 1;
+        items[0].out = S.k2i[k];
+      } else {
+        cr.push_back({S.k2i[k], cpart_n, (int)nel, (int)items.size()});
+        for (auto& it : items) { it.direct = 0; it.out = cpart_n; cpart_n += nel; }
+      }
+      for (auto& it : items) {
+        if (Pk <= CHAIN_SMALLP) cb.push_back(it);
+        else cbl.push_back(it);
+      }
+      if (Pk <= CHAIN_SMALLP) Pmax = std::max(Pmax, Pk);
+      else PmaxL = std::max(PmaxL, Pk);
       q0 = q1;
     }
     if ((int64_t)ct.size() > INT32_MAX || (int64_t)cg.size() > INT32_MAX) throw Error{ERR_SIZE, "too many chained terms"};
@@ -458,6 +489,9 @@ void Ctx::prepare(int st) {
     chain_P = std::min((Pmax + 31) / 32 * 32, CHAIN_SMALLP);
     cblocks_l.upload(cbl, stream);
     ncblocks_l = (int)cbl.size();
+    creds.upload(cr, stream);
+    ncreds = (int)cr.size();
+    cpart.alloc(std::max<int64_t>(cpart_n, 1));
     chain_P_l = std::min((PmaxL + 31) / 32 * 32, CHAIN_MAXD * CHAIN_MAXD);
     stats[15] = (int64_t)ct.size();
     chain.swap(rest);
@@ -641,8 +675,9 @@ void Ctx::apply(const double* xp, double* yp) {
   launch_bspmv(bitems.p, nbitems, breds.p, nbreds, bpart.p, Mvals.p, Mcol.p, xp, yp, stream);
   if (timing) hip_check(hipEventRecord(ev[2], stream), "event");
   launch_tiles(tiles2.p, ops2.p, ntiles2, xp, yp, stream);
-  launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, chain_P, false, stream);
-  launch_chain(cblocks_l.p, ncblocks_l, cgroups.p, cterms.p, xp, yp, chain_P_l, true, stream);
+  launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, cpart.p, chain_P, false, stream);
+  launch_chain(cblocks_l.p, ncblocks_l, cgroups.p, cterms.p, xp, yp, cpart.p, chain_P_l, true, stream);
+  launch_chain_reduce(creds.p, ncreds, cpart.p, yp, stream);
   if (timing) hip_check(hipEventRecord(ev[3], stream), "event");
   hip_check(hipGetLastError(), "Heff kernels");
   if (timing) {

@@ -154,6 +154,9 @@ struct Ctx : HostState {
   int ncblocks_l = 0;
   int chain_P_l = 1;
   bool chain_large_ok = false;  // the device grants the large variant its LDS
+  DBuf<DevBRed> creds;          // output blocks split over several work items
+  int ncreds = 0;
+  DBuf<double> cpart;
   DBuf<DevPair> d_pairs;
   DBuf<DevKTerm> d_kterms;
   HostStage stage;

@@ -240,26 +240,73 @@ __global__ __launch_bounds__(256) void k_diag(const DevDiagBlock* __restrict__ b
   }
 }
 
-// Chained products, output-stationary (see DevCBlock). Each thread owns the elements
-// e = tid + 256 q of y_kappa (column-major dL x dR). Per term the operator blocks are
-// copied into LDS (contiguous, coalesced); two-operand terms form their intermediate in
-// LDS in the cheaper association order chosen on the host. DB (double-buffered): the
-// copies alternate between two buffers, so one barrier separates copy and use; without
-// it a barrier also precedes every copy. Dynamic LDS: 6 (DB) or 4 arrays of P doubles
-// (x, intermediate, A and B copies), P = the largest block product of the launch.
-template <int PER, bool DB>
+// Chained products, output-stationary (see DevCBlock). Every product is computed in
+// 2 x 2 register tiles (4 LDS reads per 4 FMAs); tile u = ti + RT tj covers rows
+// 2ti, 2ti+1 and columns 2tj, 2tj+1, clamped at odd edges (duplicates are computed and
+// not stored). Each thread owns the output tiles u = tid + 256 q of its block. Per term
+// the operator blocks are copied into LDS (contiguous, coalesced); two-operand terms
+// form their intermediate in LDS in the cheaper association order chosen on the host.
+// DB (double-buffered): the copies alternate between two buffers, so one barrier
+// separates copy and use; without it a barrier also precedes every copy. Dynamic LDS:
+// 6 (DB) or 4 arrays of P doubles (x, intermediate, A and B copies), P = the largest
+// block product of the launch. A work item covers a range of input groups of one
+// output block; it adds its sum to y (direct) or stores it as a partial for
+// k_chain_reduce.
+struct Tile22 {
+  int i0, i1, j0, j1;
+};
+
+__device__ inline Tile22 tile22(int u, int rows, int cols) {
+  const int rt = (rows + 1) >> 1;
+  Tile22 t;
+  t.i0 = 2 * (u % rt);
+  t.j0 = 2 * (u / rt);
+  t.i1 = min(t.i0 + 1, rows - 1);
+  t.j1 = min(t.j0 + 1, cols - 1);
+  return t;
+}
+
+// s[r + 2c] = sum_k P(i_r, k) Q(k, j_c), P(i, k) = P[i * pi + k * pk], Q(k, j) = Q[k * qk + j * qj]
+__device__ inline void mm22(const double* __restrict__ P, int pi, int pk, const double* __restrict__ Q, int qk, int qj,
+                            int K, const Tile22& t, double s[4]) {
+  double s00 = 0.0, s10 = 0.0, s01 = 0.0, s11 = 0.0;
+  const double* p0 = P + t.i0 * pi;
+  const double* p1 = P + t.i1 * pi;
+  const double* q0 = Q + t.j0 * qj;
+  const double* q1 = Q + t.j1 * qj;
+  for (int k = 0; k < K; k++) {
+    const double a0 = p0[k * pk], a1 = p1[k * pk], b0 = q0[k * qk], b1 = q1[k * qk];
+    s00 += a0 * b0;
+    s10 += a1 * b0;
+    s01 += a0 * b1;
+    s11 += a1 * b1;
+  }
+  s[0] = s00; s[1] = s10; s[2] = s01; s[3] = s11;
+}
+
+__device__ inline void store22(double* __restrict__ D, int ld, int rows, int cols, const Tile22& t, const double s[4]) {
+  D[t.i0 + ld * t.j0] = s[0];
+  if (t.i0 + 1 < rows) D[t.i0 + 1 + ld * t.j0] = s[1];
+  if (t.j0 + 1 < cols) {
+    D[t.i0 + ld * (t.j0 + 1)] = s[2];
+    if (t.i0 + 1 < rows) D[t.i0 + 1 + ld * (t.j0 + 1)] = s[3];
+  }
+}
+
+template <int PERT, bool DB>
 __global__ __launch_bounds__(256) void k_chain(const DevCBlock* __restrict__ blocks, const DevCGroup* __restrict__ groups,
                                                const DevCTerm* __restrict__ terms, const double* __restrict__ x,
-                                               double* __restrict__ y, int P) {
+                                               double* __restrict__ y, double* __restrict__ part, int P) {
   extern __shared__ double lds[];
   double* xs = lds;
   double* ts = lds + P;
   const DevCBlock b = blocks[blockIdx.x];
   const int tid = threadIdx.x;
-  const int dL = b.dL, dR = b.dR, n = dL * dR;
-  double acc[PER];
+  const int dL = b.dL, dR = b.dR;
+  const int ntile = ((dL + 1) >> 1) * ((dR + 1) >> 1);
+  double acc[PERT][4];
 #pragma unroll
-  for (int q = 0; q < PER; q++) acc[q] = 0.0;
+  for (int q = 0; q < PERT; q++) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.0;
   int buf = 0;
   for (int g = b.g0; g < b.g0 + b.ng; g++) {
     const DevCGroup G = groups[g];
@@ -279,81 +326,111 @@ __global__ __launch_bounds__(256) void k_chain(const DevCBlock* __restrict__ blo
       // op(A)(i, k) = As[i * ai + k * ak], op(B)(k, j) = Bs[k * bk + j * bj]
       const int ai = T.ta ? T.lda : 1, ak = T.ta ? 1 : T.lda;
       const int bk = T.tb ? T.ldb : 1, bj = T.tb ? 1 : T.ldb;
+      double s[4];
       if (T.mode == 3) {  // ts = op(A) x  (dL x dRi), then y += al ts op(B)
-        for (int e = tid; e < dL * dRi; e += 256) {
-          const int i = e % dL, j = e / dL;
-          double s = 0.0;
-          for (int k = 0; k < dLi; k++) s += As[i * ai + k * ak] * xs[k + dLi * j];
-          ts[e] = s;
+        const int n1 = ((dL + 1) >> 1) * ((dRi + 1) >> 1);
+        for (int u = tid; u < n1; u += 256) {
+          const Tile22 tl = tile22(u, dL, dRi);
+          mm22(As, ai, ak, xs, 1, dLi, dLi, tl, s);
+          store22(ts, dL, dL, dRi, tl, s);
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-          const int e = tid + 256 * q;
-          if (e < n) {
-            const int i = e % dL, j = e / dL;
-            double s = 0.0;
-            for (int k = 0; k < dRi; k++) s += ts[i + dL * k] * Bs[k * bk + j * bj];
-            acc[q] += al * s;
+        for (int q = 0; q < PERT; q++) {
+          const int u = tid + 256 * q;
+          if (u < ntile) {
+            const Tile22 tl = tile22(u, dL, dR);
+            mm22(ts, 1, dL, Bs, bk, bj, dRi, tl, s);
+            for (int r = 0; r < 4; r++) acc[q][r] += al * s[r];
           }
         }
       } else if (T.mode == 4) {  // ts = x op(B)  (dLi x dR), then y += al op(A) ts
-        for (int e = tid; e < dLi * dR; e += 256) {
-          const int i = e % dLi, j = e / dLi;
-          double s = 0.0;
-          for (int k = 0; k < dRi; k++) s += xs[i + dLi * k] * Bs[k * bk + j * bj];
-          ts[e] = s;
+        const int n1 = ((dLi + 1) >> 1) * ((dR + 1) >> 1);
+        for (int u = tid; u < n1; u += 256) {
+          const Tile22 tl = tile22(u, dLi, dR);
+          mm22(xs, 1, dLi, Bs, bk, bj, dRi, tl, s);
+          store22(ts, dLi, dLi, dR, tl, s);
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-          const int e = tid + 256 * q;
-          if (e < n) {
-            const int i = e % dL, j = e / dL;
-            double s = 0.0;
-            for (int k = 0; k < dLi; k++) s += As[i * ai + k * ak] * ts[k + dLi * j];
-            acc[q] += al * s;
+        for (int q = 0; q < PERT; q++) {
+          const int u = tid + 256 * q;
+          if (u < ntile) {
+            const Tile22 tl = tile22(u, dL, dR);
+            mm22(As, ai, ak, ts, 1, dLi, dLi, tl, s);
+            for (int r = 0; r < 4; r++) acc[q][r] += al * s[r];
           }
         }
       } else if (T.mode == 1) {  // y += al op(A) x
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-          const int e = tid + 256 * q;
-          if (e < n) {
-            const int i = e % dL, j = e / dL;
-            double s = 0.0;
-            for (int k = 0; k < dLi; k++) s += As[i * ai + k * ak] * xs[k + dLi * j];
-            acc[q] += al * s;
+        for (int q = 0; q < PERT; q++) {
+          const int u = tid + 256 * q;
+          if (u < ntile) {
+            const Tile22 tl = tile22(u, dL, dR);
+            mm22(As, ai, ak, xs, 1, dLi, dLi, tl, s);
+            for (int r = 0; r < 4; r++) acc[q][r] += al * s[r];
           }
         }
       } else if (T.mode == 2) {  // y += al x op(B)
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-          const int e = tid + 256 * q;
-          if (e < n) {
-            const int i = e % dL, j = e / dL;
-            double s = 0.0;
-            for (int k = 0; k < dRi; k++) s += xs[i + dLi * k] * Bs[k * bk + j * bj];
-            acc[q] += al * s;
+        for (int q = 0; q < PERT; q++) {
+          const int u = tid + 256 * q;
+          if (u < ntile) {
+            const Tile22 tl = tile22(u, dL, dR);
+            mm22(xs, 1, dLi, Bs, bk, bj, dRi, tl, s);
+            for (int r = 0; r < 4; r++) acc[q][r] += al * s[r];
           }
         }
       } else {  // y += al x
 #pragma unroll
-        for (int q = 0; q < PER; q++) {
-          const int e = tid + 256 * q;
-          if (e < n) acc[q] += al * xs[e];
+        for (int q = 0; q < PERT; q++) {
+          const int u = tid + 256 * q;
+          if (u < ntile) {
+            const Tile22 tl = tile22(u, dL, dR);
+            acc[q][0] += al * xs[tl.i0 + dL * tl.j0];
+            acc[q][1] += al * xs[tl.i1 + dL * tl.j0];
+            acc[q][2] += al * xs[tl.i0 + dL * tl.j1];
+            acc[q][3] += al * xs[tl.i1 + dL * tl.j1];
+          }
         }
       }
     }
   }
+  double* Y = b.direct ? y + b.out : part + b.out;
 #pragma unroll
-  for (int q = 0; q < PER; q++) {
-    const int e = tid + 256 * q;
-    if (e < n) y[b.yoff + e] += acc[q];
+  for (int q = 0; q < PERT; q++) {
+    const int u = tid + 256 * q;
+    if (u < ntile) {
+      const Tile22 tl = tile22(u, dL, dR);
+      const int e00 = tl.i0 + dL * tl.j0;
+      const bool r1 = tl.i0 + 1 < dL, c1 = tl.j0 + 1 < dR;
+      if (b.direct) {
+        Y[e00] += acc[q][0];
+        if (r1) Y[e00 + 1] += acc[q][1];
+        if (c1) Y[e00 + dL] += acc[q][2];
+        if (r1 && c1) Y[e00 + dL + 1] += acc[q][3];
+      } else {
+        Y[e00] = acc[q][0];
+        if (r1) Y[e00 + 1] = acc[q][1];
+        if (c1) Y[e00 + dL] = acc[q][2];
+        if (r1 && c1) Y[e00 + dL + 1] = acc[q][3];
+      }
+    }
   }
 }
-constexpr int CHAIN_PER_S = (CHAIN_SMALLP + 255) / 256;
-constexpr int CHAIN_PER_L = (CHAIN_MAXD * CHAIN_MAXD + 255) / 256;
+constexpr int CHAIN_PER_S = (18 * 18 + 255) / 256;                                  // 2 x 2 tiles of a 36 x 36 block
+constexpr int CHAIN_PER_L = ((CHAIN_MAXD / 2) * (CHAIN_MAXD / 2) + 255) / 256;     // of a 64 x 64 block
+
+// y[yoff + e] += sum_q part[p0 + q n + e] over the work items of a split output block
+__global__ __launch_bounds__(256) void k_chain_reduce(const DevBRed* __restrict__ reds, const double* __restrict__ part,
+                                                      double* __restrict__ y) {
+  const DevBRed r = reds[blockIdx.x];
+  for (int e = threadIdx.x; e < r.R; e += 256) {
+    double s = 0.0;
+    for (int q = 0; q < r.nparts; q++) s += part[r.p0 + (int64_t)q * r.R + e];
+    y[r.yoff + e] += s;
+  }
+}
 
 // --------------------------------------------------------- vector kernels --
 __global__ __launch_bounds__(256) void k_dot_partial(const double* __restrict__ a, const double* __restrict__ b,
@@ -663,14 +740,18 @@ void launch_diag(const DevDiagBlock* blocks, const DevDiag* terms, int nblocks, 
 }
 
 void launch_chain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups, const DevCTerm* terms, const double* x,
-                  double* y, int P, bool large, hipStream_t s) {
+                  double* y, double* part, int P, bool large, hipStream_t s) {
   if (nblocks <= 0) return;
   if (large)
     hipLaunchKernelGGL((k_chain<CHAIN_PER_L, false>), dim3(nblocks), dim3(256), (size_t)4 * P * sizeof(double), s, blocks,
-                       groups, terms, x, y, P);
+                       groups, terms, x, y, part, P);
   else
     hipLaunchKernelGGL((k_chain<CHAIN_PER_S, true>), dim3(nblocks), dim3(256), (size_t)6 * P * sizeof(double), s, blocks,
-                       groups, terms, x, y, P);
+                       groups, terms, x, y, part, P);
+}
+
+void launch_chain_reduce(const DevBRed* reds, int nreds, const double* part, double* y, hipStream_t s) {
+  if (nreds > 0) hipLaunchKernelGGL(k_chain_reduce, dim3(nreds), dim3(256), 0, s, reds, part, y);
 }
 
 bool chain_kernel_setup() {
