@@ -217,6 +217,24 @@ struct DevTile {  // one 64 x 64 output tile and the products that accumulate in
   int op0, nop;
   int flags;  // bit 0: C relative to y; bit 1: add to C instead of overwriting it
 };
+// Small-block products of the operator updates: one workgroup per work item computes
+// out (m x n, contiguous) = sum of its products (DevGemmOp, absolute pointers, kind 0:
+// alpha op(A) op(B); kind 1: alpha op(A)) in 2 x 2 register tiles with the operands
+// copied into LDS; an output block with many products is split over several items
+// whose partials DevPRed sums (overwriting the output).
+constexpr int OPS_MAXD = 64;    // m, n of the small path
+constexpr int OPS_P = 2048;     // operand elements (m k, k n) of the small path
+constexpr int OPS_ITEM = 64;    // products per work item
+struct DevOItem {
+  double* out;
+  int m, n;
+  int op0, nop;
+};
+struct DevPRed {  // out[e] = sum_q part[p0 + q R + e], q < nparts
+  double* out;
+  int64_t p0;
+  int R, nparts;
+};
 // Output-stationary chained products for output blocks of at most CHAIN_MAXD rows and
 // columns (and inputs of the same bound): one workgroup per block keeps y_kappa in
 // registers and runs every term of the block; terms sharing an input block kappa'
@@ -350,6 +368,8 @@ void launch_diag(const DevDiagBlock* blocks, const DevDiag* terms, int nblocks, 
 void launch_chain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups, const DevCTerm* terms, const double* x,
                   double* y, double* part, int P, bool large, hipStream_t s);
 void launch_chain_reduce(const DevBRed* reds, int nreds, const double* part, double* y, hipStream_t s);
+void launch_ops(const DevOItem* items, int nitems, const DevGemmOp* ops, int P, hipStream_t s);
+void launch_part_reduce(const DevPRed* reds, int nreds, const double* part, hipStream_t s);
 bool chain_kernel_setup();  // false: the large variant cannot get its LDS on this device
 void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs, double* M, hipStream_t s);
 void launch_bspmv(const DevBItem* items, int nitems, const DevBRed* reds, int nreds, double* P, const double* M,

@@ -1,6 +1,7 @@
 // Context, device work-list construction, the Davidson driver and the C ABI.
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
@@ -762,6 +763,13 @@ void Ctx::run_terms(const std::vector<UTerm>& terms, const std::vector<std::pair
 // per output block, the two-operand products, W op(C) and the transposed axpys. The
 // lists are built on the host threads; positions come from prefix sums, so they do
 // not depend on the thread count.
+// Executes update terms: every output block is overwritten by the sum of its terms.
+// Three-operand terms first form W = alpha op(A) op(B); then every output block sums its
+// two-operand products, W op(C) and the transposed axpys. Products whose blocks are
+// small run in k_ops (2 x 2 register tiles, output blocks with many products split
+// over several work items and summed by k_part_reduce), the rest in k_tiles. The lists
+// are built on the host threads; positions come from prefix sums, so they do not
+// depend on the thread count.
 void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<Ref, std::pair<int, int>>>& outs,
                     const double* combo_base) {
   const double t_begin = now_s();
@@ -787,8 +795,13 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
   int* seg_of = u_seg.data();
   int64_t* woff = u_woff.data();
   int64_t* order = u_order.data();
-  // output block of every term; phase-1 counts per segment
-  struct P1 { int64_t ops = 0, tiles = 0, w = 0; };
+  auto small_gemm = [](int m, int nn, int k, int ta, int lda, int tb, int ldb) {
+    return m <= OPS_MAXD && nn <= OPS_MAXD && (int64_t)m * k <= OPS_P && (int64_t)k * nn <= OPS_P &&
+           lda == (ta ? k : m) && ldb == (tb ? nn : k);
+  };
+  const bool ops_on = use_ops_kernel;
+  // output block of every term; phase-1 counts per segment (k_ops items or GEMM tiles)
+  struct P1 { int64_t ops = 0, tiles = 0, items = 0, w = 0; };
   std::vector<P1> p1(nseg + 1);
   parallel_for(nseg, nthr, [&](int64_t s, int) {
     P1 c;
@@ -800,51 +813,18 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
       seg_of[s0[s] + j] = (int)s;
       if (u.kind != 1) continue;
       c.ops++;
-      c.tiles += (int64_t)((u.m + 63) / 64) * ((u.k2 + 63) / 64);
+      if (ops_on && small_gemm(u.m, u.k2, u.k1, u.ta, u.lda, u.tb, u.ldb)) c.items++;
+      else c.tiles += (int64_t)((u.m + 63) / 64) * ((u.k2 + 63) / 64);
       c.w += ((int64_t)u.m * u.k2 + 7) / 8 * 8;
     }
     p1[s + 1] = c;
   });
-  for (int64_t s = 0; s < nseg; s++) { p1[s + 1].ops += p1[s].ops; p1[s + 1].tiles += p1[s].tiles; p1[s + 1].w += p1[s].w; }
-  // phase-2 tiles per output block
-  std::vector<int64_t> tstart(nq + 1, 0);
-  for (int64_t q = 0; q < nq; q++) {
-    const int m = outs[q].second.first, nn = outs[q].second.second;
-    tstart[q + 1] = tstart[q] + ((m > 0 && nn > 0) ? (int64_t)((m + 63) / 64) * ((nn + 63) / 64) : 0);
+  for (int64_t s = 0; s < nseg; s++) {
+    p1[s + 1].ops += p1[s].ops; p1[s + 1].tiles += p1[s].tiles; p1[s + 1].items += p1[s].items; p1[s + 1].w += p1[s].w;
   }
-  const int64_t n1 = p1[nseg].ops, nt1 = p1[nseg].tiles, nt2 = tstart[nq];
-  if (nt1 > INT32_MAX || nt2 > INT32_MAX) throw Error{ERR_SIZE, "too many update tiles"};
-  ustage.reset();
-  ustage.reserve(256 * 4 + (n1 + n) * sizeof(DevGemmOp) + (nt1 + nt2) * sizeof(DevTile));
-  DevGemmOp* o1 = ustage.take<DevGemmOp>(n1);
-  DevGemmOp* o2 = ustage.take<DevGemmOp>(n);
-  DevTile* t1 = ustage.take<DevTile>(nt1);
-  DevTile* t2 = ustage.take<DevTile>(nt2);
+  const int64_t n1 = p1[nseg].ops, nt1 = p1[nseg].tiles, ni1 = p1[nseg].items;
   u_W.alloc(std::max<int64_t>(p1[nseg].w, 1));
   double* Wp = u_W.p;
-  parallel_for(nseg, nthr, [&](int64_t s, int) {
-    int64_t io = p1[s].ops, it = p1[s].tiles, iw = p1[s].w;
-    const double* cb = combo_base ? combo_base + segs[s].combo_base : nullptr;
-    for (size_t j = 0; j < segs[s].terms.size(); j++) {
-      const UTerm& u = segs[s].terms[j];
-      if (u.kind != 1) continue;
-      DevGemmOp op{};
-      op.A = resolve_with(u.A, cb); op.ta = u.ta; op.lda = u.lda;
-      op.B = resolve_with(u.B, cb); op.tb = u.tb; op.ldb = u.ldb;
-      op.k = u.k1; op.alpha = u.alpha; op.kind = 0;
-      woff[s0[s] + j] = iw;
-      const int opi = (int)io;
-      o1[io++] = op;
-      for (int m0 = 0; m0 < u.m; m0 += 64)
-        for (int n0 = 0; n0 < u.k2; n0 += 64) {
-          DevTile tl{};
-          tl.C = Wp + iw;
-          tl.ldc = u.m; tl.m0 = m0; tl.n0 = n0; tl.m = u.m; tl.n = u.k2; tl.op0 = opi; tl.nop = 1; tl.flags = 0;
-          t1[it++] = tl;
-        }
-      iw += ((int64_t)u.m * u.k2 + 7) / 8 * 8;
-    }
-  });
   // terms grouped by output block, stable in (segment, term) order: a counting sort
   // with one histogram per range of terms
   std::vector<int64_t> qstart(nq + 1, 0);
@@ -857,7 +837,6 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
       const auto rg = range((int)r);
       for (int64_t g = rg.first; g < rg.second; g++) hist[r * nq + qof[g]]++;
     });
-    // qstart[q] = terms of blocks before q; hist[r][q] becomes the fill position of range r
     int64_t pos = 0;
     for (int64_t q = 0; q < nq; q++) {
       qstart[q] = pos;
@@ -873,7 +852,86 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
       for (int64_t g = rg.first; g < rg.second; g++) order[hist[r * nq + qof[g]]++] = g;
     });
   }
-  // phase 2: one op per term at its grouped position, then the tiles of every block
+  // phase 2 per output block: small path or tiles; counts, then positions
+  std::vector<char> qsmall(nq, 0);
+  std::vector<int64_t> t2start(nq + 1, 0), i2start(nq + 1, 0), p2start(nq + 1, 0);
+  parallel_for((nq + 1023) / 1024, nthr, [&](int64_t c, int) {
+    for (int64_t q = c * 1024; q < std::min<int64_t>(nq, (c + 1) * 1024); q++) {
+      const int m = outs[q].second.first, nn = outs[q].second.second;
+      if (m <= 0 || nn <= 0) continue;
+      bool sm = ops_on && m <= OPS_MAXD && nn <= OPS_MAXD;
+      for (int64_t pos = qstart[q]; sm && pos < qstart[q + 1]; pos++) {
+        const int64_t g = order[pos];
+        const UTerm& u = segs[seg_of[g]].terms[g - s0[seg_of[g]]];
+        if (u.kind == 1) sm = small_gemm(m, nn, u.k2, 0, m, u.tc, u.ldc);
+        else if (u.kind == 0) sm = small_gemm(m, nn, u.k1, u.ta, u.lda, u.tb, u.ldb);
+      }
+      qsmall[q] = sm;
+      const int64_t nops = qstart[q + 1] - qstart[q];
+      if (sm) {
+        const int64_t ni = std::max<int64_t>(1, (nops + OPS_ITEM - 1) / OPS_ITEM);
+        i2start[q + 1] = ni;
+        p2start[q + 1] = ni > 1 ? ni * (int64_t)m * nn : 0;
+      } else {
+        t2start[q + 1] = (int64_t)((m + 63) / 64) * ((nn + 63) / 64);
+      }
+    }
+  });
+  int64_t nred = 0;
+  for (int64_t q = 0; q < nq; q++) {
+    if (qsmall[q] && i2start[q + 1] > 1) nred++;
+    t2start[q + 1] += t2start[q]; i2start[q + 1] += i2start[q]; p2start[q + 1] += p2start[q];
+  }
+  const int64_t nt2 = t2start[nq], ni2 = i2start[nq], npart = p2start[nq];
+  if (nt1 > INT32_MAX || nt2 > INT32_MAX || ni1 + ni2 > INT32_MAX) throw Error{ERR_SIZE, "too many update tiles"};
+  ustage.reset();
+  ustage.reserve(256 * 8 + (n1 + n) * sizeof(DevGemmOp) + (nt1 + nt2) * sizeof(DevTile) + (ni1 + ni2) * sizeof(DevOItem) +
+                 nred * sizeof(DevPRed));
+  DevGemmOp* o1 = ustage.take<DevGemmOp>(n1);
+  DevGemmOp* o2 = ustage.take<DevGemmOp>(n);
+  DevTile* t1 = ustage.take<DevTile>(nt1);
+  DevTile* t2 = ustage.take<DevTile>(nt2);
+  DevOItem* it1 = ustage.take<DevOItem>(ni1);
+  DevOItem* it2 = ustage.take<DevOItem>(ni2);
+  DevPRed* rd = ustage.take<DevPRed>(nred);
+  u_part.alloc(std::max<int64_t>(npart, 1));
+  std::atomic<int> Pneed{1};
+  auto need = [&](int v) {
+    int cur = Pneed.load();
+    while (v > cur && !Pneed.compare_exchange_weak(cur, v)) {}
+  };
+  // phase 1: W = alpha op(A) op(B) for three-operand terms
+  parallel_for(nseg, nthr, [&](int64_t s, int) {
+    int64_t io = p1[s].ops, it = p1[s].tiles, ii = p1[s].items, iw = p1[s].w;
+    const double* cb = combo_base ? combo_base + segs[s].combo_base : nullptr;
+    int pl = 1;
+    for (size_t j = 0; j < segs[s].terms.size(); j++) {
+      const UTerm& u = segs[s].terms[j];
+      if (u.kind != 1) continue;
+      DevGemmOp op{};
+      op.A = resolve_with(u.A, cb); op.ta = u.ta; op.lda = u.lda;
+      op.B = resolve_with(u.B, cb); op.tb = u.tb; op.ldb = u.ldb;
+      op.k = u.k1; op.alpha = u.alpha; op.kind = 0;
+      woff[s0[s] + j] = iw;
+      const int opi = (int)io;
+      o1[io++] = op;
+      if (ops_on && small_gemm(u.m, u.k2, u.k1, u.ta, u.lda, u.tb, u.ldb)) {
+        it1[ii++] = DevOItem{Wp + iw, u.m, u.k2, opi, 1};
+        pl = std::max({pl, u.m * u.k1, u.k1 * u.k2});
+      } else {
+        for (int m0 = 0; m0 < u.m; m0 += 64)
+          for (int n0 = 0; n0 < u.k2; n0 += 64) {
+            DevTile tl{};
+            tl.C = Wp + iw;
+            tl.ldc = u.m; tl.m0 = m0; tl.n0 = n0; tl.m = u.m; tl.n = u.k2; tl.op0 = opi; tl.nop = 1; tl.flags = 0;
+            t1[it++] = tl;
+          }
+      }
+      iw += ((int64_t)u.m * u.k2 + 7) / 8 * 8;
+    }
+    need(pl);
+  });
+  // phase 2: one op per term at its grouped position
   parallel_for((n + 65535) / 65536, nthr, [&](int64_t c, int) {
     for (int64_t pos = c * 65536; pos < std::min<int64_t>(n, (c + 1) * 65536); pos++) {
       const int64_t g = order[pos];
@@ -896,38 +954,70 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
       o2[pos] = op;
     }
   });
+  // phase 2 records per output block
+  std::vector<int64_t> rstart(nq + 1, 0);
+  for (int64_t q = 0; q < nq; q++) rstart[q + 1] = rstart[q] + ((qsmall[q] && i2start[q + 1] - i2start[q] > 1) ? 1 : 0);
   parallel_for((nq + 1023) / 1024, nthr, [&](int64_t c, int) {
+    int pl = 1;
     for (int64_t q = c * 1024; q < std::min<int64_t>(nq, (c + 1) * 1024); q++) {
       const int m = outs[q].second.first, nn = outs[q].second.second;
       if (m <= 0 || nn <= 0) continue;
       double* C = const_cast<double*>(resolve_with(outs[q].first, combo_base));
-      int64_t it = tstart[q];
-      for (int m0 = 0; m0 < m; m0 += 64)
-        for (int n0 = 0; n0 < nn; n0 += 64) {
-          DevTile tl{};
-          tl.C = C; tl.ldc = m; tl.m0 = m0; tl.n0 = n0; tl.m = m; tl.n = nn;
-          tl.op0 = (int)qstart[q]; tl.nop = (int)(qstart[q + 1] - qstart[q]); tl.flags = 0;
-          t2[it++] = tl;
+      if (qsmall[q]) {
+        const int64_t ni = i2start[q + 1] - i2start[q];
+        const int64_t nops = qstart[q + 1] - qstart[q];
+        for (int64_t i = 0; i < ni; i++) {
+          DevOItem itm{};
+          itm.m = m; itm.n = nn;
+          itm.op0 = (int)(qstart[q] + i * OPS_ITEM);
+          itm.nop = (int)std::min<int64_t>(OPS_ITEM, nops - i * OPS_ITEM);
+          itm.out = ni > 1 ? u_part.p + p2start[q] + i * (int64_t)m * nn : C;
+          it2[i2start[q] + i] = itm;
         }
+        if (ni > 1) rd[rstart[q]] = DevPRed{C, p2start[q], m * nn, (int)ni};
+        for (int64_t pos = qstart[q]; pos < qstart[q + 1]; pos++)
+          if (o2[pos].kind == 0) pl = std::max({pl, m * o2[pos].k, o2[pos].k * nn});
+      } else {
+        int64_t it = t2start[q];
+        for (int m0 = 0; m0 < m; m0 += 64)
+          for (int n0 = 0; n0 < nn; n0 += 64) {
+            DevTile tl{};
+            tl.C = C; tl.ldc = m; tl.m0 = m0; tl.n0 = n0; tl.m = m; tl.n = nn;
+            tl.op0 = (int)qstart[q]; tl.nop = (int)(qstart[q + 1] - qstart[q]); tl.flags = 0;
+            t2[it++] = tl;
+          }
+      }
     }
+    need(pl);
   });
   const double t_lists = now_s();
   u_d1.alloc(std::max<int64_t>(n1, 1));
   u_d2.alloc(std::max<int64_t>(n, 1));
   u_dt1.alloc(std::max<int64_t>(nt1, 1));
   u_dt2.alloc(std::max<int64_t>(nt2, 1));
+  u_it1.alloc(std::max<int64_t>(ni1, 1));
+  u_it2.alloc(std::max<int64_t>(ni2, 1));
+  u_rd.alloc(std::max<int64_t>(nred, 1));
   if (n1) hip_check(hipMemcpyAsync(u_d1.p, o1, n1 * sizeof(DevGemmOp), hipMemcpyHostToDevice, stream), "ops");
   if (n) hip_check(hipMemcpyAsync(u_d2.p, o2, n * sizeof(DevGemmOp), hipMemcpyHostToDevice, stream), "ops");
   if (nt1) hip_check(hipMemcpyAsync(u_dt1.p, t1, nt1 * sizeof(DevTile), hipMemcpyHostToDevice, stream), "tiles");
   if (nt2) hip_check(hipMemcpyAsync(u_dt2.p, t2, nt2 * sizeof(DevTile), hipMemcpyHostToDevice, stream), "tiles");
+  if (ni1) hip_check(hipMemcpyAsync(u_it1.p, it1, ni1 * sizeof(DevOItem), hipMemcpyHostToDevice, stream), "items");
+  if (ni2) hip_check(hipMemcpyAsync(u_it2.p, it2, ni2 * sizeof(DevOItem), hipMemcpyHostToDevice, stream), "items");
+  if (nred) hip_check(hipMemcpyAsync(u_rd.p, rd, nred * sizeof(DevPRed), hipMemcpyHostToDevice, stream), "reductions");
   const double t_up = now_s();
+  const int P = std::min(OPS_P, (Pneed.load() + 31) / 32 * 32);
   launch_tiles(u_dt1.p, u_d1.p, (int)nt1, nullptr, nullptr, stream);
+  launch_ops(u_it1.p, (int)ni1, u_d1.p, P, stream);
   launch_tiles(u_dt2.p, u_d2.p, (int)nt2, nullptr, nullptr, stream);
+  launch_ops(u_it2.p, (int)ni2, u_d2.p, P, stream);
+  launch_part_reduce(u_rd.p, (int)nred, u_part.p, stream);
   hip_check(hipGetLastError(), "update kernels");
   hip_check(hipStreamSynchronize(stream), "update");
   if (verbose)
-    std::fprintf(stderr, "    terms %lld: lists %.3f s, upload %.3f s, kernels %.3f s (%lld + %lld tiles)\n", (long long)n,
-                 t_lists - t_begin, t_up - t_lists, now_s() - t_up, (long long)nt1, (long long)nt2);
+    std::fprintf(stderr, "    terms %lld: lists %.3f s, upload %.3f s, kernels %.3f s (%lld + %lld tiles, %lld + %lld small items)\n",
+                 (long long)n, t_lists - t_begin, t_up - t_lists, now_s() - t_up, (long long)nt1, (long long)nt2,
+                 (long long)ni1, (long long)ni2);
 }
 
 // DMRG::updateMovingRight/Left (DMRGoperators.cpp:243-907) for slot `index`.
@@ -1284,6 +1374,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_VERBOSE")) ctx->c.verbose = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_SVD_JACOBI_MAXN")) ctx->c.svd_jacobi_maxn = std::atoi(e);
   if (const char* e = std::getenv("CHM_CHAIN_KERNEL")) ctx->c.use_chain_kernel = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CHM_OPS_KERNEL")) ctx->c.use_ops_kernel = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return CHM_ERR_HIP;

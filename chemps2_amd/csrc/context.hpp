@@ -171,6 +171,10 @@ struct Ctx : HostState {
   DBuf<DevGemmOp> u_d1, u_d2;
   DBuf<DevTile> u_dt1, u_dt2;
   DBuf<double> u_W;
+  DBuf<DevOItem> u_it1, u_it2;
+  DBuf<DevPRed> u_rd;
+  DBuf<double> u_part;
+  bool use_ops_kernel = true;  // CHM_OPS_KERNEL=0: every update product through k_tiles
   double prep_s[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // seconds of the last prepare, by phase
   bool verbose = false;                         // CHM_VERBOSE: phase times on stderr
   int64_t item_elems = 32768;        // CHM_ITEM_ELEMS: matrix elements per work item

@@ -421,6 +421,76 @@ __global__ __launch_bounds__(256) void k_chain(const DevCBlock* __restrict__ blo
 constexpr int CHAIN_PER_S = (18 * 18 + 255) / 256;                                  // 2 x 2 tiles of a 36 x 36 block
 constexpr int CHAIN_PER_L = ((CHAIN_MAXD / 2) * (CHAIN_MAXD / 2) + 255) / 256;     // of a 64 x 64 block
 
+// Small-block update products (see DevOItem). Dynamic LDS: 2 arrays of P doubles.
+template <int PERT>
+__global__ __launch_bounds__(256) void k_ops(const DevOItem* __restrict__ items, const DevGemmOp* __restrict__ ops, int P) {
+  extern __shared__ double lds[];
+  double* As = lds;
+  double* Bs = lds + P;
+  const DevOItem it = items[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int m = it.m, n = it.n;
+  const int ntile = ((m + 1) >> 1) * ((n + 1) >> 1);
+  double acc[PERT][4];
+#pragma unroll
+  for (int q = 0; q < PERT; q++) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.0;
+  for (int o = it.op0; o < it.op0 + it.nop; o++) {
+    const DevGemmOp op = ops[o];
+    const double al = op.alpha;
+    const int ai = op.ta ? op.lda : 1, ak = op.ta ? 1 : op.lda;
+    if (op.kind == 1) {  // acc += al op(A), op(A) m x n read in place
+#pragma unroll
+      for (int q = 0; q < PERT; q++) {
+        const int u = tid + 256 * q;
+        if (u < ntile) {
+          const Tile22 tl = tile22(u, m, n);
+          acc[q][0] += al * op.A[tl.i0 * ai + tl.j0 * ak];
+          acc[q][1] += al * op.A[tl.i1 * ai + tl.j0 * ak];
+          acc[q][2] += al * op.A[tl.i0 * ai + tl.j1 * ak];
+          acc[q][3] += al * op.A[tl.i1 * ai + tl.j1 * ak];
+        }
+      }
+      continue;
+    }
+    const int k = op.k;
+    const int bk = op.tb ? op.ldb : 1, bj = op.tb ? 1 : op.ldb;
+    __syncthreads();  // every reader of the previous copies is done
+    for (int e = tid; e < m * k; e += 256) As[e] = op.A[e];
+    for (int e = tid; e < k * n; e += 256) Bs[e] = op.B[e];
+    __syncthreads();
+    double s[4];
+#pragma unroll
+    for (int q = 0; q < PERT; q++) {
+      const int u = tid + 256 * q;
+      if (u < ntile) {
+        const Tile22 tl = tile22(u, m, n);
+        mm22(As, ai, ak, Bs, bk, bj, k, tl, s);
+        for (int r = 0; r < 4; r++) acc[q][r] += al * s[r];
+      }
+    }
+  }
+  double s4[4];
+#pragma unroll
+  for (int q = 0; q < PERT; q++) {
+    const int u = tid + 256 * q;
+    if (u < ntile) {
+      const Tile22 tl = tile22(u, m, n);
+      for (int r = 0; r < 4; r++) s4[r] = acc[q][r];
+      store22(it.out, m, m, n, tl, s4);
+    }
+  }
+}
+constexpr int OPS_PER = ((OPS_MAXD / 2) * (OPS_MAXD / 2) + 255) / 256;
+
+__global__ __launch_bounds__(256) void k_part_reduce(const DevPRed* __restrict__ reds, const double* __restrict__ part) {
+  const DevPRed r = reds[blockIdx.x];
+  for (int e = threadIdx.x; e < r.R; e += 256) {
+    double s = 0.0;
+    for (int q = 0; q < r.nparts; q++) s += part[r.p0 + (int64_t)q * r.R + e];
+    r.out[e] = s;
+  }
+}
+
 // y[yoff + e] += sum_q part[p0 + q n + e] over the work items of a split output block
 __global__ __launch_bounds__(256) void k_chain_reduce(const DevBRed* __restrict__ reds, const double* __restrict__ part,
                                                       double* __restrict__ y) {
@@ -748,6 +818,15 @@ void launch_chain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups,
   else
     hipLaunchKernelGGL((k_chain<CHAIN_PER_S, true>), dim3(nblocks), dim3(256), (size_t)6 * P * sizeof(double), s, blocks,
                        groups, terms, x, y, part, P);
+}
+
+void launch_ops(const DevOItem* items, int nitems, const DevGemmOp* ops, int P, hipStream_t s) {
+  if (nitems > 0)
+    hipLaunchKernelGGL(k_ops<OPS_PER>, dim3(nitems), dim3(256), (size_t)2 * P * sizeof(double), s, items, ops, P);
+}
+
+void launch_part_reduce(const DevPRed* reds, int nreds, const double* part, hipStream_t s) {
+  if (nreds > 0) hipLaunchKernelGGL(k_part_reduce, dim3(nreds), dim3(256), 0, s, reds, part);
 }
 
 void launch_chain_reduce(const DevBRed* reds, int nreds, const double* part, double* y, hipStream_t s) {

@@ -43,3 +43,14 @@ def test_update_operators(ctx, oracle_lib, index, mr):
             assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (kind, k, m)
         n += 1
     assert n > 50
+
+
+@pytest.mark.parametrize("index,mr", [(3, True), (3, False)])
+def test_update_operators_tile_path(oracle_lib, monkeypatch, index, mr):
+    # every update product through the GEMM tiles (CHM_OPS_KERNEL=0) agrees as well
+    monkeypatch.setenv("CHM_OPS_KERNEL", "0")
+    c = chemps2_amd.Context(0)
+    try:
+        test_update_operators(c, oracle_lib, index, mr)
+    finally:
+        c.close()
