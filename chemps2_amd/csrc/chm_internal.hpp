@@ -174,10 +174,24 @@ struct SLayout {
 // y_out += alpha * op(Lb) * x_in * op(Rb); an absent Lb or Rb is the identity. With
 // both present the reference's association order is kept: (op(L) x) op(R), or
 // op(L) (x op(R)) when xr_first is set.
-struct Term {
+// Ref with a 32-bit offset, the form stored in the (tens of millions of) Heff terms.
+struct TRef {
+  int32_t buf = -1;
+  int32_t off = 0;
+  TRef() = default;
+  TRef(const Ref& r) : buf(r.buf), off((int32_t)r.off) {}
+  operator Ref() const {
+    Ref r;
+    r.buf = buf;
+    r.off = off;
+    return r;
+  }
+  bool ok() const { return buf >= 0; }
+};
+struct Term {  // 40 bytes
   int out, in;
   double alpha;
-  Ref Lb, Rb;
+  TRef Lb, Rb;
   char tl, tr;
   bool xr_first;
 };

@@ -42,6 +42,7 @@ void HeffPlanner::term(int ik, int kin, double alpha, char tl, Ref Lb, char tr, 
   else if (tr != 'I') flops += 2.0 * dL * dR * dRi;
   if ((tl != 'I' && !Lb.ok()) || (tr != 'I' && !Rb.ok())) return;  // identically zero combination
   if (alpha == 0.0) return;
+  if (Lb.off > INT32_MAX || Rb.off > INT32_MAX) throw Error{ERR_SIZE, "operator block offset beyond 32 bits"};
   Term t;
   t.out = ik; t.in = kin; t.alpha = alpha; t.Lb = Lb; t.Rb = Rb; t.tl = tl; t.tr = tr; t.xr_first = xr_first;
   terms.push_back(t);

@@ -24,7 +24,31 @@ static double tri_coef(int a, int b, int c) {  // Delta(abc) with doubled argume
 
 // Racah's single-sum formula for {a b c; d e f} (doubled arguments), the quantity
 // Wigner::wigner6j evaluates (Wigner.cpp:294-339).
+static double wigner6j_eval(int a, int b, int c, int d, int e, int f);
+
+// Direct-mapped per-thread cache in front of the evaluation: the work-list generators
+// ask for the same few thousand symbols tens of millions of times.
 double wigner6j(int a, int b, int c, int d, int e, int f) {
+  if (a < 0 || b < 0 || c < 0 || d < 0 || e < 0 || f < 0) return 0.0;
+  if ((a | b | c | d | e | f) >= 512) return wigner6j_eval(a, b, c, d, e, f);
+  struct Cache {
+    uint64_t key[1 << 14];
+    double val[1 << 14];
+    Cache() { for (auto& k : key) k = ~0ULL; }
+  };
+  thread_local Cache cache;
+  const uint64_t key = (uint64_t)a | ((uint64_t)b << 9) | ((uint64_t)c << 18) | ((uint64_t)d << 27) | ((uint64_t)e << 36) |
+                       ((uint64_t)f << 45);
+  uint64_t h = key * 0x9E3779B97F4A7C15ULL;
+  const size_t slot = (size_t)(h >> 50);
+  if (cache.key[slot] == key) return cache.val[slot];
+  const double v = wigner6j_eval(a, b, c, d, e, f);
+  cache.key[slot] = key;
+  cache.val[slot] = v;
+  return v;
+}
+
+static double wigner6j_eval(int a, int b, int c, int d, int e, int f) {
   fact_init();
   if (a < 0 || b < 0 || c < 0 || d < 0 || e < 0 || f < 0) return 0.0;
   if (tri_bad(a, b, c) || tri_bad(d, e, c) || tri_bad(a, e, f) || tri_bad(d, b, f)) return 0.0;
