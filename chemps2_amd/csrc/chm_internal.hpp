@@ -55,6 +55,7 @@ struct Sectors {
   struct Sec { int N, S, I, d; };
   std::vector<std::vector<Sec>> list;                   // [b], sorted by (N, 2S, I), d > 0
   std::vector<std::shared_ptr<const SecIndex>> idx;     // [b]
+  uint64_t gen = 0;                                      // incremented by every change
   void resize(int L) {
     list.assign(L + 1, {});
     idx.assign(L + 1, std::make_shared<const SecIndex>());
@@ -149,7 +150,8 @@ struct TLay {
 // All operators of one boundary, indexed like DMRG::allocateTensors
 // (DMRGoperators.cpp:909-1145): slot cnt holds boundary cnt + 1.
 struct OpSet {
-  int dir = 0;  // 1 built moving right, 2 built moving left
+  int dir = 0;       // 1 built moving right, 2 built moving left
+  uint64_t gen = 0;  // Sectors::gen when it was laid out
   std::vector<OpT*> L, Q;
   std::vector<std::vector<OpT*>> F0, F1, S0, S1, A, B, C, D;
   OpT* X = nullptr;

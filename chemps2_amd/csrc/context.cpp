@@ -17,6 +17,7 @@ void launch_fill_hash(double* p, int64_t n, uint64_t seed, double scale, hipStre
 void destroy_rocblas(void* h);
 
 Ctx::~Ctx() {
+  drop_plan();
   reap_join();
   destroy_rocblas(rocblas);
   if (s_buf >= 0) bufs[s_buf] = nullptr;  // owned by Sdev
@@ -56,6 +57,8 @@ void Ctx::set_problem(int L_, int group_, const int* irr, int N_, int TwoS_, int
   mps.assign(L, TLay());
   mps_buf.assign(L, nullptr);
   site = -1;
+  drop_plan();
+  prealloc.clear();
   reap_join();
   plan.reset();
   have_problem = true;
@@ -159,6 +162,51 @@ static double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+void Ctx::start_plan(int st) {
+  drop_plan();
+  if (st < 0 || st > L - 2) return;
+  OpSet* Lo = st > 0 ? &sets[st - 1] : nullptr;
+  OpSet* Ro = st < L - 2 ? &sets[st + 1] : nullptr;
+  job = std::make_unique<PlanJob>();
+  PlanJob* j = job.get();
+  j->site = st;
+  j->gen = sec.gen;
+  j->S.build(sec, st, irreps[st], irreps[st + 1]);
+  j->plan = std::make_unique<HeffPlanner>();
+  j->plan->init(&sec, &irreps, nirrep, mx.data(), L, &j->S, Lo, Ro);
+  j->plan->pool = std::move(term_pool);
+  j->th = std::thread([j]() {
+    try {
+      j->plan->build();
+    } catch (...) {
+      j->err = std::current_exception();
+    }
+  });
+}
+
+void Ctx::drop_plan() {
+  if (!job) return;
+  if (job->th.joinable()) job->th.join();
+  job.reset();
+}
+
+// The layout part of update_operators: the new slot's tensors with the flags of
+// DMRG::allocateTensors, before their values are computed.
+void Ctx::alloc_slot(int index, bool mr) {
+  const int prepared = site;  // a prepared Heff reads slots site - 1 and site + 1 only
+  clear_set(index);
+  for (const OpShape& sh : opset_shapes(*this, index, mr)) {
+    OpT* t = add_tensor(index, mr, sh.kind, sh.k, sh.m, sh.two_j, sh.n_elec, sh.n_irrep, -1);
+    t->moving_right = mr;
+    t->jw_phase = (sh.kind == KIND_L || sh.kind == KIND_Q);
+    t->prime_last = (sh.kind == KIND_F1 || sh.kind == KIND_D) ? mr : true;
+  }
+  sets[index].gen = sec.gen;
+  if ((int)prealloc.size() < L - 1) prealloc.assign(L - 1, 0);
+  prealloc[index] = 1;
+  if (prepared >= 0 && index != prepared - 1 && index != prepared + 1) site = prepared;
+}
+
 void Ctx::prepare(int st) {
   const double t_start = now_s();
   if (!have_problem) throw Error{ERR_STATE, "set the problem first"};
@@ -168,13 +216,28 @@ void Ctx::prepare(int st) {
   OpSet* Ro = nullptr;
   if (st > 0) { validate_set(st - 1, true); Lo = &sets[st - 1]; }
   if (st < L - 2) { validate_set(st + 1, false); Ro = &sets[st + 1]; }
-  S.build(sec, st, irreps[st], irreps[st + 1]);
+  bool planned = false;
+  if (job && job->site == st) {
+    job->th.join();
+    if (!job->err && job->gen == sec.gen && job->plan->Lo == Lo && job->plan->Ro == Ro) {
+      S = std::move(job->S);
+      plan = std::move(job->plan);
+      plan->S = &S;
+      planned = true;
+    }
+    job.reset();
+  } else {
+    drop_plan();
+  }
+  if (!planned) {
+    S.build(sec, st, irreps[st], irreps[st + 1]);
+    plan = std::make_unique<HeffPlanner>();
+    plan->init(&sec, &irreps, nirrep, mx.data(), L, &S, Lo, Ro);
+    plan->pool = std::move(term_pool);
+    plan->build();
+  }
   V = S.size();
   if (V <= 0) throw Error{ERR_STATE, "empty two-site object"};
-  plan = std::make_unique<HeffPlanner>();
-  plan->init(&sec, &irreps, nirrep, mx.data(), L, &S, Lo, Ro);
-  plan->pool = std::move(term_pool);
-  plan->build();
   F_H = plan->flops;
   const double t_built = now_s();
 
@@ -1034,14 +1097,12 @@ void Ctx::update_operators(int index, bool mr) {
   OpSet* prev = nullptr;
   if (mr && index > 0) { validate_set(index - 1, true); prev = &sets[index - 1]; }
   if (!mr && index < L - 2) { validate_set(index + 1, false); prev = &sets[index + 1]; }
-  // lay out the new slot with the flags of DMRG::allocateTensors
-  clear_set(index);
-  for (const OpShape& sh : opset_shapes(*this, index, mr)) {
-    OpT* t = add_tensor(index, mr, sh.kind, sh.k, sh.m, sh.two_j, sh.n_elec, sh.n_irrep, -1);
-    t->moving_right = mr;
-    t->jw_phase = (sh.kind == KIND_L || sh.kind == KIND_Q);
-    t->prime_last = (sh.kind == KIND_F1 || sh.kind == KIND_D) ? mr : true;
-  }
+  // the new slot laid out with the flags of DMRG::allocateTensors (already done by
+  // alloc_slot when the next site was planned ahead with these dimensions)
+  const bool laid_out = index < (int)prealloc.size() && prealloc[index] && sets[index].dir == (mr ? 1 : 2) &&
+                        sets[index].gen == sec.gen;
+  if (!laid_out) alloc_slot(index, mr);
+  prealloc[index] = 0;
   OpSet& o = sets[index];
   std::vector<int> Qsite;
   const int nC = mr ? L - 1 - index : index + 1;

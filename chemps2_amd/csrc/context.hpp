@@ -115,6 +115,22 @@ struct Ctx : HostState {
   int site = -1;
   SLayout S;
   std::unique_ptr<HeffPlanner> plan;
+  // Work lists of the next site generated on a background thread while the GPU runs the
+  // current one (used while the sector dimensions stay fixed); valid only if the sectors
+  // are unchanged when prepare() takes them.
+  struct PlanJob {
+    int site = -1;
+    uint64_t gen = 0;
+    SLayout S;
+    std::unique_ptr<HeffPlanner> plan;
+    std::thread th;
+    std::exception_ptr err;
+  };
+  std::unique_ptr<PlanJob> job;
+  void start_plan(int site);
+  void drop_plan();
+  std::vector<char> prealloc;     // [slot]: laid out by alloc_slot for the coming update
+  void alloc_slot(int index, bool moving_right);
   // Work lists are released on a background thread while the GPU runs the step (tens
   // of millions of records); the previous release is joined first.
   std::thread reaper;
@@ -217,7 +233,7 @@ struct Ctx : HostState {
   double split(int site, int D, bool moving_right, bool change);
   void mps_random(uint64_t seed);
   void presolve();
-  double solve_site(int index, double rtol, double noise, int D, bool moving_right, bool change);
+  double solve_site(int index, double rtol, double noise, int D, bool moving_right, bool change, int plan_next = -1);
   double dmrg_solve(int n, const int* D, const double* econv, const int* max_sweeps, const double* noise, const double* rtol);
 
   ~Ctx();

@@ -381,12 +381,18 @@ void Ctx::presolve() {  // DMRG.cpp:255-265
   max_disc_last = 0.0;
 }
 
-double Ctx::solve_site(int index, double rtol, double noise, int D, bool mr, bool change) {  // DMRG.cpp:419-446
+double Ctx::solve_site(int index, double rtol, double noise, int D, bool mr, bool change, int plan_next) {  // DMRG.cpp:419-446
   double t0 = now();
   join(index);
   double t1 = now();
   tm[0] += t1 - t0;
   prepare(index);
+  if (plan_next >= 0) {
+    // dimensions stay fixed in this sweep: lay out this step's new operator slot and
+    // generate the next site's work lists while the GPU solves this one
+    alloc_slot(index, mr);
+    start_plan(plan_next);
+  }
   double t2 = now();
   tm[1] += t2 - t1;
   int nm = 0;
@@ -418,7 +424,9 @@ double Ctx::sweep(bool mr, int D, double rtol, double noise_prefactor, bool chan
   last_min = 1e8;
   double E = 0.0;
   auto step = [&](int index) {
-    E = solve_site(index, rtol, nl, D, mr, change);
+    int next = -1;
+    if (!change) next = mr ? (index + 1 <= L - 3 ? index + 1 : -1) : (index - 1 >= 1 ? index - 1 : -1);
+    E = solve_site(index, rtol, nl, D, mr, change, next);
     total_min = std::min(total_min, E);
     last_min = std::min(last_min, E);
     const double t = now();
@@ -434,8 +442,10 @@ double Ctx::sweep(bool mr, int D, double rtol, double noise_prefactor, bool chan
   } catch (...) {
     mv_timing = false;
     mv_used = 0;
+    drop_plan();
     throw;
   }
+  drop_plan();
   mv_timing = false;
   tm[5] += now() - t_start;
   return E;

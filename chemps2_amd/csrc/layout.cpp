@@ -93,6 +93,7 @@ void Sectors::set(int b, std::vector<Sec> v) {
   for (size_t p = 0; p < keep.size(); p++) ix->pos[((size_t)keep[p].N * (ix->smax + 1) + keep[p].S) * 8 + keep[p].I] = (int32_t)p;
   list[b] = std::move(keep);
   idx[b] = std::move(ix);
+  gen++;
 }
 
 // ---------------------------------------------------------------- OpT ------
