@@ -387,7 +387,7 @@ double Ctx::solve_site(int index, double rtol, double noise, int D, bool mr, boo
   double t1 = now();
   tm[0] += t1 - t0;
   prepare(index);
-  if (plan_next >= 0) {
+  if (plan_next >= 0 && !change) {
     // dimensions stay fixed in this sweep: lay out this step's new operator slot and
     // generate the next site's work lists while the GPU solves this one
     alloc_slot(index, mr);
@@ -410,6 +410,12 @@ double Ctx::solve_site(int index, double rtol, double noise, int D, bool mr, boo
     hip_check(hipStreamSynchronize(stream), "noise");
   }
   const double disc = split(index, D, mr, change);
+  if (plan_next >= 0 && change) {
+    // the new dimensions are known once Split is done: plan the next site while the
+    // operator update of this one runs
+    alloc_slot(index, mr);
+    start_plan(plan_next);
+  }
   tm[3] += now() - t3;
   if (disc > max_disc_last) max_disc_last = disc;
   return E;
@@ -424,8 +430,7 @@ double Ctx::sweep(bool mr, int D, double rtol, double noise_prefactor, bool chan
   last_min = 1e8;
   double E = 0.0;
   auto step = [&](int index) {
-    int next = -1;
-    if (!change) next = mr ? (index + 1 <= L - 3 ? index + 1 : -1) : (index - 1 >= 1 ? index - 1 : -1);
+    const int next = mr ? (index + 1 <= L - 3 ? index + 1 : -1) : (index - 1 >= 1 ? index - 1 : -1);
     E = solve_site(index, rtol, nl, D, mr, change, next);
     total_min = std::min(total_min, E);
     last_min = std::min(last_min, E);
