@@ -21,7 +21,7 @@ EXPORTS = [
     "chm_tensor_clear", "chm_heff_prepare", "chm_heff_veclen", "chm_heff_apply", "chm_heff_diag",
     "chm_davidson", "chm_flops", "chm_plan_stats", "chm_stream", "chm_timing_enable", "chm_timing_read",
     "chm_mps_upload", "chm_mps_download", "chm_tensor_download", "chm_update_operators",
-    "chm_join", "chm_s_get", "chm_s_set", "chm_split", "chm_dmrg_init", "chm_dmrg_solve", "chm_dmrg_sweep", "chm_dmrg_heff_stats", "chm_dmrg_stats",
+    "chm_join", "chm_s_get", "chm_s_set", "chm_split", "chm_get_problem", "chm_mps_layout_size", "chm_dmrg_init", "chm_dmrg_presolve", "chm_dmrg_solve", "chm_dmrg_sweep", "chm_dmrg_heff_stats", "chm_dmrg_stats",
 ]
 
 _lib = None
@@ -64,6 +64,9 @@ def load():
         "chm_dmrg_init": (i, [vp, i, C.c_uint64]),
         "chm_dmrg_solve": (i, [vp, i, _ip, _dp, _ip, _dp, _dp, C.POINTER(d)]),
         "chm_dmrg_sweep": (i, [vp, i, i, d, d, i, C.POINTER(d)]),
+        "chm_get_problem": (i, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
+        "chm_dmrg_presolve": (i, [vp]),
+        "chm_mps_layout_size": (i, [vp, i, C.POINTER(i64)]),
         "chm_dmrg_stats": (i, [vp, C.POINTER(d), C.POINTER(i64)]),
         "chm_dmrg_heff_stats": (i, [vp, C.POINTER(d), C.POINTER(d)]),
         "chm_heff_prepare": (i, [vp, i]),
@@ -83,6 +86,71 @@ def load():
         f.argtypes = args
     _lib = l
     return l
+
+
+IO_LIB_PATH = os.path.join(HERE, "libchemps2_amd_io.so")
+IO_EXPORTS = ["chm_mps_save", "chm_mps_load_dims", "chm_mps_load", "chm_io_write", "chm_io_read_dims", "chm_io_read_site"]
+_io = None
+
+
+def load_io():
+    """Load libchemps2_amd_io.so (MPS checkpoints in the reference's HDF5 layout)."""
+    global _io
+    if _io is not None:
+        return _io
+    if not os.path.exists(IO_LIB_PATH):
+        raise ChmError(f"{IO_LIB_PATH} is missing: build it with chemps2_amd/build.py (needs HDF5)")
+    load()  # the checkpoint library resolves the core library's symbols
+    l = C.CDLL(IO_LIB_PATH)
+    vp, i, i64 = C.c_void_p, C.c_int, C.c_int64
+    sig = {
+        "chm_mps_save": (i, [vp, C.c_char_p, i]),
+        "chm_mps_load_dims": (i, [vp, C.c_char_p]),
+        "chm_mps_load": (i, [vp, C.c_char_p, C.POINTER(i)]),
+        "chm_io_write": (i, [C.c_char_p, i, i, i, i, i, _ip, C.POINTER(C.c_void_p), np.ctypeslib.ndpointer(dtype=np.int64)]),
+        "chm_io_read_dims": (i, [C.c_char_p, i, i, i, i, vp, C.POINTER(i)]),
+        "chm_io_read_site": (i, [C.c_char_p, i, vp, i64, vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(l, name)
+        f.restype = res
+        f.argtypes = args
+    _io = l
+    return l
+
+
+def io_write(path, L, N, two_s, nirrep, converged, dims, mps):
+    """Host-only checkpoint writer: dims over SyBookkeeper's sector ranges, mps a list of
+    L arrays in TensorT order (DMRG::saveMPS layout)."""
+    l = load_io()
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in mps]
+    ptrs = (C.c_void_p * L)(*[a.ctypes.data for a in arrs])
+    sizes = np.array([a.size for a in arrs], dtype=np.int64)
+    rc = l.chm_io_write(path.encode(), L, N, two_s, nirrep, int(converged), np.ascontiguousarray(dims, dtype=np.int32),
+                        ptrs, sizes)
+    if rc != 0:
+        raise ChmError(f"chm_io_write failed ({rc})")
+
+
+def io_read_dims(path, L, N, two_s, nirrep):
+    l = load_io()
+    n = C.c_int(0)
+    l.chm_io_read_dims(path.encode(), L, N, two_s, nirrep, None, C.byref(n))
+    out = np.zeros(n.value, dtype=np.int32)
+    rc = l.chm_io_read_dims(path.encode(), L, N, two_s, nirrep, out.ctypes.data, C.byref(n))
+    if rc != 0:
+        raise ChmError(f"chm_io_read_dims failed ({rc})")
+    return out
+
+
+def io_read_site(path, site, size):
+    l = load_io()
+    out = np.zeros(max(size, 1))
+    conv = C.c_int(-1)
+    rc = l.chm_io_read_site(path.encode(), int(site), out.ctypes.data, int(size), C.addressof(conv))
+    if rc != 0:
+        raise ChmError(f"chm_io_read_site failed ({rc})")
+    return out[:size], conv.value
 
 
 def _ptr(a):
@@ -154,8 +222,14 @@ class Context:
         data = np.ascontiguousarray(data, dtype=np.float64)
         self._check(self.lib.chm_mps_upload(self.h, int(site), data, data.size), "chm_mps_upload")
 
-    def mps_download(self, site, n):
-        out = np.zeros(int(n))
+    def mps_layout_size(self, site):
+        """TensorT storage size of a site under the current sectors."""
+        n = C.c_int64(0)
+        self._check(self.lib.chm_mps_layout_size(self.h, int(site), C.byref(n)), "chm_mps_layout_size")
+        return n.value
+
+    def mps_download(self, site, n=None):
+        out = np.zeros(int(self.mps_layout_size(site) if n is None else n))
         self._check(self.lib.chm_mps_download(self.h, int(site), out, out.size), "chm_mps_download")
         return out
 
@@ -190,6 +264,30 @@ class Context:
 
     def dmrg_init(self, D, seed=1):
         self._check(self.lib.chm_dmrg_init(self.h, int(D), int(seed)), "chm_dmrg_init")
+
+    def mps_save(self, path, converged=False):
+        """DMRG::saveMPS: sectors and site tensors in the reference's HDF5 layout."""
+        rc = load_io().chm_mps_save(self.h, str(path).encode(), int(converged))
+        self._check(rc, "chm_mps_save")
+
+    def mps_load_dims(self, path):
+        """DMRG::loadDIM: the file's virtual dimensions become the current sectors."""
+        self._check(load_io().chm_mps_load_dims(self.h, str(path).encode()), "chm_mps_load_dims")
+
+    def mps_load(self, path):
+        """DMRG::loadMPS: every site tensor from the file; returns the converged flag."""
+        conv = C.c_int(0)
+        self._check(load_io().chm_mps_load(self.h, str(path).encode(), C.byref(conv)), "chm_mps_load")
+        return bool(conv.value)
+
+    def dmrg_presolve(self):
+        """DMRG::PreSolve on the current MPS."""
+        self._check(self.lib.chm_dmrg_presolve(self.h), "chm_dmrg_presolve")
+
+    def get_problem(self):
+        v = [C.c_int(0) for _ in range(5)]
+        self._check(self.lib.chm_get_problem(self.h, *[C.byref(x) for x in v]), "chm_get_problem")
+        return dict(zip(["L", "N", "two_s", "irrep", "nirrep"], [x.value for x in v]))
 
     def dmrg_solve(self, scheme):
         """scheme: list of (D, econv, max_sweeps, noise, rtol), as ConvergenceScheme."""

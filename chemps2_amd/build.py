@@ -31,7 +31,28 @@ def build(verbose=False):
     subprocess.run(cmd, check=True)
     for o in objs:
         os.remove(o)
+    build_io(verbose)
     return LIB
+
+
+HDF5 = "/opt/conda"
+IO_LIB = os.path.join(HERE, "libchemps2_amd_io.so")
+
+
+def build_io(verbose=False):
+    """libchemps2_amd_io.so: MPS checkpoints in the reference's HDF5 layout, on top of the
+    C ABI of libchemps2_amd (host code only). Skipped when no HDF5 is installed."""
+    if not os.path.exists(os.path.join(HDF5, "include", "hdf5.h")):
+        print("chemps2_amd: no HDF5 under " + HDF5 + "; the checkpoint library is not built", file=sys.stderr)
+        return None
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", IO_LIB, os.path.join(CSRC, "mpsio.cpp"),
+           "-I" + os.path.join(HDF5, "include"), "-L" + os.path.join(HDF5, "lib"), "-lhdf5", "-L" + HERE, "-lchemps2_amd",
+           # the system's libstdc++ (which ROCm needs) before the older one beside libhdf5
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/usr/lib/x86_64-linux-gnu", "-Wl,-rpath," + os.path.join(HDF5, "lib")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return IO_LIB
 
 
 if __name__ == "__main__":

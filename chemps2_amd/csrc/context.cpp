@@ -1542,6 +1542,16 @@ int chm_mps_upload(chm_ctx* ctx, int site, const double* data, int64_t n) {
   });
 }
 
+int chm_mps_layout_size(const chm_ctx* ctx, int site, int64_t* n) {
+  if (!ctx || !n) return CHM_ERR_ARG;
+  const Ctx& c = ctx->c;
+  if (!c.have_problem || site < 0 || site >= c.L) return CHM_ERR_ARG;
+  chm::TLay T;
+  T.build(c.sec, site, c.irreps[site]);
+  *n = T.size;
+  return CHM_OK;
+}
+
 int chm_mps_download(chm_ctx* ctx, int site, double* data, int64_t n) {
   return guard(ctx, [&](Ctx& c) {
     if (!c.have_problem || site < 0 || site >= c.L || c.mps[site].buf < 0) throw Error{CHM_ERR_STATE, "no MPS tensor at this site"};
@@ -1580,6 +1590,24 @@ int chm_dmrg_init(chm_ctx* ctx, int D, uint64_t seed) {
     c.n_matvec_total = 0;
     for (double& t : c.tm) t = 0.0;
     c.mv_flops = c.mv_ms = 0.0;
+  });
+}
+
+int chm_get_problem(const chm_ctx* ctx, int* L, int* N, int* two_s, int* irrep, int* nirrep) {
+  if (!ctx || !L || !N || !two_s || !irrep || !nirrep) return CHM_ERR_ARG;
+  if (!ctx->c.have_problem) return CHM_ERR_STATE;
+  *L = ctx->c.L; *N = ctx->c.N; *two_s = ctx->c.TwoS; *irrep = ctx->c.Irrep; *nirrep = ctx->c.nirrep;
+  return CHM_OK;
+}
+
+int chm_dmrg_presolve(chm_ctx* ctx) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!c.have_problem) throw Error{CHM_ERR_STATE, "set the problem first"};
+    if ((int)c.fci.size() != c.L + 1) throw Error{CHM_ERR_STATE, "call chm_set_sectors_auto first (FCI dimensions)"};
+    for (int s = 0; s < c.L; s++)
+      if (c.mps[s].buf < 0 || !c.bufs[c.mps[s].buf]) throw Error{CHM_ERR_MISSING, "MPS tensor of site " + std::to_string(s) + " not set"};
+    (void)hipSetDevice(c.device);
+    c.presolve();
   });
 }
 

@@ -68,6 +68,10 @@ int chm_set_sectors(chm_ctx* ctx, int boundary, int n, const int* N, const int* 
 int chm_set_sectors_auto(chm_ctx* ctx, int D);
 int chm_get_sectors(const chm_ctx* ctx, int boundary, int* n, int* N, int* two_s, int* irrep, int* dim);
 
+/* The problem set by chm_set_problem: Problem::gL, gN, gTwoS, gIrrep and the number of
+ * irreps of the group (Problem.h:44-80, Irreps::getNumberOfIrreps). */
+int chm_get_problem(const chm_ctx* ctx, int* L, int* N, int* two_s, int* irrep, int* nirrep);
+
 /* One operator tensor of slot cnt (boundary cnt+1), built moving right (1) or left
  * (0): the storage of TensorOperator::gStorage() (TensorOperator.cpp:29-102) for
  * DMRG::<kind>tensors[cnt][k][m] (k, m ignored where the array has fewer indices).
@@ -86,6 +90,9 @@ int chm_tensor_clear(chm_ctx* ctx, int cnt);
  * (NL, 2SL, IL) x (NR, 2SR, IR) for the current sectors of boundaries site and site+1). */
 int chm_mps_upload(chm_ctx* ctx, int site, const double* data, int64_t n);
 int chm_mps_download(chm_ctx* ctx, int site, double* data, int64_t n);
+/* Number of doubles of site's TensorT storage for the current sectors
+ * (TensorT::gKappa2index(gNKappa()), TensorT.cpp:38-104). */
+int chm_mps_layout_size(const chm_ctx* ctx, int site, int64_t* n);
 /* Copy of one operator tensor of slot cnt to the host (the TensorOperator storage). */
 int chm_tensor_download(chm_ctx* ctx, int cnt, int kind, int k, int m, double* data, int64_t n);
 /* Renormalised operators of slot `index` built on the device from the neighbouring slot
@@ -113,6 +120,11 @@ int chm_split(chm_ctx* ctx, int site, int D, int moving_right, int change, doubl
  * prefactor, Davidson tolerance) swept as DMRG::Solve (DMRG.cpp:268-417); energy =
  * the lowest energy met (TotalMinEnergy, with the constant of the Hamiltonian). */
 int chm_dmrg_init(chm_ctx* ctx, int D, uint64_t seed);
+/* DMRG::PreSolve (DMRG.cpp:255-265) on the current MPS (for instance one loaded from a
+ * checkpoint with chm_mps_upload): operators of every boundary rebuilt moving right,
+ * TotalMinEnergy and the last sweep's discarded weight reset. Needs the FCI dimensions
+ * of chm_set_sectors_auto for the later Splits. */
+int chm_dmrg_presolve(chm_ctx* ctx);
 int chm_dmrg_solve(chm_ctx* ctx, int n, const int* D, const double* econv, const int* max_sweeps, const double* noise,
                    const double* rtol, double* energy);
 /* One half sweep, DMRG::sweepleft (moving_right = 0) or DMRG::sweepright

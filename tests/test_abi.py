@@ -32,3 +32,17 @@ def test_no_device_create_fails_cleanly():
     lib = chemps2_amd.load()
     h = ctypes.c_void_p()
     assert lib.chm_ctx_create(0, ctypes.byref(h)) != 0
+
+
+def test_checkpoint_library_exports():
+    # include/chemps2_amd_io.h: the HDF5 checkpoint entry points (libchemps2_amd_io.so)
+    text = open(os.path.join(ROOT, "include", "chemps2_amd_io.h")).read()
+    names = sorted(set(re.findall(r"\b(chm_[a-z0-9_]+)\s*\(", text)))
+    assert names == sorted(chemps2_amd.IO_EXPORTS)
+    if not os.path.exists(chemps2_amd.IO_LIB_PATH):
+        import pytest
+
+        pytest.skip("checkpoint library not built (no HDF5)")
+    lib = chemps2_amd.load_io()
+    for name in names:
+        assert hasattr(lib, name), name
