@@ -18,6 +18,13 @@ sweepright over sites 0..29 with truncation to D): Join, Heff preparation, David
 is reported under "sweep", with the phase times and the Heff.x rate averaged over
 every application of the sweep (sum of n_matvec * F_H over the summed device time).
 
+Both are also measured in the mixed form ("mixed"): the explicit sector matrix stored as
+float pairs, hi-only products in the Davidson until its residual is below 1e-5, hi + lo
+products after that. |dE| of the forms is measured twice: the Davidson eigenvalue at the
+benchmark site from one guess (rtol 1e-8), where only the product precision differs,
+and the energies after the sweep on the same seeds and schedule, where the two runs
+also follow different Davidson trajectories (rtol 1e-5) from the random start.
+
 One JSON line goes to stdout; progress goes to stderr.
 """
 import argparse
@@ -88,6 +95,74 @@ def cpu_baseline(L, D, site, seed, irr, T, V, x, y_gpu, budget_s):
     }, err
 
 
+def timed_apply(ctx, x, y, steps, warmup, f32=False):
+    """ms per application (events on the library stream) and per-kernel ms."""
+    import torch
+
+    stream = torch.cuda.ExternalStream(ctx.stream())
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        ctx.apply(x, y, f32=f32)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        ctx.apply(x, y, f32=f32)
+    e1.record(stream)
+    e1.synchronize()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    ctx.timing(True)
+    for _ in range(steps):
+        ctx.apply(x, y, f32=f32)
+    kms, cnt = ctx.timing_read()
+    ctx.timing(False)
+    return ms, [v / max(cnt, 1) for v in kms]
+
+
+def run_sweep(L, D, seed, irr, mx, rtol, form):
+    """Initialisation and the first left+right sweep of DMRG::Solve in one Heff form."""
+    import chemps2_amd
+
+    name = "mixed" if form == chemps2_amd.FORM_MIXED else "fp64"
+    sw = chemps2_amd.Context(0)
+    sw.set_heff_form(form)
+    sw.set_problem(L, 7, irr, L, 0, 0, mx)
+    t0 = time.time()
+    sw.dmrg_init(D, seed + 1)
+    t_init = time.time() - t0
+    log(f"sweep ({name}): MPS and operators of every boundary built in {t_init:.1f}s")
+    t0 = time.time()
+    e_left = sw.dmrg_sweep(False, D, rtol, 0.0, False)
+    t_left = time.time() - t0
+    log(f"sweep ({name}): left half (sites {L - 2}..1) in {t_left:.1f}s, energy {e_left:.10f}")
+    t1 = time.time()
+    e_right = sw.dmrg_sweep(True, D, rtol, 0.0, True)
+    t_right = time.time() - t1
+    phases, nmv = sw.dmrg_stats()
+    hf, hms = sw.dmrg_heff_stats()
+    low, full = sw.mixed_stats()
+    log(f"sweep ({name}): right half (sites 0..{L - 3}) in {t_right:.1f}s, energy {e_right:.10f}; {nmv} Heff "
+        f"applications ({low} hi-only), {hf / max(hms, 1e-9) / 1e9:.2f} TFLOP/s over the sweep; phases {phases}")
+    sw.close()
+    return {
+        "seconds": round(t_left + t_right, 3),
+        "left_seconds": round(t_left, 3),
+        "right_seconds": round(t_right, 3),
+        "init_seconds": round(t_init, 3),
+        "D": D,
+        "davidson_rtol": rtol,
+        "heff_applications": nmv,
+        "heff_applications_hi_only": low,
+        "heff_tflops_sweep_average": round(hf / max(hms, 1e-9) / 1e9, 4),
+        "heff_device_seconds": round(hms / 1e3, 3),
+        "energy_left_half": e_left,
+        "energy_last_site": e_right,
+        "phase_seconds": {k: round(v, 3) for k, v in phases.items()},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +175,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--sweep-rtol", type=float, default=1e-5)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--dav-rtol", type=float, default=1e-8)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -133,30 +210,9 @@ def main():
     xh = rng.normal(size=Vlen)
     x = torch.from_numpy(xh).to("cuda")
     y = torch.empty_like(x)
-    stream = torch.cuda.ExternalStream(ctx.stream())
-    torch.cuda.synchronize()
-    for _ in range(args.warmup):
-        ctx.apply(x, y)
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.steps):
-        ctx.apply(x, y)
-    e1.record(stream)
-    e1.synchronize()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
+    ms, kms = timed_apply(ctx, x, y, args.steps, args.warmup)
     y_gpu = y.cpu().numpy()
     log(f"Heff x: {ms:.3f} ms per application, {F_H / ms / 1e9:.2f} TFLOP/s (F_H basis)")
-
-    # per-kernel durations with events around each launch
-    ctx.timing(True)
-    for _ in range(args.steps):
-        ctx.apply(x, y)
-    kms, cnt = ctx.timing_read()
-    ctx.timing(False)
-    kms = [v / max(cnt, 1) for v in kms]
     log(f"kernels (ms per application): chain phase 1 {kms[0]:.4f}, explicit block product {kms[1]:.4f}, "
         f"chain phase 2 {kms[2]:.4f}")
     nnz = st["explicit_doubles"]
@@ -172,6 +228,45 @@ def main():
         "traffic": read_traffic(),
     }
 
+    # same-site Davidson in both forms from one guess: |dE| from the product precision
+    guess = rng.normal(size=Vlen)
+    g = guess.copy()
+    e_dav64, nm64 = ctx.davidson(g, args.dav_rtol)
+    log(f"Davidson at the site (FP64, rtol {args.dav_rtol:g}): {e_dav64!r} after {nm64} products")
+
+    mixed = None
+    if not args.no_mixed:
+        # the same site prepared in the mixed form (same bytes for the explicit matrix)
+        ctx.set_heff_form(chemps2_amd.FORM_MIXED)
+        t0 = time.time()
+        ctx.prepare(site)
+        t_prep_m = time.time() - t0
+        ms32, kms32 = timed_apply(ctx, x, y, args.steps, args.warmup, f32=True)
+        d32 = float(np.abs(y.cpu().numpy() - y_gpu).max() / np.abs(y_gpu).max())
+        msr, kmsr = timed_apply(ctx, x, y, args.steps, args.warmup)
+        dr = float(np.abs(y.cpu().numpy() - y_gpu).max() / np.abs(y_gpu).max())
+        b32 = 4.0 * nnz + 16.0 * Vlen + ncol_bytes
+        log(f"mixed: prepare {t_prep_m:.1f}s; hi-only Heff x {ms32:.3f} ms ({F_H / ms32 / 1e9:.2f} TFLOP/s, explicit product "
+            f"{kms32[1]:.4f} ms = {b32 / (kms32[1] * 1e-3) / 1e9:.0f} GB/s), max rel. difference to FP64 {d32:.2e}; "
+            f"hi + lo {msr:.3f} ms, difference {dr:.2e}")
+        g = guess.copy()
+        e_davm, nmm = ctx.davidson(g, args.dav_rtol)
+        low, full = ctx.mixed_stats()
+        log(f"Davidson at the site (mixed): {e_davm!r} after {nmm} products ({low} hi-only), "
+            f"|dE| = {abs(e_davm - e_dav64):.2e}")
+        mixed = {
+            "davidson_same_site": {"rtol": args.dav_rtol, "energy_fp64": e_dav64, "energy_mixed": e_davm,
+                                   "delta_E": abs(e_davm - e_dav64), "products_fp64": nm64, "products_mixed": nmm,
+                                   "products_mixed_hi_only": low},
+            "heff_tflops_hi_only": round(F_H / (ms32 * 1e-3) / 1e12, 4),
+            "ms_per_step_hi_only": round(ms32, 4),
+            "explicit_product_gbs_hi_only": round(b32 / (kms32[1] * 1e-3) / 1e9, 1) if kms32[1] > 0 else None,
+            "kernel_ms_hi_only": [round(v, 4) for v in kms32],
+            "max_rel_diff_hi_only_vs_fp64": d32,
+            "ms_per_step_refined": round(msr, 4),
+            "max_rel_diff_refined_vs_fp64": dr,
+        }
+
     mx = mx_table(T, V, L)
     ctx.close()
 
@@ -183,37 +278,13 @@ def main():
 
     sweep = None
     if not args.no_sweep:
-        sw = chemps2_amd.Context(0)
-        sw.set_problem(L, 7, irr, L, 0, 0, mx)
-        t0 = time.time()
-        sw.dmrg_init(D, seed + 1)
-        t_init = time.time() - t0
-        log(f"sweep: MPS and operators of every boundary built in {t_init:.1f}s")
-        t0 = time.time()
-        e_left = sw.dmrg_sweep(False, D, args.sweep_rtol, 0.0, False)
-        t_left = time.time() - t0
-        log(f"sweep: left half (sites {L - 2}..1) in {t_left:.1f}s, energy {e_left:.10f}")
-        t1 = time.time()
-        e_right = sw.dmrg_sweep(True, D, args.sweep_rtol, 0.0, True)
-        t_right = time.time() - t1
-        phases, nmv = sw.dmrg_stats()
-        hf, hms = sw.dmrg_heff_stats()
-        log(f"sweep: right half (sites 0..{L - 3}) in {t_right:.1f}s, energy {e_right:.10f}; {nmv} Heff applications, "
-            f"{hf / max(hms, 1e-9) / 1e9:.2f} TFLOP/s over the sweep; phases {phases}")
-        sweep = {
-            "seconds": round(t_left + t_right, 3),
-            "left_seconds": round(t_left, 3),
-            "right_seconds": round(t_right, 3),
-            "init_seconds": round(t_init, 3),
-            "D": D,
-            "davidson_rtol": args.sweep_rtol,
-            "heff_applications": nmv,
-            "heff_tflops_sweep_average": round(hf / max(hms, 1e-9) / 1e9, 4),
-            "heff_device_seconds": round(hms / 1e3, 3),
-            "energy_last_site": e_right,
-            "phase_seconds": {k: round(v, 3) for k, v in phases.items()},
-        }
-        sw.close()
+        sweep = run_sweep(L, D, seed, irr, mx, args.sweep_rtol, chemps2_amd.FORM_FP64)
+        if mixed is not None:
+            sm = run_sweep(L, D, seed, irr, mx, args.sweep_rtol, chemps2_amd.FORM_MIXED)
+            sm["delta_E_vs_fp64"] = abs(sm["energy_last_site"] - sweep["energy_last_site"])
+            sm["speedup_vs_fp64"] = round(sweep["seconds"] / sm["seconds"], 3)
+            mixed["sweep"] = sm
+            log(f"mixed sweep: {sm['seconds']:.1f}s vs FP64 {sweep['seconds']:.1f}s, |dE| = {sm['delta_E_vs_fp64']:.2e}")
 
     out = {
         "metric": "Heff·x TFLOP/s + Davidson sweep time at L=32 D=2000; |ΔE| vs FP64 ≤1e-8 Ha",
@@ -237,6 +308,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "sweep": sweep,
+        "mixed": mixed,
     }
     print(json.dumps(out), flush=True)
 

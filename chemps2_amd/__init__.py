@@ -22,7 +22,9 @@ EXPORTS = [
     "chm_davidson", "chm_flops", "chm_plan_stats", "chm_stream", "chm_timing_enable", "chm_timing_read",
     "chm_mps_upload", "chm_mps_download", "chm_tensor_download", "chm_update_operators",
     "chm_join", "chm_s_get", "chm_s_set", "chm_split", "chm_get_problem", "chm_mps_layout_size", "chm_dmrg_init", "chm_dmrg_presolve", "chm_dmrg_solve", "chm_dmrg_sweep", "chm_dmrg_heff_stats", "chm_dmrg_stats",
+    "chm_set_heff_form", "chm_heff_apply_f32", "chm_mixed_stats",
 ]
+FORM_FP64, FORM_MIXED = 0, 1
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -72,6 +74,9 @@ def load():
         "chm_heff_prepare": (i, [vp, i]),
         "chm_heff_veclen": (i, [vp, C.POINTER(i64)]),
         "chm_heff_apply": (i, [vp, vp, vp, i]),
+        "chm_heff_apply_f32": (i, [vp, vp, vp, i]),
+        "chm_set_heff_form": (i, [vp, i]),
+        "chm_mixed_stats": (i, [vp, C.POINTER(i64), C.POINTER(i64)]),
         "chm_heff_diag": (i, [vp, vp, i]),
         "chm_davidson": (i, [vp, vp, i, d, C.POINTER(d), C.POINTER(i)]),
         "chm_flops": (i, [vp, C.POINTER(d), C.POINTER(d)]),
@@ -326,14 +331,25 @@ class Context:
         self.V = v.value
         return self.V
 
-    def apply(self, x, y=None):
+    def apply(self, x, y=None, f32=False):
+        """y = Heff x; f32: with the hi floats of a site prepared in the mixed form."""
+        fn, name = (self.lib.chm_heff_apply_f32, "chm_heff_apply_f32") if f32 else (self.lib.chm_heff_apply, "chm_heff_apply")
         if isinstance(x, np.ndarray):
             x = np.ascontiguousarray(x, dtype=np.float64)
             y = np.zeros(self.V) if y is None else y
-            self._check(self.lib.chm_heff_apply(self.h, _ptr(x), _ptr(y), 0), "chm_heff_apply")
+            self._check(fn(self.h, _ptr(x), _ptr(y), 0), name)
             return y
-        self._check(self.lib.chm_heff_apply(self.h, _ptr(x), _ptr(y), 1), "chm_heff_apply")
+        self._check(fn(self.h, _ptr(x), _ptr(y), 1), name)
         return y
+
+    def set_heff_form(self, form):
+        """FORM_FP64 or FORM_MIXED; applies from the next prepare."""
+        self._check(self.lib.chm_set_heff_form(self.h, int(form)), "chm_set_heff_form")
+
+    def mixed_stats(self):
+        lo, hi = C.c_int64(0), C.c_int64(0)
+        self._check(self.lib.chm_mixed_stats(self.h, C.byref(lo), C.byref(hi)), "chm_mixed_stats")
+        return lo.value, hi.value
 
     def diag(self):
         d = np.zeros(self.V)

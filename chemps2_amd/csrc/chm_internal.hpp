@@ -387,9 +387,12 @@ void launch_chain_reduce(const DevBRed* reds, int nreds, const double* part, dou
 void launch_ops(const DevOItem* items, int nitems, const DevGemmOp* ops, int P, hipStream_t s);
 void launch_part_reduce(const DevPRed* reds, int nreds, const double* part, hipStream_t s);
 bool chain_kernel_setup();  // false: the large variant cannot get its LDS on this device
-void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs, double* M, hipStream_t s);
+// split: the matrix is stored as hi/lo float pairs (see k_assemble); mode 0 double
+// matrix, 1 hi floats only, 2 hi + lo
+void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs, double* M, int64_t nnz, bool split,
+                     hipStream_t s);
 void launch_bspmv(const DevBItem* items, int nitems, const DevBRed* reds, int nreds, double* P, const double* M,
-                  const int* col, const double* x, double* y, hipStream_t s);
+                  int64_t nnz, int mode, const int* col, const double* x, double* y, hipStream_t s);
 int dot_parts(int64_t n);
 void launch_dot(const double* a, const double* b, int64_t n, double* partial, double* out, hipStream_t s);
 void launch_multi_dot(const double* const* d_vecs, int nv, const double* b, int64_t n, double* partial, double* out,

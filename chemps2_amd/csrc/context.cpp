@@ -404,6 +404,8 @@ void Ctx::prepare(int st) {
   const double t_lists = now_s();
   {
     Mvals.alloc(std::max<int64_t>(tot.m, 1));
+    M_nnz = tot.m;
+    M_split = heff_form == 1;
     Mcol.alloc(std::max<int64_t>(tot.cols, 1));
     hip_check(hipMemcpyAsync(Mcol.p, h_col, std::max<int64_t>(tot.cols, 1) * sizeof(int), hipMemcpyHostToDevice, stream), "columns");
     // work items: rows in chunks of at most 256, columns in chunks of about
@@ -445,7 +447,7 @@ void Ctx::prepare(int st) {
       d_kterms.alloc(tot.kt);
       hip_check(hipMemcpyAsync(d_pairs.p, h_dp, tot.pairs * sizeof(DevPair), hipMemcpyHostToDevice, stream), "pairs");
       hip_check(hipMemcpyAsync(d_kterms.p, h_kt, tot.kt * sizeof(DevKTerm), hipMemcpyHostToDevice, stream), "terms");
-      launch_assemble(d_pairs.p, d_kterms.p, tot.pairs, Mvals.p, stream);
+      launch_assemble(d_pairs.p, d_kterms.p, tot.pairs, Mvals.p, tot.m, M_split, stream);
       hip_check(hipGetLastError(), "assembly kernel launch");
     }
     hip_check(hipStreamSynchronize(stream), "assembly");
@@ -736,7 +738,8 @@ void Ctx::apply(const double* xp, double* yp) {
   if (timing) hip_check(hipEventRecord(ev[0], stream), "event");
   launch_tiles(tiles1.p, ops1.p, ntiles1, xp, yp, stream);
   if (timing) hip_check(hipEventRecord(ev[1], stream), "event");
-  launch_bspmv(bitems.p, nbitems, breds.p, nbreds, bpart.p, Mvals.p, Mcol.p, xp, yp, stream);
+  launch_bspmv(bitems.p, nbitems, breds.p, nbreds, bpart.p, Mvals.p, M_nnz, M_split ? (mv_low ? 1 : 2) : 0, Mcol.p, xp,
+               yp, stream);
   if (timing) hip_check(hipEventRecord(ev[2], stream), "event");
   launch_tiles(tiles2.p, ops2.p, ntiles2, xp, yp, stream);
   launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, cpart.p, chain_P, false, stream);
@@ -1270,7 +1273,15 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
     copy_sync(stream, d_coef.p, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice, "coefs");
     launch_lincomb_vecs(d_ptrs.p, d_coef.p, (int)vs.size(), out, n, stream);
   };
+  // mixed form: products with the hi floats of the explicit matrix first, hi + lo once
+  // the residual has fallen below mixed_switch (restarting from the Ritz vector)
+  struct LowPhase {
+    bool& f;
+    ~LowPhase() { f = false; }
+  } low_phase{mv_low};
+  mv_low = M_split;
   auto matvec = [&](int i) {
+    (mv_low ? n_low_mults : n_full_mults)++;
     if (mv_timing) {
       if (mv_ev.empty()) {
         mv_ev.assign(512, nullptr);
@@ -1339,6 +1350,15 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
     }
     const double rnorm = std::sqrt(dot(t, t));
     if (!std::isfinite(rnorm)) throw Error{ERR_NUMERIC, "Davidson: residual is not finite"};
+    if (mv_low && rnorm <= std::max(rtol, mixed_switch)) {
+      mv_low = false;
+      hip_check(hipMemcpyAsync(t, u, n * sizeof(double), hipMemcpyDeviceToDevice, stream), "refinement start");
+      num_vec = 0;
+      add_new_vec();
+      matvec(num_vec);
+      nmult++;
+      continue;
+    }
     if (rnorm <= rtol) break;
     // Davidson.cpp:322-350
     launch_precond(dg, u, eig[0], CUT, work, t, n, 0, 0.0, stream);
@@ -1687,20 +1707,53 @@ int chm_heff_veclen(const chm_ctx* ctx, int64_t* V) {
   return CHM_OK;
 }
 
+static void apply_abi(Ctx& c, const double* x, double* y, int on_device) {
+  if (!x || !y) throw Error{CHM_ERR_ARG, "null pointer"};
+  if (c.site < 0) throw Error{CHM_ERR_STATE, "no prepared site"};
+  (void)hipSetDevice(c.device);
+  if (on_device) {
+    c.apply(x, y);
+  } else {
+    chm::hip_check(hipMemcpyAsync(c.dx.p, x, c.V * sizeof(double), hipMemcpyHostToDevice, c.stream), "x upload");
+    c.apply(c.dx.p, c.dy.p);
+    chm::hip_check(hipMemcpyAsync(y, c.dy.p, c.V * sizeof(double), hipMemcpyDeviceToHost, c.stream), "y download");
+    chm::hip_check(hipStreamSynchronize(c.stream), "apply");
+  }
+}
+
 int chm_heff_apply(chm_ctx* ctx, const double* x, double* y, int on_device) {
   return guard(ctx, [&](Ctx& c) {
-    if (!x || !y) throw Error{CHM_ERR_ARG, "null pointer"};
-    if (c.site < 0) throw Error{CHM_ERR_STATE, "no prepared site"};
-    (void)hipSetDevice(c.device);
-    if (on_device) {
-      c.apply(x, y);
-    } else {
-      chm::hip_check(hipMemcpyAsync(c.dx.p, x, c.V * sizeof(double), hipMemcpyHostToDevice, c.stream), "x upload");
-      c.apply(c.dx.p, c.dy.p);
-      chm::hip_check(hipMemcpyAsync(y, c.dy.p, c.V * sizeof(double), hipMemcpyDeviceToHost, c.stream), "y download");
-      chm::hip_check(hipStreamSynchronize(c.stream), "apply");
-    }
+    c.mv_low = false;
+    apply_abi(c, x, y, on_device);
   });
+}
+
+int chm_heff_apply_f32(chm_ctx* ctx, const double* x, double* y, int on_device) {
+  return guard(ctx, [&](Ctx& c) {
+    if (c.site >= 0 && !c.M_split) throw Error{CHM_ERR_STATE, "the site was prepared in the FP64 form"};
+    c.mv_low = true;
+    try {
+      apply_abi(c, x, y, on_device);
+    } catch (...) {
+      c.mv_low = false;
+      throw;
+    }
+    c.mv_low = false;
+  });
+}
+
+int chm_set_heff_form(chm_ctx* ctx, int form) {
+  return guard(ctx, [&](Ctx& c) {
+    if (form != CHM_FORM_FP64 && form != CHM_FORM_MIXED) throw Error{CHM_ERR_ARG, "unknown Heff form"};
+    c.heff_form = form;
+  });
+}
+
+int chm_mixed_stats(const chm_ctx* ctx, int64_t* low_products, int64_t* full_products) {
+  if (!ctx || !low_products || !full_products) return CHM_ERR_ARG;
+  *low_products = ctx->c.n_low_mults;
+  *full_products = ctx->c.n_full_mults;
+  return CHM_OK;
 }
 
 int chm_heff_diag(chm_ctx* ctx, double* d, int on_device) {

@@ -160,6 +160,15 @@ struct Ctx : HostState {
   int64_t explicit_threshold = 0;
   DBuf<double> Mvals;
   DBuf<int> Mcol;
+  // Heff forms: 0 FP64; 1 mixed: the explicit matrix stored as hi/lo float pairs, the
+  // Davidson iterating with the hi floats until the residual falls below mixed_switch
+  // and refining with hi + lo (2^-48) products after that.
+  int heff_form = 0;
+  bool M_split = false;              // storage of Mvals at the prepared site
+  int64_t M_nnz = 0;
+  bool mv_low = false;               // apply() reads the hi floats only
+  double mixed_switch = 1e-5;
+  int64_t n_low_mults = 0, n_full_mults = 0;
   bool use_chain_kernel = true;      // CHM_CHAIN_KERNEL=0: every chained term through k_tiles
   DBuf<DevCBlock> cblocks;
   DBuf<DevCGroup> cgroups;

@@ -125,8 +125,12 @@ __global__ __launch_bounds__(NT) void k_tiles(const DevTile* __restrict__ tiles,
 
 // Assembles the explicit sub-block of every (kappa, kappa') pair:
 // M[(l + dL r), (l' + dLi r')] = sum_t alpha_t op(A_t)[l, l'] op(B_t)[r', r].
+// SPLIT stores each value v as two floats, hi = fl32(v) in the first nnz floats of the
+// buffer and lo = fl32(v - hi) in the next nnz: hi alone is v to 2^-24, hi + lo (exact
+// in double) is v to 2^-48, in the same bytes as the double matrix.
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_assemble(const DevPair* __restrict__ pairs, const DevKTerm* __restrict__ terms,
-                                                  double* __restrict__ M) {
+                                                  double* __restrict__ M, int64_t nnz) {
   const DevPair p = pairs[blockIdx.x];
   const int ncol = p.dLi * p.dRi;
   const int n = p.R * ncol;
@@ -141,7 +145,14 @@ __global__ __launch_bounds__(256) void k_assemble(const DevPair* __restrict__ pa
       const double b = t.B ? (t.tb ? t.B[r + (size_t)t.ldb * ri] : t.B[ri + (size_t)t.ldb * r]) : (r == ri ? 1.0 : 0.0);
       v += t.alpha * a * b;
     }
-    M[p.moff + e] = v;
+    if constexpr (SPLIT) {
+      float* hi = reinterpret_cast<float*>(M);
+      const float h = (float)v;
+      hi[p.moff + e] = h;
+      hi[nnz + p.moff + e] = (float)(v - (double)h);
+    } else {
+      M[p.moff + e] = v;
+    }
   }
 }
 
@@ -149,10 +160,25 @@ __global__ __launch_bounds__(256) void k_assemble(const DevPair* __restrict__ pa
 // columns). The item's x values are gathered into LDS first; then thread t takes row
 // t % R and every ncl-th column (ncl = 256 / R column lanes) with eight matrix loads in
 // flight, streamed non-temporally since every element is used once per product. The
-// column lanes are summed through LDS in a fixed order.
+// column lanes are summed through LDS in a fixed order. MODE 0 reads the double
+// matrix; MODE 1 the hi floats of the split storage (half the bytes); MODE 2 hi + lo.
+// Products and sums are in double in every mode.
 constexpr int BX_COLS = 4096;
+template <int MODE>
+__device__ __forceinline__ double mload(const double* M, int64_t nnz, int64_t e) {
+  if constexpr (MODE == 0) {
+    return __builtin_nontemporal_load(M + e);
+  } else if constexpr (MODE == 1) {
+    return (double)__builtin_nontemporal_load(reinterpret_cast<const float*>(M) + e);
+  } else {
+    const float* hi = reinterpret_cast<const float*>(M);
+    return (double)__builtin_nontemporal_load(hi + e) + (double)__builtin_nontemporal_load(hi + nnz + e);
+  }
+}
+
+template <int MODE>
 __global__ __launch_bounds__(256) void k_bspmv(const DevBItem* __restrict__ items, const double* __restrict__ M,
-                                               const int* __restrict__ col, const double* __restrict__ x,
+                                               int64_t nnz, const int* __restrict__ col, const double* __restrict__ x,
                                                double* __restrict__ y, double* __restrict__ P) {
   __shared__ double xs[BX_COLS];
   __shared__ double part[256];
@@ -168,17 +194,17 @@ __global__ __launch_bounds__(256) void k_bspmv(const DevBItem* __restrict__ item
 #pragma unroll
   for (int u = 0; u < 8; u++) a[u] = 0.0;
   if (cl < ncl) {
-    const double* m = M + it.moff + r + (size_t)it.ld * it.c0;
-    const size_t ld = (size_t)it.ld;
+    const int64_t m = it.moff + r + (int64_t)it.ld * it.c0;
+    const int64_t ld = it.ld;
     int c = cl;
     for (; c + 7 * ncl < nc; c += 8 * ncl) {
       double mv[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) mv[u] = __builtin_nontemporal_load(m + ld * (c + u * ncl));
+      for (int u = 0; u < 8; u++) mv[u] = mload<MODE>(M, nnz, m + ld * (c + u * ncl));
 #pragma unroll
       for (int u = 0; u < 8; u++) a[u] = fma(mv[u], xs[c + u * ncl], a[u]);
     }
-    for (; c < nc; c += ncl) a[0] = fma(__builtin_nontemporal_load(m + ld * c), xs[c], a[0]);
+    for (; c < nc; c += ncl) a[0] = fma(mload<MODE>(M, nnz, m + ld * c), xs[c], a[0]);
   }
   part[t] = (cl < ncl) ? ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7])) : 0.0;
   __syncthreads();
@@ -785,16 +811,22 @@ void launch_tiles(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const 
   hipLaunchKernelGGL(k_tiles, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase);
 }
 
-void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs, double* M, hipStream_t s) {
+void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs, double* M, int64_t nnz, bool split,
+                     hipStream_t s) {
   for (int64_t p0 = 0; p0 < npairs; p0 += (1LL << 30)) {
     const int64_t n = std::min<int64_t>(npairs - p0, 1LL << 30);
-    hipLaunchKernelGGL(k_assemble, dim3((unsigned)n), dim3(256), 0, s, pairs + p0, terms, M);
+    if (split) hipLaunchKernelGGL(k_assemble<true>, dim3((unsigned)n), dim3(256), 0, s, pairs + p0, terms, M, nnz);
+    else hipLaunchKernelGGL(k_assemble<false>, dim3((unsigned)n), dim3(256), 0, s, pairs + p0, terms, M, nnz);
   }
 }
 
 void launch_bspmv(const DevBItem* items, int nitems, const DevBRed* reds, int nreds, double* P, const double* M,
-                  const int* col, const double* x, double* y, hipStream_t s) {
-  if (nitems > 0) hipLaunchKernelGGL(k_bspmv, dim3(nitems), dim3(256), 0, s, items, M, col, x, y, P);
+                  int64_t nnz, int mode, const int* col, const double* x, double* y, hipStream_t s) {
+  if (nitems > 0) {
+    if (mode == 1) hipLaunchKernelGGL(k_bspmv<1>, dim3(nitems), dim3(256), 0, s, items, M, nnz, col, x, y, P);
+    else if (mode == 2) hipLaunchKernelGGL(k_bspmv<2>, dim3(nitems), dim3(256), 0, s, items, M, nnz, col, x, y, P);
+    else hipLaunchKernelGGL(k_bspmv<0>, dim3(nitems), dim3(256), 0, s, items, M, nnz, col, x, y, P);
+  }
   if (nreds > 0) hipLaunchKernelGGL(k_bspmv_reduce, dim3(nreds), dim3(256), 0, s, reds, P, y);
 }
 

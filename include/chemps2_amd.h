@@ -149,6 +149,21 @@ int chm_heff_prepare(chm_ctx* ctx, int site);
 int chm_heff_veclen(const chm_ctx* ctx, int64_t* V);
 /* y = Heff x in the symmetric convention (Heff::makeHeff, Heff.cpp:43-248). */
 int chm_heff_apply(chm_ctx* ctx, const double* x, double* y, int on_device);
+/* Heff forms (no reference counterpart; the reference computes in FP64 only).
+ * CHM_FORM_FP64: every product in FP64. CHM_FORM_MIXED: the explicit sector matrix is
+ * stored as float pairs (hi = fl32(v), lo = fl32(v - hi)); chm_davidson iterates with
+ * the hi floats until the residual norm falls below max(rtol, 1e-5), then restarts
+ * from the Ritz vector with hi + lo products (v to 2^-48) until it falls below rtol.
+ * Products and sums are FP64 in both forms. Takes effect at the next chm_heff_prepare. */
+#define CHM_FORM_FP64 0
+#define CHM_FORM_MIXED 1
+int chm_set_heff_form(chm_ctx* ctx, int form);
+/* y = Heff x with the hi floats only (a site prepared in CHM_FORM_MIXED); chm_heff_apply
+ * is the refined product in that form. */
+int chm_heff_apply_f32(chm_ctx* ctx, const double* x, double* y, int on_device);
+/* Products chm_davidson made with the hi floats and with full precision since the
+ * context was created. */
+int chm_mixed_stats(const chm_ctx* ctx, int64_t* low_products, int64_t* full_products);
 /* d = diag(Heff) (Heff::fillHeffDiag, Heff.cpp:250-315; nLower = 0). */
 int chm_heff_diag(chm_ctx* ctx, double* d, int on_device);
 /* Lowest eigenpair of Heff from the guess x (program convention, as Sobject storage
