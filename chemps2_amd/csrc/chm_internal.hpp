@@ -384,6 +384,12 @@ void launch_diag(const DevDiagBlock* blocks, const DevDiag* terms, int nblocks, 
 void launch_chain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups, const DevCTerm* terms, const double* x,
                   double* y, double* part, int P, bool large, hipStream_t s);
 void launch_chain_reduce(const DevBRed* reds, int nreds, const double* part, double* y, hipStream_t s);
+// Chained products on the matrix cores: work items sorted by tile count nt = 1..3
+// (every dimension of the item's blocks at most 16 nt); items [offs[nt-1], offs[nt]).
+// offs is a host array. Larger blocks run in k_chain.
+constexpr int CHAIN_MFMA_NT = 3;
+void launch_chain_mfma(const DevCBlock* blocks, const int* offs, const DevCGroup* groups, const DevCTerm* terms,
+                       const double* x, double* y, double* part, hipStream_t s);
 void launch_ops(const DevOItem* items, int nitems, const DevGemmOp* ops, int P, hipStream_t s);
 void launch_part_reduce(const DevPRed* reds, int nreds, const double* part, hipStream_t s);
 bool chain_kernel_setup();  // false: the large variant cannot get its LDS on this device

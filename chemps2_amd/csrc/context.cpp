@@ -462,7 +462,7 @@ void Ctx::prepare(int st) {
   // CHAIN_MAXD run in the output-stationary kernel (two launches: block products up to
   // CHAIN_SMALLP, and larger ones when the device grants the LDS); the rest as GEMM tiles
   {
-    std::vector<DevCBlock> cb, cbl;
+    std::vector<DevCBlock> cb, cbl, cbm[CHAIN_MFMA_NT];
     std::vector<DevCGroup> cg;
     std::vector<DevCTerm> ct;
     std::vector<DevBRed> cr;
@@ -487,6 +487,7 @@ void Ctx::prepare(int st) {
       B.dL = dLof(k); B.dR = dRof(k);
       const int g_first = (int)cg.size();
       int Pk = B.dL * B.dR;
+      int dmax = std::max(B.dL, B.dR);
       for (size_t a = q0; a < q1;) {
         const int in = chain[a]->in;
         size_t e = a;
@@ -494,6 +495,7 @@ void Ctx::prepare(int st) {
         DevCGroup G{};
         G.xoff = S.k2i[in]; G.dLi = dLof(in); G.dRi = dRof(in); G.t0 = (int)ct.size();
         Pk = std::max({Pk, G.dLi * G.dRi, B.dL * G.dRi, G.dLi * B.dR, B.dL * G.dLi, G.dRi * B.dR});
+        dmax = std::max({dmax, G.dLi, G.dRi});
         for (size_t q = a; q < e; q++) {
           const Term& t = *chain[q];
           const int dL = B.dL, dR = B.dR, dLi = G.dLi, dRi = G.dRi;
@@ -539,16 +541,30 @@ void Ctx::prepare(int st) {
         cr.push_back({S.k2i[k], cpart_n, (int)nel, (int)items.size()});
         for (auto& it : items) { it.direct = 0; it.out = cpart_n; cpart_n += nel; }
       }
-      for (auto& it : items) {
-        if (Pk <= CHAIN_SMALLP) cb.push_back(it);
-        else cbl.push_back(it);
+      const int nt = (dmax + 15) / 16;
+      if (use_chain_mfma && nt <= CHAIN_MFMA_NT) {
+        for (auto& it : items) cbm[nt - 1].push_back(it);
+      } else {
+        for (auto& it : items) {
+          if (Pk <= CHAIN_SMALLP) cb.push_back(it);
+          else cbl.push_back(it);
+        }
+        if (Pk <= CHAIN_SMALLP) Pmax = std::max(Pmax, Pk);
+        else PmaxL = std::max(PmaxL, Pk);
       }
-      if (Pk <= CHAIN_SMALLP) Pmax = std::max(Pmax, Pk);
-      else PmaxL = std::max(PmaxL, Pk);
       q0 = q1;
     }
     if ((int64_t)ct.size() > INT32_MAX || (int64_t)cg.size() > INT32_MAX) throw Error{ERR_SIZE, "too many chained terms"};
     cblocks.upload(cb, stream);
+    {
+      std::vector<DevCBlock> all;
+      cblocks_m_offs[0] = 0;
+      for (int n = 0; n < CHAIN_MFMA_NT; n++) {
+        all.insert(all.end(), cbm[n].begin(), cbm[n].end());
+        cblocks_m_offs[n + 1] = (int)all.size();
+      }
+      cblocks_m.upload(all, stream);
+    }
     cgroups.upload(cg, stream);
     cterms.upload(ct, stream);
     ncblocks = (int)cb.size();
@@ -742,6 +758,7 @@ void Ctx::apply(const double* xp, double* yp) {
                yp, stream);
   if (timing) hip_check(hipEventRecord(ev[2], stream), "event");
   launch_tiles(tiles2.p, ops2.p, ntiles2, xp, yp, stream);
+  launch_chain_mfma(cblocks_m.p, cblocks_m_offs, cgroups.p, cterms.p, xp, yp, cpart.p, stream);
   launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, cpart.p, chain_P, false, stream);
   launch_chain(cblocks_l.p, ncblocks_l, cgroups.p, cterms.p, xp, yp, cpart.p, chain_P_l, true, stream);
   launch_chain_reduce(creds.p, ncreds, cpart.p, yp, stream);
@@ -1455,6 +1472,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_VERBOSE")) ctx->c.verbose = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_SVD_JACOBI_MAXN")) ctx->c.svd_jacobi_maxn = std::atoi(e);
   if (const char* e = std::getenv("CHM_CHAIN_KERNEL")) ctx->c.use_chain_kernel = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CHM_CHAIN_MFMA")) ctx->c.use_chain_mfma = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_OPS_KERNEL")) ctx->c.use_ops_kernel = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;

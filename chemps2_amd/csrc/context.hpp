@@ -170,6 +170,12 @@ struct Ctx : HostState {
   double mixed_switch = 1e-5;
   int64_t n_low_mults = 0, n_full_mults = 0;
   bool use_chain_kernel = true;      // CHM_CHAIN_KERNEL=0: every chained term through k_tiles
+  // CHM_CHAIN_MFMA=1: chained blocks up to 48 on the matrix cores (k_chain_mfma). Off by
+  // default: at L = 32, D = 2000 it matches k_chain for blocks of 33-48 and is slower
+  // below (its operand loads from global memory are not coalesced).
+  bool use_chain_mfma = false;
+  DBuf<DevCBlock> cblocks_m;         // k_chain_mfma items sorted by tile count
+  int cblocks_m_offs[CHAIN_MFMA_NT + 1] = {};
   DBuf<DevCBlock> cblocks;
   DBuf<DevCGroup> cgroups;
   DBuf<DevCTerm> cterms;

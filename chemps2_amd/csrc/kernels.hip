@@ -444,6 +444,187 @@ __global__ __launch_bounds__(256) void k_chain(const DevCBlock* __restrict__ blo
     }
   }
 }
+// Chained products on the matrix cores (v_mfma_f64_16x16x4f64), one wavefront per work
+// item and no LDS: the wave keeps its output block Y (dL x dR <= 16 NT) as NT x NT
+// 16 x 16 accumulator tiles. MFMA operand layout: lane (lr = lane % 16, lg = lane / 16)
+// supplies A(lr, k) and B(k, lr) for the k-index lg of each 4-wide step, and receives
+// D(4 r + lg, lr) in register r = 0..3. A product's D tile is therefore directly the B
+// operand of the next one (register r holds k-index 4 r + lg of step r), and the D tile of a
+// transposed product is the A operand: two-operand terms chain through registers.
+//   mode 3: T^T = X^T op(A)^T (dRi x dL tiles), then Y += al T op(B)
+//   mode 4: T = X op(B) (dLi x dR tiles), then Y += al op(A) T
+//   mode 0 (Y += al X) runs as mode 1 with an identity operand
+// Operands are read straight from global memory (operator blocks and x stay in the
+// caches across the terms of a work item); indices past a dimension read element 0
+// and contribute zero.
+typedef double chm_d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double ld0(const double* __restrict__ p, bool ok, int idx) {
+  const double v = p[ok ? idx : 0];
+  return ok ? v : 0.0;
+}
+
+#define CHM_MFMA(a, b, c) __builtin_amdgcn_mfma_f64_16x16x4f64((a), (b), (c), 0, 0, 0)
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_chain_mfma(const DevCBlock* __restrict__ blocks, int nblocks,
+                                                    const DevCGroup* __restrict__ groups,
+                                                    const DevCTerm* __restrict__ terms, const double* __restrict__ x,
+                                                    double* __restrict__ y, double* __restrict__ part) {
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= nblocks) return;  // the whole wave leaves; the kernel has no barriers
+  const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
+  const DevCBlock b = blocks[w];
+  const int dL = b.dL, dR = b.dR;
+  const int nL = (dL + 15) >> 4, nR = (dR + 15) >> 4;
+  chm_d4 acc[NT][NT];
+#pragma unroll
+  for (int i = 0; i < NT; i++)
+#pragma unroll
+    for (int j = 0; j < NT; j++) acc[i][j] = chm_d4{0.0, 0.0, 0.0, 0.0};
+  for (int g = b.g0; g < b.g0 + b.ng; g++) {
+    const DevCGroup G = groups[g];
+    const int dLi = G.dLi, dRi = G.dRi;
+    const double* __restrict__ X = x + G.xoff;  // dLi x dRi, column-major
+    for (int t = G.t0; t < G.t0 + G.nt; t++) {
+      const DevCTerm T = terms[t];
+      const double al = T.alpha;
+      // op(A)(i, k) = A[i * ai + k * ak] (dL x dLi), op(B)(k, j) = B[k * bk + j * bj] (dRi x dR)
+      const int ai = T.ta ? T.lda : 1, ak = T.ta ? 1 : T.lda;
+      const int bk = T.tb ? T.ldb : 1, bj = T.tb ? 1 : T.ldb;
+      // absent operands (identity) point at x, so that no address formed is invalid
+      const double* __restrict__ A = T.A ? T.A : X;
+      const double* __restrict__ B = T.B ? T.B : X;
+      const int mode = T.mode;
+      if (mode == 3) {
+        const int nRi = (dRi + 15) >> 4;
+        chm_d4 t1[NT][NT];
+#pragma unroll
+        for (int i = 0; i < NT; i++)
+#pragma unroll
+          for (int j = 0; j < NT; j++) t1[i][j] = chm_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int k = lg; k < ((dLi + 3) & ~3); k += 4) {
+          double av[NT], bv[NT];
+#pragma unroll
+          for (int i = 0; i < NT; i++) av[i] = ld0(X, k < dLi && 16 * i + lr < dRi, k + dLi * (16 * i + lr));
+#pragma unroll
+          for (int j = 0; j < NT; j++) bv[j] = ld0(A, k < dLi && 16 * j + lr < dL, (16 * j + lr) * ai + k * ak);
+#pragma unroll
+          for (int i = 0; i < NT; i++)
+            if (i < nRi)
+#pragma unroll
+              for (int j = 0; j < NT; j++)
+                if (j < nL) t1[i][j] = CHM_MFMA(av[i], bv[j], t1[i][j]);
+        }
+#pragma unroll
+        for (int i = 0; i < NT; i++) {
+          if (i >= nRi) break;
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int k = 16 * i + 4 * r + lg;
+            double bv[NT];
+#pragma unroll
+            for (int j = 0; j < NT; j++) bv[j] = ld0(B, k < dRi && 16 * j + lr < dR, k * bk + (16 * j + lr) * bj);
+#pragma unroll
+            for (int yi = 0; yi < NT; yi++)
+              if (yi < nL) {
+                const double a = al * t1[i][yi][r];
+#pragma unroll
+                for (int yj = 0; yj < NT; yj++)
+                  if (yj < nR) acc[yi][yj] = CHM_MFMA(a, bv[yj], acc[yi][yj]);
+              }
+          }
+        }
+      } else if (mode == 4) {
+        const int nLi = (dLi + 15) >> 4;
+        chm_d4 t1[NT][NT];
+#pragma unroll
+        for (int i = 0; i < NT; i++)
+#pragma unroll
+          for (int j = 0; j < NT; j++) t1[i][j] = chm_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int k = lg; k < ((dRi + 3) & ~3); k += 4) {
+          double av[NT], bv[NT];
+#pragma unroll
+          for (int i = 0; i < NT; i++) av[i] = ld0(X, 16 * i + lr < dLi && k < dRi, 16 * i + lr + dLi * k);
+#pragma unroll
+          for (int j = 0; j < NT; j++) bv[j] = ld0(B, k < dRi && 16 * j + lr < dR, k * bk + (16 * j + lr) * bj);
+#pragma unroll
+          for (int i = 0; i < NT; i++)
+            if (i < nLi)
+#pragma unroll
+              for (int j = 0; j < NT; j++)
+                if (j < nR) t1[i][j] = CHM_MFMA(av[i], bv[j], t1[i][j]);
+        }
+#pragma unroll
+        for (int i = 0; i < NT; i++) {
+          if (i >= nLi) break;
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int k = 16 * i + 4 * r + lg;
+            double av[NT];
+#pragma unroll
+            for (int yi = 0; yi < NT; yi++) av[yi] = al * ld0(A, 16 * yi + lr < dL && k < dLi, (16 * yi + lr) * ai + k * ak);
+#pragma unroll
+            for (int yi = 0; yi < NT; yi++)
+              if (yi < nL)
+#pragma unroll
+                for (int yj = 0; yj < NT; yj++)
+                  if (yj < nR) acc[yi][yj] = CHM_MFMA(av[yi], t1[i][yj][r], acc[yi][yj]);
+          }
+        }
+      } else if (mode <= 1) {  // Y += al op(A) X, X is dLi x dR; mode 0: op(A) = identity
+#pragma unroll 1
+        for (int k = lg; k < ((dLi + 3) & ~3); k += 4) {
+          double av[NT], bv[NT];
+#pragma unroll
+          for (int i = 0; i < NT; i++)
+            av[i] = mode == 0 ? (16 * i + lr == k ? al : 0.0)
+                              : al * ld0(A, 16 * i + lr < dL && k < dLi, (16 * i + lr) * ai + k * ak);
+#pragma unroll
+          for (int j = 0; j < NT; j++) bv[j] = ld0(X, k < dLi && 16 * j + lr < dR, k + dLi * (16 * j + lr));
+#pragma unroll
+          for (int i = 0; i < NT; i++)
+            if (i < nL)
+#pragma unroll
+              for (int j = 0; j < NT; j++)
+                if (j < nR) acc[i][j] = CHM_MFMA(av[i], bv[j], acc[i][j]);
+        }
+      } else if (mode == 2) {  // Y += al X op(B), X is dL x dRi
+#pragma unroll 1
+        for (int k = lg; k < ((dRi + 3) & ~3); k += 4) {
+          double av[NT], bv[NT];
+#pragma unroll
+          for (int i = 0; i < NT; i++) av[i] = al * ld0(X, 16 * i + lr < dL && k < dRi, 16 * i + lr + dL * k);
+#pragma unroll
+          for (int j = 0; j < NT; j++) bv[j] = ld0(B, k < dRi && 16 * j + lr < dR, k * bk + (16 * j + lr) * bj);
+#pragma unroll
+          for (int i = 0; i < NT; i++)
+            if (i < nL)
+#pragma unroll
+              for (int j = 0; j < NT; j++)
+                if (j < nR) acc[i][j] = CHM_MFMA(av[i], bv[j], acc[i][j]);
+        }
+      }
+    }
+  }
+  double* Y = b.direct ? y + b.out : part + b.out;
+#pragma unroll
+  for (int i = 0; i < NT; i++)
+#pragma unroll
+    for (int j = 0; j < NT; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int l = 16 * i + 4 * r + lg, c = 16 * j + lr;
+        if (l < dL && c < dR) {
+          if (b.direct) Y[l + dL * c] += acc[i][j][r];
+          else Y[l + dL * c] = acc[i][j][r];
+        }
+      }
+}
+#undef CHM_MFMA
+
 constexpr int CHAIN_PER_S = (18 * 18 + 255) / 256;                                  // 2 x 2 tiles of a 36 x 36 block
 constexpr int CHAIN_PER_L = ((CHAIN_MAXD / 2) * (CHAIN_MAXD / 2) + 255) / 256;     // of a 64 x 64 block
 
@@ -850,6 +1031,19 @@ void launch_chain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups,
   else
     hipLaunchKernelGGL((k_chain<CHAIN_PER_S, true>), dim3(nblocks), dim3(256), (size_t)6 * P * sizeof(double), s, blocks,
                        groups, terms, x, y, part, P);
+}
+
+void launch_chain_mfma(const DevCBlock* blocks, const int* offs, const DevCGroup* groups, const DevCTerm* terms,
+                       const double* x, double* y, double* part, hipStream_t s) {
+  for (int nt = 1; nt <= CHAIN_MFMA_NT; nt++) {
+    const int n = offs[nt] - offs[nt - 1];
+    if (n <= 0) continue;
+    const DevCBlock* bl = blocks + offs[nt - 1];
+    const dim3 grid((unsigned)((n + 3) / 4));
+    if (nt == 1) hipLaunchKernelGGL(k_chain_mfma<1>, grid, dim3(256), 0, s, bl, n, groups, terms, x, y, part);
+    else if (nt == 2) hipLaunchKernelGGL(k_chain_mfma<2>, grid, dim3(256), 0, s, bl, n, groups, terms, x, y, part);
+    else hipLaunchKernelGGL(k_chain_mfma<3>, grid, dim3(256), 0, s, bl, n, groups, terms, x, y, part);
+  }
 }
 
 void launch_ops(const DevOItem* items, int nitems, const DevGemmOp* ops, int P, hipStream_t s) {

@@ -114,10 +114,13 @@ def test_explicit_threshold_paths(oracle_lib, explicit_max, chain_kernel, monkey
 
 
 @pytest.mark.parametrize("site", [0, 3, 6])
-def test_chain_kernel_d2h(oracle_lib, site, monkeypatch):
-    # every term through the output-stationary chained kernel (small D2h blocks: all
-    # four term modes, transposed and plain operator blocks)
+@pytest.mark.parametrize("mfma", ["1", "0"])
+def test_chain_kernel_d2h(oracle_lib, site, mfma, monkeypatch):
+    # every term through the output-stationary chained kernels (small D2h blocks: all
+    # five term modes, transposed and plain operator blocks), on the matrix cores
+    # (k_chain_mfma) or in LDS tiles (k_chain)
     monkeypatch.setenv("CHM_EXPLICIT_MAX", "0")
+    monkeypatch.setenv("CHM_CHAIN_MFMA", mfma)
     c = chemps2_amd.Context(0)
     irr, T, V = random_abinitio(8, 11, group=7)
     o = oracle_lib.OracleDMRG(8, 7, irr, 8, 0, 0, T, V)
