@@ -35,6 +35,32 @@ def hubbard(L, t=1.0, U=4.0, periodic=False):
     return T, V
 
 
+def hubbard_cylinder(nx=8, ny=4, t=1.0, U=8.0):
+    """2D Hubbard model on an nx x ny cylinder, periodic around the ny-site circumference
+    and open along nx, snaked into a chain: site (x, y) has index ny x + y for even x and
+    ny x + ny - 1 - y for odd x. Returns T (hopping -t on every bond), V (U on site) and
+    the list of bonds (i, j) in chain indices."""
+    L = nx * ny
+
+    def idx(x, y):
+        return ny * x + (y if x % 2 == 0 else ny - 1 - y)
+
+    bonds = set()
+    for x in range(nx):
+        for y in range(ny):
+            if ny > 2 or y + 1 < ny:
+                bonds.add(tuple(sorted((idx(x, y), idx(x, (y + 1) % ny)))))
+            if x + 1 < nx:
+                bonds.add(tuple(sorted((idx(x, y), idx(x + 1, y)))))
+    T = np.zeros((L, L))
+    for i, j in bonds:
+        T[i, j] = T[j, i] = -t
+    V = np.zeros((L, L, L, L))
+    for i in range(L):
+        V[i, i, i, i] = U
+    return T, V, sorted(bonds)
+
+
 def random_abinitio(L, seed, group=7):
     """Seeded synthetic ab-initio Hamiltonian with D2h (or smaller) symmetry.
 

@@ -153,3 +153,47 @@ def hubbard(L, t=1.0, U=4.0, periodic=False):
     for i in range(L):
         V[i, i, i, i] = U
     return T, V
+
+
+def _hop_matrix(L, n, bonds):
+    """One spin species: sum over bonds (i, j) of a+_i a_j + a+_j a_i on the strings of n
+    electrons on L sites (Jordan-Wigner signs in site order)."""
+    strs = _strings(L, n)
+    idx = {s: k for k, s in enumerate(strs)}
+    rows, cols, vals = [], [], []
+    for k, s in enumerate(strs):
+        for (i, j) in bonds:
+            for (p, q) in ((i, j), (j, i)):
+                if (s >> q) & 1 and not (s >> p) & 1:
+                    sg = _sign(s, q)
+                    s2 = s ^ (1 << q)
+                    sg *= _sign(s2, p)
+                    s2 |= 1 << p
+                    rows.append(idx[s2])
+                    cols.append(k)
+                    vals.append(float(sg))
+    return sp.csr_matrix((vals, (rows, cols)), shape=(len(strs), len(strs))), np.array(strs)
+
+
+def hubbard_fci_energy(L, N, t=1.0, U=4.0, bonds=None, tol=1e-12):
+    """Lowest eigenvalue of the Hubbard Hamiltonian -t sum_<ij>,s (a+_is a_js + h.c.) +
+    U sum_i n_iu n_id in the S_z = 0 (N even) or 1/2 sector, as H_up x 1 + 1 x H_dn +
+    U diag: matrix-free Lanczos over the (alpha string, beta string) grid. For a
+    bipartite lattice at half filling the ground state has S = 0 (Lieb's theorem)."""
+    if bonds is None:
+        bonds = [(i, i + 1) for i in range(L - 1)]
+    na, nb = (N + 1) // 2, N // 2
+    Ha, sa = _hop_matrix(L, na, bonds)
+    Hb, sb = _hop_matrix(L, nb, bonds)
+    Ha, Hb = (-t * Ha).tocsr(), (-t * Hb).tocsr()
+    pop = np.vectorize(lambda v: bin(int(v)).count("1"))
+    dU = U * pop(np.bitwise_and.outer(sa, sb)).astype(np.float64)
+    na_s, nb_s = len(sa), len(sb)
+
+    def mv(v):
+        X = v.reshape(na_s, nb_s)
+        return (Ha @ X + (Hb @ X.T).T + dU * X).ravel()
+
+    op = spla.LinearOperator((na_s * nb_s, na_s * nb_s), matvec=mv, dtype=np.float64)
+    w = spla.eigsh(op, k=1, which="SA", tol=tol, v0=np.ones(na_s * nb_s), maxiter=100000)[0]
+    return float(w[0])

@@ -167,3 +167,19 @@ def test_split_svd_paths_agree(oracle_lib, monkeypatch, mr, D):
     assert np.array_equal(sec0, sec1)
     assert abs(dw0 - dw1) <= 1e-12 * max(1.0, dw1)
     assert np.abs(s0 - s1).max() <= 1e-11 * np.abs(s1).max()
+
+
+def test_hubbard_l12_against_fci():
+    # 1D Hubbard, L = 12, N = 12, 2S = 0, C1, t = 1 (open chain), U = 4; two instructions
+    # at D = 200 with energy convergence 1e-8 (first with noise 0.05 and Davidson
+    # tolerance 1e-5, second without noise at 1e-8); FCI over all 853,776 S_z = 0
+    # determinants (tests/fci.py hubbard_fci_energy)
+    from fci import hubbard_fci_energy
+
+    print("case Hubbard L=12 N=12 U=4 D=200", flush=True)
+    T, V = hubbard(12, t=1.0, U=4.0)
+    ef = hubbard_fci_energy(12, 12, t=1.0, U=4.0)
+    e, stats = run_dmrg(12, 0, [0] * 12, 12, 0, 0, T, V,
+                        [(200, 1e-8, 10, 0.05, 1e-5), (200, 1e-8, 10, 0.0, 1e-8)], 200)
+    print(f"  gpu {e!r} fci {ef!r} |dE| {abs(e - ef):.2e} stats {stats}", flush=True)
+    assert abs(e - ef) < 1e-8
