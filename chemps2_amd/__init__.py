@@ -23,7 +23,11 @@ EXPORTS = [
     "chm_mps_upload", "chm_mps_download", "chm_tensor_download", "chm_update_operators",
     "chm_join", "chm_s_get", "chm_s_set", "chm_split", "chm_get_problem", "chm_mps_layout_size", "chm_dmrg_init", "chm_dmrg_presolve", "chm_dmrg_solve", "chm_dmrg_sweep", "chm_dmrg_heff_stats", "chm_dmrg_stats",
     "chm_set_heff_form", "chm_heff_apply_f32", "chm_mixed_stats",
+    "chm_precond_mode", "chm_mlp_nparams", "chm_mlp_set", "chm_mlp_get", "chm_mlp_forward", "chm_mlp_loss_grad",
+    "chm_mlp_train", "chm_harvest", "chm_harvest_take",
 ]
+PRECOND_DIAGONAL, PRECOND_LEARNED = 0, 1
+MLP_IN, MLP_H = 8, 128
 FORM_FP64, FORM_MIXED = 0, 1
 
 _lib = None
@@ -77,6 +81,15 @@ def load():
         "chm_heff_apply_f32": (i, [vp, vp, vp, i]),
         "chm_set_heff_form": (i, [vp, i]),
         "chm_mixed_stats": (i, [vp, C.POINTER(i64), C.POINTER(i64)]),
+        "chm_precond_mode": (i, [vp, i]),
+        "chm_mlp_nparams": (i, [C.POINTER(i64)]),
+        "chm_mlp_set": (i, [vp, vp, i64]),
+        "chm_mlp_get": (i, [vp, vp, i64]),
+        "chm_mlp_forward": (i, [vp, vp, i64, vp]),
+        "chm_mlp_loss_grad": (i, [vp, vp, vp, vp, vp, i64, vp, C.POINTER(d)]),
+        "chm_mlp_train": (i, [vp, vp, vp, vp, vp, i64, i, i, d, C.c_uint64, C.POINTER(d), C.POINTER(d)]),
+        "chm_harvest": (i, [vp, i]),
+        "chm_harvest_take": (i, [vp, C.POINTER(i64), vp, vp, vp, vp]),
         "chm_heff_diag": (i, [vp, vp, i]),
         "chm_davidson": (i, [vp, vp, i, d, C.POINTER(d), C.POINTER(i)]),
         "chm_flops": (i, [vp, C.POINTER(d), C.POINTER(d)]),
@@ -91,6 +104,33 @@ def load():
         f.argtypes = args
     _lib = l
     return l
+
+
+def mlp_nparams():
+    n = C.c_int64(0)
+    load().chm_mlp_nparams(C.byref(n))
+    return n.value
+
+
+def mlp_init(seed=0):
+    """Initial parameters: He-normal hidden layers, zero output layer (m = 0, the plain
+    diagonal preconditioner)."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fan_in, fan_out in [(MLP_IN, MLP_H), (MLP_H, MLP_H), (MLP_H, MLP_H)]:
+        parts.append(rng.normal(scale=np.sqrt(2.0 / fan_in), size=(fan_in, fan_out)).ravel())
+        parts.append(np.zeros(fan_out))
+    parts.append(np.zeros(MLP_H))
+    parts.append(np.zeros(1))
+    return np.concatenate(parts).astype(np.float32)
+
+
+def _samples(X, p, e, w):
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_IN)
+    p, e, w = (np.ascontiguousarray(a, dtype=np.float32) for a in (p, e, w))
+    if not (len(X) == len(p) == len(e) == len(w)):
+        raise ChmError("sample arrays differ in length")
+    return X, p, e, w
 
 
 IO_LIB_PATH = os.path.join(HERE, "libchemps2_amd_io.so")
@@ -345,6 +385,54 @@ class Context:
     def set_heff_form(self, form):
         """FORM_FP64 or FORM_MIXED; applies from the next prepare."""
         self._check(self.lib.chm_set_heff_form(self.h, int(form)), "chm_set_heff_form")
+
+    # ---- learned preconditioner
+    def precond_mode(self, mode):
+        self._check(self.lib.chm_precond_mode(self.h, int(mode)), "chm_precond_mode")
+
+    def mlp_set(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self.lib.chm_mlp_set(self.h, p.ctypes.data, p.size), "chm_mlp_set")
+
+    def mlp_get(self):
+        p = np.zeros(mlp_nparams(), dtype=np.float32)
+        self._check(self.lib.chm_mlp_get(self.h, p.ctypes.data, p.size), "chm_mlp_get")
+        return p
+
+    def mlp_forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_IN)
+        out = np.zeros(len(X), dtype=np.float32)
+        self._check(self.lib.chm_mlp_forward(self.h, X.ctypes.data, len(X), out.ctypes.data), "chm_mlp_forward")
+        return out
+
+    def mlp_loss_grad(self, X, p, e, w):
+        X, p, e, w = _samples(X, p, e, w)
+        g = np.zeros(mlp_nparams(), dtype=np.float32)
+        loss = C.c_double(0.0)
+        self._check(self.lib.chm_mlp_loss_grad(self.h, X.ctypes.data, p.ctypes.data, e.ctypes.data, w.ctypes.data, len(p),
+                                               g.ctypes.data, C.byref(loss)), "chm_mlp_loss_grad")
+        return loss.value, g
+
+    def mlp_train(self, X, p, e, w, batch=4096, epochs=4, lr=1e-3, seed=1):
+        X, p, e, w = _samples(X, p, e, w)
+        l0, l1 = C.c_double(0.0), C.c_double(0.0)
+        self._check(self.lib.chm_mlp_train(self.h, X.ctypes.data, p.ctypes.data, e.ctypes.data, w.ctypes.data, len(p),
+                                           int(batch), int(epochs), float(lr), int(seed), C.byref(l0), C.byref(l1)),
+                    "chm_mlp_train")
+        return l0.value, l1.value
+
+    def harvest(self, per_iteration):
+        self._check(self.lib.chm_harvest(self.h, int(per_iteration)), "chm_harvest")
+
+    def harvest_take(self):
+        """(X, p, e, w) of the samples collected since the last take."""
+        n = C.c_int64(0)
+        self._check(self.lib.chm_harvest_take(self.h, C.byref(n), None, None, None, None), "chm_harvest_take")
+        X = np.zeros((n.value, MLP_IN), dtype=np.float32)
+        p, e, w = (np.zeros(n.value, dtype=np.float32) for _ in range(3))
+        self._check(self.lib.chm_harvest_take(self.h, C.byref(n), X.ctypes.data, p.ctypes.data, e.ctypes.data,
+                                              w.ctypes.data), "chm_harvest_take")
+        return X, p, e, w
 
     def mixed_stats(self):
         lo, hi = C.c_int64(0), C.c_int64(0)

@@ -408,8 +408,11 @@ void launch_axpy(double a, const double* x, double* y, int64_t n, hipStream_t s)
 void launch_scale(double a, double* x, int64_t n, hipStream_t s);
 void launch_lincomb_vecs(const double* const* d_vecs, const double* d_coefs, int nv, double* out, int64_t n,
                          hipStream_t s);
+// Davidson.cpp:322-350: stage 0 work = g u, stage 1 t = -g (t + alpha u), with
+// g = 1 / max(|d - shift|, cutoff) (signed) times 1 + m[i], clamped to [0.1, 10], when m
+// is given.
 void launch_precond(const double* diag, const double* u, double shift, double cutoff, double* work, double* t,
-                    int64_t n, int stage, double alpha, hipStream_t s);
+                    int64_t n, int stage, double alpha, hipStream_t s, const float* m = nullptr);
 void launch_symm_scale(double* x, const int64_t* d_k2i, const double* d_scale, int nk, hipStream_t s);
 
 }  // namespace chm

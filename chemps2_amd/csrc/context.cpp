@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <random>
 
 #include "../../include/chemps2_amd.h"
 #include "context.hpp"
@@ -1254,6 +1255,20 @@ static void jacobi_eig(int n, const std::vector<double>& A, int lda, std::vector
   }
 }
 
+// Per-element inputs of the learned preconditioner: log(dL) / 4 and log(dR) / 4 of the
+// element's block at the prepared site.
+void Ctx::build_eldims() {
+  if (eldims_site == site && eldims_gen == sec.gen && eldims.p) return;
+  std::vector<float> h((size_t)2 * V);
+  for (int k = 0; k < S.nk(); k++) {
+    const float a = (float)(std::log((double)S.dL[k]) / 4.0), b = (float)(std::log((double)S.dR[k]) / 4.0);
+    for (int64_t i = S.k2i[k]; i < S.k2i[k + 1]; i++) { h[2 * i] = a; h[2 * i + 1] = b; }
+  }
+  eldims.upload(h, stream);
+  eldims_site = site;
+  eldims_gen = sec.gen;
+}
+
 // Davidson.cpp:31-536 in eigenvalue mode with the reference's constants
 // (DAVIDSON_NUM_VEC = 32, DAVIDSON_NUM_VEC_KEEP = 3, DAVIDSON_PRECOND_CUTOFF = 1e-12;
 // Options.h:70-74). Vectors live on the device; the projected problem on the host.
@@ -1275,6 +1290,12 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
   DBuf<double> d_coef;
   d_ptrs.alloc(2 * MAXV + 8);
   d_coef.alloc(2 * MAXV + 8);
+  struct HarvestRec {
+    std::vector<int64_t> idx;
+    std::vector<float> feats, p;
+    std::vector<double> u;
+  };
+  std::vector<HarvestRec> recs;
   auto host_scalars = [&](int cnt) {
     std::vector<double> h(cnt);
     hip_check(hipMemcpyAsync(h.data(), scal.p, cnt * sizeof(double), hipMemcpyDeviceToHost, stream), "d2h");
@@ -1377,13 +1398,50 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
       continue;
     }
     if (rnorm <= rtol) break;
-    // Davidson.cpp:322-350
-    launch_precond(dg, u, eig[0], CUT, work, t, n, 0, 0.0, stream);
+    // Davidson.cpp:322-350, with the learned factor in precond_mode 1
+    const bool learned = precond_mode == 1 && mlp.ready;
+    const bool harvest = harvest_per_iter > 0 && !learned;
+    const float* mvals = nullptr;
+    if (learned || harvest) {
+      build_eldims();
+      featX.alloc((size_t)n * MLP_IN);
+      launch_features(dg, t, u, eldims.p, eig[0], std::sqrt((double)n) / rnorm, std::sqrt((double)n), n, featX.p, stream);
+      if (learned) {
+        mlp.forward(featX.p, n, stream);
+        mvals = mlp.out.p;
+      }
+    }
+    HarvestRec* rec = nullptr;
+    if (harvest) {
+      const int64_t S = std::min<int64_t>(harvest_per_iter, n), stride = n / S;
+      const int64_t off = (int64_t)((hv_counter++ * 2654435761ULL) % (uint64_t)stride);
+      recs.emplace_back();
+      rec = &recs.back();
+      rec->idx.resize(S);
+      for (int64_t q = 0; q < S; q++) rec->idx[q] = off + q * stride;
+      hv_idx.alloc(S);
+      hv_rows.alloc((size_t)S * MLP_IN);
+      copy_sync(stream, hv_idx.p, rec->idx.data(), S * sizeof(int64_t), hipMemcpyHostToDevice, "harvest indices");
+      launch_gather_rows(featX.p, hv_idx.p, S, hv_rows.p, stream);
+      rec->feats.resize((size_t)S * MLP_IN);
+      copy_sync(stream, rec->feats.data(), hv_rows.p, rec->feats.size() * sizeof(float), hipMemcpyDeviceToHost, "harvest rows");
+      std::vector<double> full(n);
+      copy_sync(stream, full.data(), u, n * sizeof(double), hipMemcpyDeviceToHost, "harvest u");
+      rec->u.resize(S);
+      for (int64_t q = 0; q < S; q++) rec->u[q] = full[rec->idx[q]];
+    }
+    launch_precond(dg, u, eig[0], CUT, work, t, n, 0, 0.0, stream, mvals);
     launch_dot(work, t, n, partial.p, scal.p, stream);
     launch_dot(work, u, n, partial.p, scal.p + 1, stream);
     std::vector<double> wr = host_scalars(2);
     const double alpha = -wr[0] / wr[1];
-    launch_precond(dg, u, eig[0], CUT, work, t, n, 1, alpha, stream);
+    launch_precond(dg, u, eig[0], CUT, work, t, n, 1, alpha, stream, mvals);
+    if (rec) {  // the plain preconditioned residual of the samples
+      std::vector<double> full(n);
+      copy_sync(stream, full.data(), t, n * sizeof(double), hipMemcpyDeviceToHost, "harvest t");
+      rec->p.resize(rec->idx.size());
+      for (size_t q = 0; q < rec->idx.size(); q++) rec->p[q] = (float)full[rec->idx[q]];
+    }
     if (num_vec == MAXV) {  // Davidson.cpp:352-412, 510-536
       hip_check(hipMemcpyAsync(E, u, n * sizeof(double), hipMemcpyDeviceToDevice, stream), "deflate");
       for (int c = 1; c < KEEP; c++) {
@@ -1422,6 +1480,30 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
     nmult++;
   }
   if (mv_timing) mv_flush();
+  if (!recs.empty()) {  // ideal corrections e = x* - u_iteration of the harvested samples
+    std::vector<double> xs(n);
+    copy_sync(stream, xs.data(), u, n * sizeof(double), hipMemcpyDeviceToHost, "harvest solution");
+    for (const HarvestRec& r : recs) {
+      if (r.p.size() != r.idx.size()) continue;
+      double ov = 0.0;
+      for (size_t q = 0; q < r.idx.size(); q++) ov += xs[r.idx[q]] * r.u[q];
+      const double sg = ov < 0.0 ? -1.0 : 1.0;
+      std::vector<double> e(r.idx.size());
+      double ss = 0.0;
+      for (size_t q = 0; q < r.idx.size(); q++) {
+        e[q] = sg * xs[r.idx[q]] - r.u[q];
+        ss += e[q] * e[q];
+      }
+      if (!(ss > 1e-28)) continue;
+      const float w = (float)(1.0 / ss);
+      hv_feats.insert(hv_feats.end(), r.feats.begin(), r.feats.end());
+      for (size_t q = 0; q < r.idx.size(); q++) {
+        hv_p.push_back(r.p[q]);
+        hv_e.push_back((float)e[q]);
+        hv_w.push_back(w);
+      }
+    }
+  }
   // 'C': copy back and convert to the program convention
   hip_check(hipMemcpyAsync(x, u, n * sizeof(double), hipMemcpyDeviceToDevice, stream), "solution");
   symm(x, false);
@@ -1764,6 +1846,145 @@ int chm_set_heff_form(chm_ctx* ctx, int form) {
   return guard(ctx, [&](Ctx& c) {
     if (form != CHM_FORM_FP64 && form != CHM_FORM_MIXED) throw Error{CHM_ERR_ARG, "unknown Heff form"};
     c.heff_form = form;
+  });
+}
+
+int chm_precond_mode(chm_ctx* ctx, int mode) {
+  return guard(ctx, [&](Ctx& c) {
+    if (mode != CHM_PRECOND_DIAGONAL && mode != CHM_PRECOND_LEARNED) throw Error{CHM_ERR_ARG, "unknown preconditioner"};
+    if (mode == CHM_PRECOND_LEARNED && !c.mlp.ready) throw Error{CHM_ERR_STATE, "set the network parameters first"};
+    c.precond_mode = mode;
+  });
+}
+
+int chm_mlp_nparams(int64_t* n) {
+  if (!n) return CHM_ERR_ARG;
+  *n = chm::Mlp::nparams();
+  return CHM_OK;
+}
+
+int chm_mlp_set(chm_ctx* ctx, const float* params, int64_t n) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!params || n != chm::Mlp::nparams()) throw Error{CHM_ERR_SIZE, "parameter count"};
+    (void)hipSetDevice(c.device);
+    c.mlp.ensure(1);
+    chm::copy_sync(c.stream, c.mlp.params.p, params, n * sizeof(float), hipMemcpyHostToDevice, "parameters");
+    c.mlp.reset_optimizer(c.stream);
+    c.mlp.ready = true;
+  });
+}
+
+int chm_mlp_get(chm_ctx* ctx, float* params, int64_t n) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!params || n != chm::Mlp::nparams()) throw Error{CHM_ERR_SIZE, "parameter count"};
+    if (!c.mlp.ready) throw Error{CHM_ERR_STATE, "no parameters"};
+    chm::copy_sync(c.stream, params, c.mlp.params.p, n * sizeof(float), hipMemcpyDeviceToHost, "parameters");
+  });
+}
+
+namespace {
+// Uploads n samples (X n x CHM_MLP_IN, p, e, w) into one device buffer.
+struct Samples {
+  chm::DBuf<float> buf;
+  float *X, *p, *e, *w;
+  void load(Ctx& c, const float* hX, const float* hp, const float* he, const float* hw, int64_t n) {
+    buf.alloc((size_t)n * (chm::MLP_IN + 3));
+    X = buf.p;
+    p = X + (size_t)n * chm::MLP_IN;
+    e = p + n;
+    w = e + n;
+    chm::copy_sync(c.stream, X, hX, (size_t)n * chm::MLP_IN * sizeof(float), hipMemcpyHostToDevice, "samples");
+    chm::copy_sync(c.stream, p, hp, n * sizeof(float), hipMemcpyHostToDevice, "samples");
+    chm::copy_sync(c.stream, e, he, n * sizeof(float), hipMemcpyHostToDevice, "samples");
+    chm::copy_sync(c.stream, w, hw, n * sizeof(float), hipMemcpyHostToDevice, "samples");
+  }
+};
+}  // namespace
+
+int chm_mlp_forward(chm_ctx* ctx, const float* X, int64_t n, float* out) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!X || !out || n < 1) throw Error{CHM_ERR_ARG, "bad arguments"};
+    if (!c.mlp.ready) throw Error{CHM_ERR_STATE, "no parameters"};
+    (void)hipSetDevice(c.device);
+    chm::DBuf<float> dX;
+    dX.alloc((size_t)n * chm::MLP_IN);
+    chm::copy_sync(c.stream, dX.p, X, (size_t)n * chm::MLP_IN * sizeof(float), hipMemcpyHostToDevice, "inputs");
+    c.mlp.forward(dX.p, n, c.stream);
+    chm::copy_sync(c.stream, out, c.mlp.out.p, n * sizeof(float), hipMemcpyDeviceToHost, "outputs");
+  });
+}
+
+int chm_mlp_loss_grad(chm_ctx* ctx, const float* X, const float* p, const float* e, const float* w, int64_t n,
+                      float* grad, double* loss) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!X || !p || !e || !w || !grad || !loss || n < 1) throw Error{CHM_ERR_ARG, "bad arguments"};
+    if (!c.mlp.ready) throw Error{CHM_ERR_STATE, "no parameters"};
+    (void)hipSetDevice(c.device);
+    Samples d;
+    d.load(c, X, p, e, w, n);
+    *loss = c.mlp.loss_grad(d.X, d.p, d.e, d.w, n, c.stream);
+    chm::copy_sync(c.stream, grad, c.mlp.grad.p, chm::Mlp::nparams() * sizeof(float), hipMemcpyDeviceToHost, "gradient");
+  });
+}
+
+int chm_mlp_train(chm_ctx* ctx, const float* X, const float* p, const float* e, const float* w, int64_t n, int batch,
+                  int epochs, double lr, uint64_t seed, double* loss_first, double* loss_last) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!X || !p || !e || !w || n < 1 || batch < 1 || epochs < 1 || !(lr > 0.0)) throw Error{CHM_ERR_ARG, "bad arguments"};
+    if (!c.mlp.ready) throw Error{CHM_ERR_STATE, "no parameters"};
+    (void)hipSetDevice(c.device);
+    Samples d;
+    d.load(c, X, p, e, w, n);
+    const int64_t B = std::min<int64_t>(batch, n);
+    chm::DBuf<float> bX, bs;
+    chm::DBuf<int64_t> bidx;
+    bX.alloc((size_t)B * chm::MLP_IN);
+    bs.alloc((size_t)3 * B);
+    bidx.alloc(B);
+    std::vector<int64_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::mt19937_64 rng(seed);
+    for (int ep = 0; ep < epochs; ep++) {
+      std::shuffle(perm.begin(), perm.end(), rng);
+      double tot = 0.0;
+      int64_t cnt = 0;
+      for (int64_t b0 = 0; b0 + B <= n; b0 += B) {
+        chm::copy_sync(c.stream, bidx.p, perm.data() + b0, B * sizeof(int64_t), hipMemcpyHostToDevice, "batch");
+        chm::launch_gather_rows(d.X, bidx.p, B, bX.p, c.stream);
+        chm::launch_gather(d.p, bidx.p, B, bs.p, c.stream);
+        chm::launch_gather(d.e, bidx.p, B, bs.p + B, c.stream);
+        chm::launch_gather(d.w, bidx.p, B, bs.p + 2 * B, c.stream);
+        tot += c.mlp.loss_grad(bX.p, bs.p, bs.p + B, bs.p + 2 * B, B, c.stream);
+        cnt++;
+        c.mlp.adam((float)lr, c.stream);
+      }
+      const double mean = cnt ? tot / cnt : 0.0;
+      if (ep == 0 && loss_first) *loss_first = mean;
+      if (loss_last) *loss_last = mean;
+    }
+    chm::hip_check(hipStreamSynchronize(c.stream), "training");
+  });
+}
+
+int chm_harvest(chm_ctx* ctx, int per_iteration) {
+  return guard(ctx, [&](Ctx& c) {
+    if (per_iteration < 0) throw Error{CHM_ERR_ARG, "negative sample count"};
+    c.harvest_per_iter = per_iteration;
+  });
+}
+
+int chm_harvest_take(chm_ctx* ctx, int64_t* n, float* X, float* p, float* e, float* w) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!n) throw Error{CHM_ERR_ARG, "null count"};
+    const int64_t have = (int64_t)c.hv_p.size();
+    if (!X || !p || !e || !w) { *n = have; return; }
+    if (*n < have) throw Error{CHM_ERR_SIZE, "arrays too small"};
+    std::copy(c.hv_feats.begin(), c.hv_feats.end(), X);
+    std::copy(c.hv_p.begin(), c.hv_p.end(), p);
+    std::copy(c.hv_e.begin(), c.hv_e.end(), e);
+    std::copy(c.hv_w.begin(), c.hv_w.end(), w);
+    *n = have;
+    c.hv_feats.clear(); c.hv_p.clear(); c.hv_e.clear(); c.hv_w.clear();
   });
 }
 

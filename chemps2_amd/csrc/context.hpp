@@ -5,7 +5,9 @@
 #include <tuple>
 
 #include "chm_internal.hpp"
+#include "dbuf.hpp"
 #include "heff_plan.hpp"
+#include "learned.hpp"
 #include "update_plan.hpp"
 
 namespace chm {
@@ -14,41 +16,6 @@ enum { KIND_L = 0, KIND_F0, KIND_F1, KIND_S0, KIND_S1, KIND_A, KIND_B, KIND_C, K
 
 struct OpShape {
   int kind, k, m, two_j, n_elec, n_irrep;
-};
-
-inline void hip_check(hipError_t e, const char* what) {
-  if (e != hipSuccess) throw Error{ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)};
-}
-
-// Owning device allocation.
-template <class T>
-struct DBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  DBuf() = default;
-  DBuf(const DBuf&) = delete;
-  DBuf& operator=(const DBuf&) = delete;
-  ~DBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  void alloc(size_t count) {
-    if (count <= n && p) return;
-    release();
-    if (count == 0) return;
-    hip_check(hipMalloc((void**)&p, count * sizeof(T)), "hipMalloc");
-    n = count;
-  }
-  // Synchronous copy: the host vector may be released right after the call.
-  void upload(const std::vector<T>& v, hipStream_t s) {
-    alloc(v.size());
-    if (!v.empty()) {
-      hip_check(hipStreamSynchronize(s), "upload");
-      hip_check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
-    }
-  }
 };
 
 // Page-locked host staging memory that only grows (hipHostMalloc); sub-allocated in
@@ -169,6 +136,20 @@ struct Ctx : HostState {
   bool mv_low = false;               // apply() reads the hi floats only
   double mixed_switch = 1e-5;
   int64_t n_low_mults = 0, n_full_mults = 0;
+  // Learned preconditioner (learned.hip): precond_mode 1 multiplies the diagonal
+  // preconditioner by (1 + m) with m from the MLP; harvest_per_iter > 0 records, at
+  // every preconditioner application, the features, the plain preconditioned residual p
+  // and (at convergence) the ideal correction e of that many elements.
+  Mlp mlp;
+  int precond_mode = 0;
+  int harvest_per_iter = 0;
+  DBuf<float> eldims, featX, hv_rows;
+  DBuf<int64_t> hv_idx;
+  int eldims_site = -1;
+  uint64_t eldims_gen = 0;
+  std::vector<float> hv_feats, hv_p, hv_e, hv_w;
+  uint64_t hv_counter = 0;
+  void build_eldims();
   bool use_chain_kernel = true;      // CHM_CHAIN_KERNEL=0: every chained term through k_tiles
   // CHM_CHAIN_MFMA=1: chained blocks up to 48 on the matrix cores (k_chain_mfma). Off by
   // default: at L = 32, D = 2000 it matches k_chain for blocks of 33-48 and is slower

@@ -786,12 +786,14 @@ __global__ __launch_bounds__(256) void k_lincomb_vecs(const double* const* __res
 // t = -(t + alpha u) / K, with K = diag - shift clamped to +-cutoff.
 __global__ __launch_bounds__(256) void k_precond(const double* __restrict__ diag, const double* __restrict__ u,
                                                  double shift, double cutoff, double* __restrict__ work,
-                                                 double* __restrict__ t, int64_t n, int stage, double alpha) {
+                                                 double* __restrict__ t, int64_t n, int stage, double alpha,
+                                                 const float* __restrict__ m) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
     const double d = diag[i] - shift;
     const double k = (fabs(d) > cutoff) ? d : cutoff;
-    if (stage == 0) work[i] = u[i] / k;
-    else t[i] = -(t[i] + alpha * u[i]) / k;
+    const double kk = m ? k / fmin(10.0, fmax(0.1, 1.0 + (double)m[i])) : k;  // learned factor kept in [0.1, 10]
+    if (stage == 0) work[i] = u[i] / kk;
+    else t[i] = -(t[i] + alpha * u[i]) / kk;
   }
 }
 
@@ -1104,8 +1106,8 @@ void launch_lincomb_vecs(const double* const* d_vecs, const double* d_coefs, int
 }
 
 void launch_precond(const double* diag, const double* u, double shift, double cutoff, double* work, double* t,
-                    int64_t n, int stage, double alpha, hipStream_t s) {
-  hipLaunchKernelGGL(k_precond, dim3(grid_for(n)), dim3(256), 0, s, diag, u, shift, cutoff, work, t, n, stage, alpha);
+                    int64_t n, int stage, double alpha, hipStream_t s, const float* m) {
+  hipLaunchKernelGGL(k_precond, dim3(grid_for(n)), dim3(256), 0, s, diag, u, shift, cutoff, work, t, n, stage, alpha, m);
 }
 
 void launch_fill_hash(double* p, int64_t n, uint64_t seed, double scale, hipStream_t s) {

@@ -164,6 +164,37 @@ int chm_heff_apply_f32(chm_ctx* ctx, const double* x, double* y, int on_device);
 /* Products chm_davidson made with the hi floats and with full precision since the
  * context was created. */
 int chm_mixed_stats(const chm_ctx* ctx, int64_t* low_products, int64_t* full_products);
+/* Learned preconditioner (no reference counterpart; the reference uses the diagonal of
+ * Heff, Davidson.cpp:322-350). A per-element MLP in FP32 (CHM_MLP_IN = 8 inputs: asinh(d
+ * - theta), r / rms(r), u sqrt(V), log(dL) / 4, log(dR) / 4, three zeros; three hidden
+ * layers of 128 with GELU; one output m) scales the Davidson correction:
+ * t = -(1 + m) (r + alpha u) / (d - theta), 1 + m clamped to [0.1, 10], alpha keeping t
+ * orthogonal to u. CHM_PRECOND_DIAGONAL (default) is the reference's preconditioner. */
+#define CHM_PRECOND_DIAGONAL 0
+#define CHM_PRECOND_LEARNED 1
+#define CHM_MLP_IN 8
+int chm_precond_mode(chm_ctx* ctx, int mode);
+/* Parameters: W1 (8 x 128), b1, W2 (128 x 128), b2, W3 (128 x 128), b3, W4 (128 x 1), b4,
+ * each W row-major (inputs x outputs); FP32. */
+int chm_mlp_nparams(int64_t* n);
+int chm_mlp_set(chm_ctx* ctx, const float* params, int64_t n);
+int chm_mlp_get(chm_ctx* ctx, float* params, int64_t n);
+/* out[i] = m of sample i (X: n x CHM_MLP_IN, host arrays). */
+int chm_mlp_forward(chm_ctx* ctx, const float* X, int64_t n, float* out);
+/* loss = mean_i w_i (p_i (1 + m_i) - e_i)^2 and its gradient (nparams floats). */
+int chm_mlp_loss_grad(chm_ctx* ctx, const float* X, const float* p, const float* e, const float* w, int64_t n,
+                      float* grad, double* loss);
+/* Adam (lr, 0.9, 0.999, 1e-8) over shuffled batches; mean batch loss of the first and last
+ * epochs. */
+int chm_mlp_train(chm_ctx* ctx, const float* X, const float* p, const float* e, const float* w, int64_t n, int batch,
+                  int epochs, double lr, uint64_t seed, double* loss_first, double* loss_last);
+/* Training samples from chm_davidson (diagonal preconditioner only): per_iteration
+ * elements of every preconditioner application, with p the plain preconditioned
+ * residual, e the corresponding element of (x* - u) for the converged x*, and w
+ * 1 / sum e^2 of that application. chm_harvest_take with null arrays returns the count;
+ * with arrays it copies and clears. */
+int chm_harvest(chm_ctx* ctx, int per_iteration);
+int chm_harvest_take(chm_ctx* ctx, int64_t* n, float* X, float* p, float* e, float* w);
 /* d = diag(Heff) (Heff::fillHeffDiag, Heff.cpp:250-315; nLower = 0). */
 int chm_heff_diag(chm_ctx* ctx, double* d, int on_device);
 /* Lowest eigenpair of Heff from the guess x (program convention, as Sobject storage
