@@ -20,10 +20,13 @@ every application of the sweep (sum of n_matvec * F_H over the summed device tim
 
 Both are also measured in the mixed form ("mixed"): the explicit sector matrix stored as
 float pairs, hi-only products in the Davidson until its residual is below 1e-5, hi + lo
-products after that. |dE| of the forms is measured twice: the Davidson eigenvalue at the
-benchmark site from one guess (rtol 1e-8), where only the product precision differs,
-and the energies after the sweep on the same seeds and schedule, where the two runs
-also follow different Davidson trajectories (rtol 1e-5) from the random start.
+products after that, and once more with the learned preconditioner on top of the mixed
+form ("mixed+learned", parameters from chemps2_amd/data/precond_mlp.npy, trained on
+other Hamiltonians). The sweeps at Davidson tolerance 1e-5 follow different Davidson
+trajectories from the random start, so |dE| between the forms is checked separately
+("same_site_check"): in the FP64 sweep, when the left half reaches sites 15-16, the
+Davidson of that step from the joined two-site tensor at tolerance 1e-8 in all three
+forms (outside the timed parts), after which the sweep goes on.
 
 One JSON line goes to stdout; progress goes to stderr.
 """
@@ -121,28 +124,76 @@ def timed_apply(ctx, x, y, steps, warmup, f32=False):
     return ms, [v / max(cnt, 1) for v in kms]
 
 
-def run_sweep(L, D, seed, irr, mx, rtol, form):
-    """Initialisation and the first left+right sweep of DMRG::Solve in one Heff form."""
+def same_site_check(sw, site, rtol, forms):
+    """Davidson at `site` from the joined two-site tensor in every form, on the operators
+    left by the sweep (same matrix for all forms up to their product precision)."""
     import chemps2_amd
 
-    name = "mixed" if form == chemps2_amd.FORM_MIXED else "fp64"
+    sw.join(site)
+    guess = sw.s_get()
+    out = {"site": site, "davidson_rtol": rtol}
+    for name, form, prm in forms:
+        sw.set_heff_form(form)
+        sw.precond_mode(chemps2_amd.PRECOND_DIAGONAL)
+        if prm is not None:
+            sw.mlp_set(prm)
+            sw.precond_mode(chemps2_amd.PRECOND_LEARNED)
+        sw.prepare(site)
+        g = guess.copy()
+        t0 = time.time()
+        e, nm = sw.davidson(g, rtol)
+        out[name] = {"energy": e, "products": nm, "seconds": round(time.time() - t0, 3)}
+    sw.set_heff_form(chemps2_amd.FORM_FP64)
+    sw.precond_mode(chemps2_amd.PRECOND_DIAGONAL)
+    for name in [f[0] for f in forms[1:]]:
+        out[name]["delta_E_vs_fp64"] = abs(out[name]["energy"] - out["fp64"]["energy"])
+        log(f"same-site check ({name}, site {site}, rtol {rtol:g}): |dE| vs FP64 = {out[name]['delta_E_vs_fp64']:.2e}, "
+            f"{out[name]['products']} products (FP64: {out['fp64']['products']})")
+    return out
+
+
+def run_sweep(L, D, seed, irr, mx, rtol, form, precond=None, check=None):
+    """Initialisation and the first left+right sweep of DMRG::Solve in one Heff form
+    (precond: parameters of the learned preconditioner, or None for the diagonal;
+    check: called with the context between the two halves, outside the timed parts)."""
+    import chemps2_amd
+
+    name = ("mixed" if form == chemps2_amd.FORM_MIXED else "fp64") + ("+learned" if precond is not None else "")
     sw = chemps2_amd.Context(0)
     sw.set_heff_form(form)
+    if precond is not None:
+        sw.mlp_set(precond)
+        sw.precond_mode(chemps2_amd.PRECOND_LEARNED)
     sw.set_problem(L, 7, irr, L, 0, 0, mx)
     t0 = time.time()
     sw.dmrg_init(D, seed + 1)
     t_init = time.time() - t0
     log(f"sweep ({name}): MPS and operators of every boundary built in {t_init:.1f}s")
+    checked = None
+    check_low = 0
     t0 = time.time()
-    e_left = sw.dmrg_sweep(False, D, rtol, 0.0, False)
-    t_left = time.time() - t0
+    if check is None:
+        e_left = sw.dmrg_sweep(False, D, rtol, 0.0, False)
+        t_left = time.time() - t0
+    else:
+        # the left half in two ranges around the check at site L/2 - 1: that site's Heff
+        # is the one the sweep's next step solves (fresh operators on both sides)
+        sw.dmrg_sweep_range(False, L - 2, L // 2, D, rtol, 0.0, False)
+        t_a = time.time() - t0
+        m0 = sw.mixed_stats()
+        checked = check(sw)
+        m1 = sw.mixed_stats()
+        check_low = m1[0] - m0[0]
+        t1 = time.time()
+        e_left = sw.dmrg_sweep_range(False, L // 2 - 1, 1, D, rtol, 0.0, False)
+        t_left = t_a + time.time() - t1
     log(f"sweep ({name}): left half (sites {L - 2}..1) in {t_left:.1f}s, energy {e_left:.10f}")
     t1 = time.time()
     e_right = sw.dmrg_sweep(True, D, rtol, 0.0, True)
     t_right = time.time() - t1
     phases, nmv = sw.dmrg_stats()
     hf, hms = sw.dmrg_heff_stats()
-    low, full = sw.mixed_stats()
+    low = sw.mixed_stats()[0] - check_low
     log(f"sweep ({name}): right half (sites 0..{L - 3}) in {t_right:.1f}s, energy {e_right:.10f}; {nmv} Heff "
         f"applications ({low} hi-only), {hf / max(hms, 1e-9) / 1e9:.2f} TFLOP/s over the sweep; phases {phases}")
     sw.close()
@@ -160,6 +211,7 @@ def run_sweep(L, D, seed, irr, mx, rtol, form):
         "energy_left_half": e_left,
         "energy_last_site": e_right,
         "phase_seconds": {k: round(v, 3) for k, v in phases.items()},
+        "check": checked,
     }
 
 
@@ -176,7 +228,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--sweep-rtol", type=float, default=1e-5)
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--dav-rtol", type=float, default=1e-8)
+    ap.add_argument("--check-rtol", type=float, default=1e-8)
+    ap.add_argument("--no-learned", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -228,12 +281,6 @@ def main():
         "traffic": read_traffic(),
     }
 
-    # same-site Davidson in both forms from one guess: |dE| from the product precision
-    guess = rng.normal(size=Vlen)
-    g = guess.copy()
-    e_dav64, nm64 = ctx.davidson(g, args.dav_rtol)
-    log(f"Davidson at the site (FP64, rtol {args.dav_rtol:g}): {e_dav64!r} after {nm64} products")
-
     mixed = None
     if not args.no_mixed:
         # the same site prepared in the mixed form (same bytes for the explicit matrix)
@@ -249,15 +296,7 @@ def main():
         log(f"mixed: prepare {t_prep_m:.1f}s; hi-only Heff x {ms32:.3f} ms ({F_H / ms32 / 1e9:.2f} TFLOP/s, explicit product "
             f"{kms32[1]:.4f} ms = {b32 / (kms32[1] * 1e-3) / 1e9:.0f} GB/s), max rel. difference to FP64 {d32:.2e}; "
             f"hi + lo {msr:.3f} ms, difference {dr:.2e}")
-        g = guess.copy()
-        e_davm, nmm = ctx.davidson(g, args.dav_rtol)
-        low, full = ctx.mixed_stats()
-        log(f"Davidson at the site (mixed): {e_davm!r} after {nmm} products ({low} hi-only), "
-            f"|dE| = {abs(e_davm - e_dav64):.2e}")
         mixed = {
-            "davidson_same_site": {"rtol": args.dav_rtol, "energy_fp64": e_dav64, "energy_mixed": e_davm,
-                                   "delta_E": abs(e_davm - e_dav64), "products_fp64": nm64, "products_mixed": nmm,
-                                   "products_mixed_hi_only": low},
             "heff_tflops_hi_only": round(F_H / (ms32 * 1e-3) / 1e12, 4),
             "ms_per_step_hi_only": round(ms32, 4),
             "explicit_product_gbs_hi_only": round(b32 / (kms32[1] * 1e-3) / 1e9, 1) if kms32[1] > 0 else None,
@@ -278,13 +317,30 @@ def main():
 
     sweep = None
     if not args.no_sweep:
-        sweep = run_sweep(L, D, seed, irr, mx, args.sweep_rtol, chemps2_amd.FORM_FP64)
+        wpath = os.path.join(ROOT, "chemps2_amd", "data", "precond_mlp.npy")
+        params = np.load(wpath, allow_pickle=False) if (not args.no_learned and os.path.exists(wpath)) else None
+        forms = [("fp64", chemps2_amd.FORM_FP64, None)]
         if mixed is not None:
+            forms.append(("mixed", chemps2_amd.FORM_MIXED, None))
+            if params is not None:
+                forms.append(("mixed+learned", chemps2_amd.FORM_MIXED, params))
+        check = (lambda c: same_site_check(c, site, args.check_rtol, forms)) if len(forms) > 1 else None
+        sweep = run_sweep(L, D, seed, irr, mx, args.sweep_rtol, chemps2_amd.FORM_FP64, check=check)
+        if mixed is not None:
+            mixed["same_site_check"] = sweep.pop("check")
             sm = run_sweep(L, D, seed, irr, mx, args.sweep_rtol, chemps2_amd.FORM_MIXED)
             sm["delta_E_vs_fp64"] = abs(sm["energy_last_site"] - sweep["energy_last_site"])
             sm["speedup_vs_fp64"] = round(sweep["seconds"] / sm["seconds"], 3)
             mixed["sweep"] = sm
             log(f"mixed sweep: {sm['seconds']:.1f}s vs FP64 {sweep['seconds']:.1f}s, |dE| = {sm['delta_E_vs_fp64']:.2e}")
+            if params is not None:
+                sl = run_sweep(L, D, seed, irr, mx, args.sweep_rtol, chemps2_amd.FORM_MIXED, precond=params)
+                sl["delta_E_vs_fp64"] = abs(sl["energy_last_site"] - sweep["energy_last_site"])
+                sl["speedup_vs_fp64"] = round(sweep["seconds"] / sl["seconds"], 3)
+                sl["network"] = "chemps2_amd/data/precond_mlp.npy (trained on L=24 D=1000 seeds 0-7)"
+                mixed["sweep_learned"] = sl
+                log(f"mixed+learned sweep: {sl['seconds']:.1f}s, {sl['heff_applications']} products "
+                    f"(diagonal: {sm['heff_applications']}), |dE| vs FP64 = {sl['delta_E_vs_fp64']:.2e}")
 
     out = {
         "metric": "Heff·x TFLOP/s + Davidson sweep time at L=32 D=2000; |ΔE| vs FP64 ≤1e-8 Ha",

@@ -24,7 +24,7 @@ EXPORTS = [
     "chm_join", "chm_s_get", "chm_s_set", "chm_split", "chm_get_problem", "chm_mps_layout_size", "chm_dmrg_init", "chm_dmrg_presolve", "chm_dmrg_solve", "chm_dmrg_sweep", "chm_dmrg_heff_stats", "chm_dmrg_stats",
     "chm_set_heff_form", "chm_heff_apply_f32", "chm_mixed_stats",
     "chm_precond_mode", "chm_mlp_nparams", "chm_mlp_set", "chm_mlp_get", "chm_mlp_forward", "chm_mlp_loss_grad",
-    "chm_mlp_train", "chm_harvest", "chm_harvest_take",
+    "chm_mlp_train", "chm_harvest", "chm_harvest_take", "chm_dmrg_sweep_range",
 ]
 PRECOND_DIAGONAL, PRECOND_LEARNED = 0, 1
 MLP_IN, MLP_H = 8, 128
@@ -89,6 +89,7 @@ def load():
         "chm_mlp_loss_grad": (i, [vp, vp, vp, vp, vp, i64, vp, C.POINTER(d)]),
         "chm_mlp_train": (i, [vp, vp, vp, vp, vp, i64, i, i, d, C.c_uint64, C.POINTER(d), C.POINTER(d)]),
         "chm_harvest": (i, [vp, i]),
+        "chm_dmrg_sweep_range": (i, [vp, i, i, i, i, d, d, i, C.POINTER(d)]),
         "chm_harvest_take": (i, [vp, C.POINTER(i64), vp, vp, vp, vp]),
         "chm_heff_diag": (i, [vp, vp, i]),
         "chm_davidson": (i, [vp, vp, i, d, C.POINTER(d), C.POINTER(i)]),
@@ -343,6 +344,13 @@ class Context:
         rt = np.array([s[4] for s in scheme], dtype=np.float64)
         e = C.c_double(0)
         self._check(self.lib.chm_dmrg_solve(self.h, len(scheme), D, ec, ms, nz, rt, C.byref(e)), "chm_dmrg_solve")
+        return e.value
+
+    def dmrg_sweep_range(self, moving_right, first, last, D, rtol=1e-5, noise=0.0, change=True):
+        """Sites first..last of a half sweep (continuing it when first is not the chain end)."""
+        e = C.c_double(0)
+        self._check(self.lib.chm_dmrg_sweep_range(self.h, int(moving_right), int(first), int(last), int(D), float(rtol),
+                                                  float(noise), int(change), C.byref(e)), "chm_dmrg_sweep_range")
         return e.value
 
     def dmrg_sweep(self, moving_right, D, rtol=1e-5, noise=0.0, change=True):

@@ -1489,9 +1489,18 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
       for (size_t q = 0; q < r.idx.size(); q++) ov += xs[r.idx[q]] * r.u[q];
       const double sg = ov < 0.0 ? -1.0 : 1.0;
       std::vector<double> e(r.idx.size());
-      double ss = 0.0;
+      double ss = 0.0, pe = 0.0, pp = 0.0;
       for (size_t q = 0; q < r.idx.size(); q++) {
         e[q] = sg * xs[r.idx[q]] - r.u[q];
+        pe += (double)r.p[q] * e[q];
+        pp += (double)r.p[q] * r.p[q];
+      }
+      // the correction's scale is free (the new vector is normalised): e is rescaled by
+      // the best scale of the plain p, and each application weighs 1 / sum e^2
+      const double sc = pp > 0.0 ? pe / pp : 0.0;
+      if (!(sc > 0.0)) continue;
+      for (size_t q = 0; q < r.idx.size(); q++) {
+        e[q] /= sc;
         ss += e[q] * e[q];
       }
       if (!(ss > 1e-28)) continue;
@@ -1746,6 +1755,16 @@ int chm_dmrg_sweep(chm_ctx* ctx, int moving_right, int D, double rtol, double no
     if (c.mps.empty() || c.fci.empty()) throw Error{CHM_ERR_STATE, "call chm_dmrg_init first"};
     (void)hipSetDevice(c.device);
     *energy = c.sweep(moving_right != 0, D, rtol, noise, change != 0);
+  });
+}
+
+int chm_dmrg_sweep_range(chm_ctx* ctx, int moving_right, int first, int last, int D, double rtol, double noise,
+                         int change, double* energy) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!energy || D < 1 || !(rtol > 0.0) || first < 0 || last < 0) throw Error{CHM_ERR_ARG, "bad sweep arguments"};
+    if (c.mps.empty() || c.fci.empty()) throw Error{CHM_ERR_STATE, "call chm_dmrg_init first"};
+    (void)hipSetDevice(c.device);
+    *energy = c.sweep(moving_right != 0, D, rtol, noise, change != 0, first, last);
   });
 }
 

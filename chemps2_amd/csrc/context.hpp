@@ -225,7 +225,11 @@ struct Ctx : HostState {
   int64_t n_matvec_total = 0;
   uint64_t noise_seed = 1;
   void join(int site);
-  double sweep(bool moving_right, int D, double rtol, double noise_prefactor, bool change);
+  // sites first..last in the sweep direction (-1: the end of the chain); a range that
+  // does not start at the chain end continues the sweep begun by the previous range
+  double sweep(bool moving_right, int D, double rtol, double noise_prefactor, bool change, int first = -1,
+               int last = -1);
+  double sweep_noise = 0.0;
   double split(int site, int D, bool moving_right, bool change);
   void mps_random(uint64_t seed);
   void presolve();

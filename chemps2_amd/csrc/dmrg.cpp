@@ -422,12 +422,21 @@ double Ctx::solve_site(int index, double rtol, double noise, int D, bool mr, boo
 }
 
 // DMRG::sweepleft / sweepright (DMRG.cpp:354-417): energy of the last site.
-double Ctx::sweep(bool mr, int D, double rtol, double noise_prefactor, bool change) {
+double Ctx::sweep(bool mr, int D, double rtol, double noise_prefactor, bool change, int first, int last) {
   const double t_start = now();
+  const int begin = mr ? 0 : L - 2, end = mr ? L - 3 : 1;
+  if (first < 0) first = begin;
+  if (last < 0) last = end;
+  if (mr ? !(0 <= first && first <= last && last <= L - 3) : !(L - 2 >= first && first >= last && last >= 1))
+    throw Error{ERR_ARG, "sweep range out of order or out of the chain"};
+  // a range that starts mid-sweep continues the sweep begun by an earlier range
+  if (first == begin) {
+    sweep_noise = std::fabs(noise_prefactor) * max_disc_last;
+    max_disc_last = 0.0;
+    last_min = 1e8;
+  }
   mv_timing = true;
-  const double nl = std::fabs(noise_prefactor) * max_disc_last;
-  max_disc_last = 0.0;
-  last_min = 1e8;
+  const double nl = sweep_noise;
   double E = 0.0;
   auto step = [&](int index) {
     const int next = mr ? (index + 1 <= L - 3 ? index + 1 : -1) : (index - 1 >= 1 ? index - 1 : -1);
@@ -441,9 +450,9 @@ double Ctx::sweep(bool mr, int D, double rtol, double noise_prefactor, bool chan
   };
   try {
     if (mr)
-      for (int index = 0; index < L - 2; index++) step(index);
+      for (int index = first; index <= last; index++) step(index);
     else
-      for (int index = L - 2; index > 0; index--) step(index);
+      for (int index = first; index >= last; index--) step(index);
   } catch (...) {
     mv_timing = false;
     mv_used = 0;

@@ -132,6 +132,13 @@ int chm_dmrg_solve(chm_ctx* ctx, int n, const int* D, const double* econv, const
  * and noise prefactor; change = 0 keeps the virtual dimensions (the reference's first
  * sweep). *energy: the energy at the last site. */
 int chm_dmrg_sweep(chm_ctx* ctx, int moving_right, int D, double rtol, double noise, int change, double* energy);
+/* Sites first..last of one sweep in its direction (moving left: L-2 >= first >= last >= 1;
+ * moving right: 0 <= first <= last <= L-3); a range that does not start at the chain end
+ * continues the sweep of the previous range (noise level and minimum energy). Splits
+ * DMRG::sweepleft / sweepright (DMRG.cpp:357-417) at a site, for callers that act between
+ * the steps (checkpoints, checks). */
+int chm_dmrg_sweep_range(chm_ctx* ctx, int moving_right, int first, int last, int D, double rtol, double noise,
+                         int change, double* energy);
 /* Heff applications inside the sweeps since chm_dmrg_init: the sum of F_H over them
  * and their device time in ms (HIP events around each application). No counterpart
  * in the reference: it is the sweep-averaged Heff.x rate of the benchmark. */
@@ -190,9 +197,10 @@ int chm_mlp_train(chm_ctx* ctx, const float* X, const float* p, const float* e, 
                   int epochs, double lr, uint64_t seed, double* loss_first, double* loss_last);
 /* Training samples from chm_davidson (diagonal preconditioner only): per_iteration
  * elements of every preconditioner application, with p the plain preconditioned
- * residual, e the corresponding element of (x* - u) for the converged x*, and w
- * 1 / sum e^2 of that application. chm_harvest_take with null arrays returns the count;
- * with arrays it copies and clears. */
+ * residual, e the corresponding element of (x* - u) for the converged x* divided by the
+ * application's best scale (p.e)/(p.p) (applications where it is not positive are
+ * dropped), and w 1 / sum e^2 of that application. chm_harvest_take with null arrays
+ * returns the count; with arrays it copies and clears. */
 int chm_harvest(chm_ctx* ctx, int per_iteration);
 int chm_harvest_take(chm_ctx* ctx, int64_t* n, float* X, float* p, float* e, float* w);
 /* d = diag(Heff) (Heff::fillHeffDiag, Heff.cpp:250-315; nLower = 0). */

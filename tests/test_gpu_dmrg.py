@@ -183,3 +183,26 @@ def test_hubbard_l12_against_fci():
                         [(200, 1e-8, 10, 0.05, 1e-5), (200, 1e-8, 10, 0.0, 1e-8)], 200)
     print(f"  gpu {e!r} fci {ef!r} |dE| {abs(e - ef):.2e} stats {stats}", flush=True)
     assert abs(e - ef) < 1e-8
+
+
+def test_sweep_range_continues_the_sweep():
+    # a left half run as two ranges (sites 6..4, then 3..1) equals the single half sweep
+    print("case sweep ranges L=8", flush=True)
+    irr, T, V = random_abinitio(8, 21, group=7)
+    mx = mx_table(T, V, 8)
+    energies = []
+    for split in (False, True):
+        c = chemps2_amd.Context(0)
+        c.set_problem(8, 7, irr, 8, 0, 0, mx)
+        c.dmrg_init(64, 5)
+        if split:
+            c.dmrg_sweep_range(False, 6, 4, 64, 1e-8, 0.0, False)
+            e_left = c.dmrg_sweep_range(False, 3, 1, 64, 1e-8, 0.0, False)
+        else:
+            e_left = c.dmrg_sweep(False, 64, 1e-8, 0.0, False)
+        e_right = c.dmrg_sweep(True, 64, 1e-8, 0.0, True)
+        with pytest.raises(chemps2_amd.ChmError):
+            c.dmrg_sweep_range(False, 3, 5, 64, 1e-8, 0.0, False)  # out of order
+        c.close()
+        energies.append((e_left, e_right))
+    assert abs(energies[0][0] - energies[1][0]) < 1e-12 and abs(energies[0][1] - energies[1][1]) < 1e-12

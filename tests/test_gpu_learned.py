@@ -118,3 +118,19 @@ def test_harvest_train_solve_unseen():
           f"E diagonal {ed!r} learned {el!r} FCI {ef!r}", flush=True)
     assert l1 < l0
     assert abs(ed - ef) < 1e-9 and abs(el - ef) < 1e-9
+
+
+def test_committed_network_unseen():
+    # the parameters shipped in chemps2_amd/data (trained by chemps2_amd.precond_train on
+    # L = 24, D = 1000 sweeps) on a Hamiltonian none of its samples came from
+    import os
+
+    print("case committed network", flush=True)
+    path = os.path.join(os.path.dirname(chemps2_amd.__file__), "data", "precond_mlp.npy")
+    params = np.load(path, allow_pickle=False)
+    irr, T, V, mx = d2h_problem(77)
+    ef = fci_energy(8, irr, 8, 0, 0, T, V)
+    ed, nd, _ = run(irr, mx, 8, 8)
+    el, nl, _ = run(irr, mx, 8, 8, params=params)
+    print(f"  products diagonal {nd}, learned {nl}; |E - FCI| {abs(ed - ef):.1e} / {abs(el - ef):.1e}", flush=True)
+    assert abs(ed - ef) < 1e-9 and abs(el - ef) < 1e-9
