@@ -374,8 +374,10 @@ constexpr int SVD_JACOBI_MAXN = 256;  // largest n of the batched Jacobi kernel
 void launch_copies(const DevCopy* items, int n, hipStream_t s);
 void launch_svd_jacobi(const DevSvd* items, int n, hipStream_t s);
 void launch_add_noise(double* x, int64_t n, uint64_t seed, double level, hipStream_t s);
+// mfma: k_tiles_mfma (matrix cores, prefetched double-buffered staging), else k_tiles
 void launch_tiles(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const double* xbase, double* ybase,
-                  hipStream_t s);
+                  hipStream_t s,
+                  bool mfma = true);
 void launch_combos(const DevCombo* d_combos, const DevSrc* d_src, int ncombo, const int64_t* d_chunk_start,
                    int64_t nchunks, hipStream_t s);
 void launch_diag(const DevDiagBlock* blocks, const DevDiag* terms, int nblocks, hipStream_t s);

@@ -753,12 +753,12 @@ void Ctx::prepare(int st) {
 void Ctx::apply(const double* xp, double* yp) {
   if (site < 0) throw Error{ERR_STATE, "no prepared site"};
   if (timing) hip_check(hipEventRecord(ev[0], stream), "event");
-  launch_tiles(tiles1.p, ops1.p, ntiles1, xp, yp, stream);
+  launch_tiles(tiles1.p, ops1.p, ntiles1, xp, yp, stream, use_tiles_mfma);
   if (timing) hip_check(hipEventRecord(ev[1], stream), "event");
   launch_bspmv(bitems.p, nbitems, breds.p, nbreds, bpart.p, Mvals.p, M_nnz, M_split ? (mv_low ? 1 : 2) : 0, Mcol.p, xp,
                yp, stream);
   if (timing) hip_check(hipEventRecord(ev[2], stream), "event");
-  launch_tiles(tiles2.p, ops2.p, ntiles2, xp, yp, stream);
+  launch_tiles(tiles2.p, ops2.p, ntiles2, xp, yp, stream, use_tiles_mfma);
   launch_chain_mfma(cblocks_m.p, cblocks_m_offs, cgroups.p, cterms.p, xp, yp, cpart.p, stream);
   launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, cpart.p, chain_P, false, stream);
   launch_chain(cblocks_l.p, ncblocks_l, cgroups.p, cterms.p, xp, yp, cpart.p, chain_P_l, true, stream);
@@ -1091,9 +1091,9 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
   if (nred) hip_check(hipMemcpyAsync(u_rd.p, rd, nred * sizeof(DevPRed), hipMemcpyHostToDevice, stream), "reductions");
   const double t_up = now_s();
   const int P = std::min(OPS_P, (Pneed.load() + 31) / 32 * 32);
-  launch_tiles(u_dt1.p, u_d1.p, (int)nt1, nullptr, nullptr, stream);
+  launch_tiles(u_dt1.p, u_d1.p, (int)nt1, nullptr, nullptr, stream, use_tiles_mfma);
   launch_ops(u_it1.p, (int)ni1, u_d1.p, P, stream);
-  launch_tiles(u_dt2.p, u_d2.p, (int)nt2, nullptr, nullptr, stream);
+  launch_tiles(u_dt2.p, u_d2.p, (int)nt2, nullptr, nullptr, stream, use_tiles_mfma);
   launch_ops(u_it2.p, (int)ni2, u_d2.p, P, stream);
   launch_part_reduce(u_rd.p, (int)nred, u_part.p, stream);
   hip_check(hipGetLastError(), "update kernels");
@@ -1564,6 +1564,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_SVD_JACOBI_MAXN")) ctx->c.svd_jacobi_maxn = std::atoi(e);
   if (const char* e = std::getenv("CHM_CHAIN_KERNEL")) ctx->c.use_chain_kernel = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_CHAIN_MFMA")) ctx->c.use_chain_mfma = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CHM_TILES_MFMA")) ctx->c.use_tiles_mfma = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_OPS_KERNEL")) ctx->c.use_ops_kernel = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;

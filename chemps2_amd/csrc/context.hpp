@@ -156,6 +156,7 @@ struct Ctx : HostState {
   // below (its operand loads from global memory are not coalesced).
   bool use_chain_mfma = false;
   DBuf<DevCBlock> cblocks_m;         // k_chain_mfma items sorted by tile count
+  bool use_tiles_mfma = true;        // CHM_TILES_MFMA=0: GEMM tiles in k_tiles (vector FMA) instead of k_tiles_mfma
   int cblocks_m_offs[CHAIN_MFMA_NT + 1] = {};
   DBuf<DevCBlock> cblocks;
   DBuf<DevCGroup> cgroups;

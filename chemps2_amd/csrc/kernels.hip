@@ -123,6 +123,123 @@ __global__ __launch_bounds__(NT) void k_tiles(const DevTile* __restrict__ tiles,
     }
 }
 
+// The same tiles on the matrix cores: each of the four wavefronts owns a 32 x 32 quarter
+// of the 64 x 64 tile as 2 x 2 v_mfma_f64_16x16x4f64 accumulators (lane l holds rows
+// 4 r + l / 16 of column l % 16 of each 16 x 16 block, register r). The K chunks of all
+// products of the tile form one sequence: the next chunk's operands are loaded into
+// registers while the current one computes from LDS, and the two LDS buffers alternate,
+// so one barrier per chunk suffices. axpy products (kind 1) are added first.
+typedef double chm_t4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(NT) void k_tiles_mfma(const DevTile* __restrict__ tiles, const DevGemmOp* __restrict__ ops,
+                                                   const double* __restrict__ xbase, double* __restrict__ ybase) {
+  __shared__ double As[2][TK][TM + 1];
+  __shared__ double Bs[2][TK][TN + 1];
+  const DevTile t = tiles[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, lr = lane & 15, lg = lane >> 4;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;
+  const int mrem = min(TM, t.m - t.m0), nrem = min(TN, t.n - t.n0);
+  chm_t4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = chm_t4{0.0, 0.0, 0.0, 0.0};
+  // axpy products: C += alpha * op(A) on this lane's accumulator elements
+  for (int o = 0; o < t.nop; o++) {
+    const DevGemmOp op = ops[t.op0 + o];
+    if (op.kind != 1) continue;
+    const double* A = rel(op.A, op.flags & 1, xbase);
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int rr = wm + 16 * i + 4 * r + lg, cc = wn + 16 * j + lr;
+          if (rr < mrem && cc < nrem)
+            acc[i][j][r] += op.alpha * (op.ta ? A[(size_t)(t.n0 + cc) + (size_t)op.lda * (t.m0 + rr)]
+                                              : A[(size_t)(t.m0 + rr) + (size_t)op.lda * (t.n0 + cc)]);
+        }
+  }
+  // the GEMM chunks: (product o, k0) in order
+  int o = 0, k0 = 0;
+  auto skip = [&]() {  // advance o to the next GEMM product with work left
+    while (o < t.nop && (ops[t.op0 + o].kind == 1 || k0 >= ops[t.op0 + o].k)) { o++; k0 = 0; }
+  };
+  skip();
+  double ra[4], rb[4];
+  auto fetch = [&]() {  // this thread's share of chunk (o, k0), alpha folded into A
+    const DevGemmOp op = ops[t.op0 + o];
+    const double* A = rel(op.A, op.flags & 1, xbase);
+    const double* B = rel(op.B, (op.flags >> 1) & 1, xbase);
+    const int krem = min(TK, op.k - k0);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int r, kk;
+      if (op.ta == 0) { r = tid & 63; kk = (tid >> 6) + 4 * q; } else { kk = tid & 15; r = (tid >> 4) + 16 * q; }
+      double v = 0.0;
+      if (r < mrem && kk < krem)
+        v = op.ta ? A[(size_t)(k0 + kk) + (size_t)op.lda * (t.m0 + r)] : A[(size_t)(t.m0 + r) + (size_t)op.lda * (k0 + kk)];
+      ra[q] = op.alpha * v;
+      int c, kb;
+      if (op.tb == 0) { kb = tid & 15; c = (tid >> 4) + 16 * q; } else { c = tid & 63; kb = (tid >> 6) + 4 * q; }
+      double w = 0.0;
+      if (c < nrem && kb < krem)
+        w = op.tb ? B[(size_t)(t.n0 + c) + (size_t)op.ldb * (k0 + kb)] : B[(size_t)(k0 + kb) + (size_t)op.ldb * (t.n0 + c)];
+      rb[q] = w;
+    }
+  };
+  auto put = [&](int buf, int ta, int tb) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (ta == 0) As[buf][(tid >> 6) + 4 * q][tid & 63] = ra[q];
+      else As[buf][tid & 15][(tid >> 4) + 16 * q] = ra[q];
+      if (tb == 0) Bs[buf][tid & 15][(tid >> 4) + 16 * q] = rb[q];
+      else Bs[buf][(tid >> 6) + 4 * q][tid & 63] = rb[q];
+    }
+  };
+  int buf = 0;
+  if (o < t.nop) fetch();
+  while (o < t.nop) {
+    const DevGemmOp op = ops[t.op0 + o];
+    put(buf, op.ta, op.tb);
+    __syncthreads();
+    k0 += TK;
+    skip();
+    if (o < t.nop) fetch();  // in flight while this chunk computes
+#pragma unroll
+    for (int ks = 0; ks < TK / 4; ks++) {
+      const int kk = 4 * ks + lg;
+      double a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++) a[i] = As[buf][kk][wm + 16 * i + lr];
+#pragma unroll
+      for (int j = 0; j < 2; j++) b[j] = Bs[buf][kk][wn + 16 * j + lr];
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    buf ^= 1;
+  }
+  double* C = (t.flags & 1) ? ybase + (intptr_t)t.C : t.C;
+  const bool add = (t.flags & 2) != 0;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int rr = wm + 16 * i + 4 * r + lg, cc = wn + 16 * j + lr;
+        if (rr < mrem && cc < nrem) {
+          double* p = C + (size_t)(t.m0 + rr) + (size_t)t.ldc * (t.n0 + cc);
+          *p = add ? *p + acc[i][j][r] : acc[i][j][r];
+        }
+      }
+}
+
 // Assembles the explicit sub-block of every (kappa, kappa') pair:
 // M[(l + dL r), (l' + dLi r')] = sum_t alpha_t op(A_t)[l, l'] op(B_t)[r', r].
 // SPLIT stores each value v as two floats, hi = fl32(v) in the first nnz floats of the
@@ -989,9 +1106,10 @@ inline unsigned grid_for(int64_t n) {
 }  // namespace
 
 void launch_tiles(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const double* xbase, double* ybase,
-                  hipStream_t s) {
+                  hipStream_t s, bool mfma) {
   if (ntiles <= 0) return;
-  hipLaunchKernelGGL(k_tiles, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase);
+  if (mfma) hipLaunchKernelGGL(k_tiles_mfma, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase);
+  else hipLaunchKernelGGL(k_tiles, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase);
 }
 
 void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs, double* M, int64_t nnz, bool split,

@@ -46,9 +46,12 @@ def test_update_operators(ctx, oracle_lib, index, mr):
 
 
 @pytest.mark.parametrize("index,mr", [(3, True), (3, False)])
-def test_update_operators_tile_path(oracle_lib, monkeypatch, index, mr):
-    # every update product through the GEMM tiles (CHM_OPS_KERNEL=0) agrees as well
+@pytest.mark.parametrize("tiles_mfma", ["1", "0"])
+def test_update_operators_tile_path(oracle_lib, monkeypatch, index, mr, tiles_mfma):
+    # every update product through the GEMM tiles (CHM_OPS_KERNEL=0), on the matrix
+    # cores or with vector FMA, agrees as well
     monkeypatch.setenv("CHM_OPS_KERNEL", "0")
+    monkeypatch.setenv("CHM_TILES_MFMA", tiles_mfma)
     c = chemps2_amd.Context(0)
     try:
         test_update_operators(c, oracle_lib, index, mr)
