@@ -586,8 +586,46 @@ void Ctx::prepare(int st) {
   struct WRec { int64_t off; int rows, cols; };
   std::vector<WRec> wrec(chain.size(), {-1, 0, 0});
   auto rel_x = [&](int k) { return (const double*)(intptr_t)S.k2i[k]; };
+  // The intermediates W of one batch of output blocks share the scratch buffer, so that
+  // it stays within a cap (one block larger than that is a batch of its own); apply()
+  // runs phase 1 and phase 2 batch after batch. The cap is tile_w_cap doubles if set,
+  // else a quarter of the free memory (the current buffer counted as free), at least
+  // 4 GiB.
+  int64_t TILE_W_CAP = tile_w_cap;
+  if (TILE_W_CAP <= 0) {
+    size_t freeb = 0, totb = 0;
+    hip_check(hipMemGetInfo(&freeb, &totb), "hipMemGetInfo");
+    TILE_W_CAP = std::max<int64_t>((int64_t)1 << 29, (int64_t)((freeb + 8.0 * W.n) / 4 / 8));
+  }
+  auto wneed = [&](const Term& t) -> int64_t {
+    if (t.tl == 'I' || t.tr == 'I') return 0;
+    const int dL = dLof(t.out), dR = dRof(t.out), dLi = dLof(t.in), dRi = dRof(t.in);
+    return t.xr_first ? ((int64_t)dLi * dR + 7) / 8 * 8 : ((int64_t)dL * dRi + 7) / 8 * 8;
+  };
+  std::vector<size_t> bstart;  // first chain index of every batch
+  int64_t Wmax = 0;
+  {
+    int64_t cur = 0;
+    for (size_t q0 = 0; q0 < chain.size();) {
+      size_t q1 = q0;
+      int64_t need = 0;
+      while (q1 < chain.size() && chain[q1]->out == chain[q0]->out) need += wneed(*chain[q1++]);
+      if (bstart.empty() || (cur > 0 && cur + need > TILE_W_CAP)) { bstart.push_back(q0); cur = 0; }
+      cur += need;
+      Wmax = std::max(Wmax, cur);
+      q0 = q1;
+    }
+  }
+  tb1.assign(1, 0);
+  tb2.assign(1, 0);
+  size_t nextb = 1;
   // phase 1: one group per two-operand term
   for (size_t q = 0; q < chain.size(); q++) {
+    if (nextb < bstart.size() && q == bstart[nextb]) {
+      tb1.push_back((int)t1.size());
+      Wsize = 0;
+      nextb++;
+    }
     const Term& t = *chain[q];
     if (t.tl == 'I' || t.tr == 'I') continue;
     const int dL = dLof(t.out), dR = dRof(t.out), dLi = dLof(t.in), dRi = dRof(t.in);
@@ -616,10 +654,16 @@ void Ctx::prepare(int st) {
         t1.push_back(tl);
       }
   }
-  W.alloc(std::max<int64_t>(Wsize, 1));
+  tb1.push_back((int)t1.size());
+  W.alloc(std::max<int64_t>(Wmax, 1));
   for (auto& tl : t1) tl.C = W.p + (intptr_t)tl.C;
   // phase 2: one group per output block, added to the explicit product
+  nextb = 1;
   for (size_t q0 = 0; q0 < chain.size();) {
+    if (nextb < bstart.size() && q0 == bstart[nextb]) {
+      tb2.push_back((int)t2.size());
+      nextb++;
+    }
     const int k = chain[q0]->out;
     size_t q1 = q0;
     while (q1 < chain.size() && chain[q1]->out == k) q1++;
@@ -661,6 +705,7 @@ void Ctx::prepare(int st) {
       }
     q0 = q1;
   }
+  tb2.push_back((int)t2.size());
   ops1.upload(o1, stream);
   ops2.upload(o2, stream);
   tiles1.upload(t1, stream);
@@ -742,7 +787,7 @@ void Ctx::prepare(int st) {
   stats[2] = plan->combo_size;
   term_pool = std::move(plan->chunks);  // their capacity serves the next site
   reap(std::move(plan));
-  stats[3] = Wsize;
+  stats[3] = Wmax;
   stats[4] = ntiles1;
   stats[5] = ntiles2;
   stats[8] = S.nk();
@@ -753,12 +798,14 @@ void Ctx::prepare(int st) {
 void Ctx::apply(const double* xp, double* yp) {
   if (site < 0) throw Error{ERR_STATE, "no prepared site"};
   if (timing) hip_check(hipEventRecord(ev[0], stream), "event");
-  launch_tiles(tiles1.p, ops1.p, ntiles1, xp, yp, stream, use_tiles_mfma);
-  if (timing) hip_check(hipEventRecord(ev[1], stream), "event");
   launch_bspmv(bitems.p, nbitems, breds.p, nbreds, bpart.p, Mvals.p, M_nnz, M_split ? (mv_low ? 1 : 2) : 0, Mcol.p, xp,
                yp, stream);
+  if (timing) hip_check(hipEventRecord(ev[1], stream), "event");
+  for (size_t b = 0; b + 1 < tb1.size(); b++) {  // GEMM tiles, batch by batch (W is shared)
+    launch_tiles(tiles1.p + tb1[b], ops1.p, tb1[b + 1] - tb1[b], xp, yp, stream, use_tiles_mfma);
+    launch_tiles(tiles2.p + tb2[b], ops2.p, tb2[b + 1] - tb2[b], xp, yp, stream, use_tiles_mfma);
+  }
   if (timing) hip_check(hipEventRecord(ev[2], stream), "event");
-  launch_tiles(tiles2.p, ops2.p, ntiles2, xp, yp, stream, use_tiles_mfma);
   launch_chain_mfma(cblocks_m.p, cblocks_m_offs, cgroups.p, cterms.p, xp, yp, cpart.p, stream);
   launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, cpart.p, chain_P, false, stream);
   launch_chain(cblocks_l.p, ncblocks_l, cgroups.p, cterms.p, xp, yp, cpart.p, chain_P_l, true, stream);
@@ -767,10 +814,11 @@ void Ctx::apply(const double* xp, double* yp) {
   hip_check(hipGetLastError(), "Heff kernels");
   if (timing) {
     hip_check(hipEventSynchronize(ev[3]), "event sync");
+    static const int slot[3] = {1, 0, 2};  // explicit product, GEMM tiles, chained kernels
     for (int q = 0; q < 3; q++) {
       float a = 0.f;
       hip_check(hipEventElapsedTime(&a, ev[q], ev[q + 1]), "elapsed");
-      t_ms[q] += a;
+      t_ms[slot[q]] += a;
     }
     t_count++;
   }
@@ -1565,6 +1613,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_CHAIN_KERNEL")) ctx->c.use_chain_kernel = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_CHAIN_MFMA")) ctx->c.use_chain_mfma = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_TILES_MFMA")) ctx->c.use_tiles_mfma = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CHM_TILE_W_CAP")) ctx->c.tile_w_cap = std::max(1LL, std::atoll(e));
   if (const char* e = std::getenv("CHM_OPS_KERNEL")) ctx->c.use_ops_kernel = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;

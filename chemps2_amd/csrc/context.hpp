@@ -113,6 +113,8 @@ struct Ctx : HostState {
   int64_t V = 0;
   DBuf<double> combo_arena, W, dx, dy, partial, scal;
   DBuf<DevTile> tiles1, tiles2;
+  std::vector<int> tb1, tb2;  // tile ranges of the batches that share the W scratch
+  int64_t tile_w_cap = 0;  // doubles of W scratch per batch (CHM_TILE_W_CAP; 0: a quarter of free memory)
   DBuf<DevGemmOp> ops1, ops2;
   int ntiles1 = 0, ntiles2 = 0;
   DBuf<DevDiagBlock> dblocks;
@@ -255,6 +257,16 @@ struct Ctx : HostState {
   void prepare(int site);
   const double* resolve(const Ref& r) const;
   void apply(const double* dxp, double* dyp);
+  // frees the prepared site's explicit matrix and tile scratch; the site must be
+  // prepared again before the next product
+  void release_site_scratch() {
+    Mvals.release();
+    Mcol.release();
+    bpart.release();
+    W.release();
+    M_nnz = 0;
+    site = -1;
+  }
   void diag(double* d);
   double davidson(double* x, double rtol, int* n_matvec);
   void symm(double* x, bool to_symm);

@@ -401,6 +401,13 @@ double Ctx::solve_site(int index, double rtol, double noise, int D, bool mr, boo
   n_matvec_total += nm;
   double t3 = now();
   tm[2] += t3 - t2;
+  {
+    // under memory pressure the explicit matrix and the GEMM-tile scratch of this site go
+    // back to the pool before Split and the operator update allocate (the next prepare
+    // builds its own)
+    size_t fb = 0, tb = 0;
+    if (hipMemGetInfo(&fb, &tb) == hipSuccess && fb < tb / 4) release_site_scratch();
+  }
   if (verbose)
     std::fprintf(stderr, "  site %d: join %.3f s, prepare %.3f s, davidson %.3f s (%d products, %.3f s in Heff kernels)\n", index,
                  t1 - t0, t2 - t1, t3 - t2, nm, (mv_ms - mv0) * 1e-3);

@@ -221,8 +221,9 @@ int chm_plan_stats(const chm_ctx* ctx, int64_t* stats);
 /* The context's HIP stream (hipStream_t) for callers that time or order work on it. */
 int chm_stream(const chm_ctx* ctx, void** stream);
 /* Timing of chm_heff_apply with HIP events on the context stream, accumulated per
- * kernel: ms[0] chained-GEMM phase 1, ms[1] explicit block product, ms[2] chained-GEMM
- * phase 2; count = number of applications timed. */
+ * kernel group: ms[0] GEMM tiles of large chained blocks (both phases), ms[1] explicit
+ * block product, ms[2] output-stationary chained kernels; count = number of
+ * applications timed. */
 int chm_timing_enable(chm_ctx* ctx, int enable);
 int chm_timing_read(const chm_ctx* ctx, double* ms, int64_t* count);
 

@@ -89,15 +89,18 @@ def test_davidson(ctx, oracle_lib, index):
     assert abs(np.linalg.norm(xs) - 1.0) <= 1e-10
 
 
-@pytest.mark.parametrize("explicit_max,chain_kernel,tiles_mfma", [("0", "1", "1"), ("0", "0", "1"), ("0", "0", "0"),
-                                                                  ("64", "1", "1"), ("100000000", "1", "1")])
-def test_explicit_threshold_paths(oracle_lib, explicit_max, chain_kernel, tiles_mfma, monkeypatch):
+@pytest.mark.parametrize("explicit_max,chain_kernel,tiles_mfma,w_cap", [
+    ("0", "1", "1", None), ("0", "0", "1", None), ("0", "0", "0", None), ("0", "0", "1", "20000"),
+    ("64", "1", "1", None), ("100000000", "1", "1", None)])
+def test_explicit_threshold_paths(oracle_lib, explicit_max, chain_kernel, tiles_mfma, w_cap, monkeypatch):
     # all chained (output-stationary kernel for blocks up to 64, GEMM tiles above; or
-    # GEMM tiles only, on the matrix cores or with vector FMA), a mix, and everything
-    # assembled explicitly
+    # GEMM tiles only, on the matrix cores or with vector FMA, in one batch or in many
+    # batches sharing a small intermediate buffer), a mix, and everything explicit
     monkeypatch.setenv("CHM_EXPLICIT_MAX", explicit_max)
     monkeypatch.setenv("CHM_CHAIN_KERNEL", chain_kernel)
     monkeypatch.setenv("CHM_TILES_MFMA", tiles_mfma)
+    if w_cap:
+        monkeypatch.setenv("CHM_TILE_W_CAP", w_cap)
     c = chemps2_amd.Context(0)
     T, V = hubbard(10, U=4.0)
     o = oracle_lib.OracleDMRG(10, 0, [0] * 10, 10, 0, 0, T, V)
