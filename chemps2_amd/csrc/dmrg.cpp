@@ -270,12 +270,24 @@ double Ctx::split(int st, int D, bool mr, bool change) {
     launch_svd_jacobi(d_jobs.p, (int)jobs.size(), stream);
     hip_check(hipGetLastError(), "split SVD kernel");
     int ci = 0;
+    DBuf<double> jres;
+    DBuf<rocblas_int> jsw;
+    if (svd_large_gesvdj) {
+      jres.alloc(std::max<size_t>(cs.size(), 1));
+      jsw.alloc(std::max<size_t>(cs.size(), 1));
+    }
     for (auto& c : cs) {
       if (c.cd > 0 && !c.jac) {
-        const rocblas_status st2 = rocsolver_dgesvd(hnd, rocblas_svect_singular, rocblas_svect_singular, c.dLt, c.dRt,
-                                                   M.p + c.moff, c.dLt, Sg.p + c.soff, U.p + c.uoff, c.dLt,
-                                                   VT.p + c.voff, c.cd, E.p + c.soff, rocblas_outofplace, info.p + ci);
-        if (st2 != rocblas_status_success) throw Error{ERR_HIP, "rocsolver_dgesvd failed"};
+        rocblas_status st2;
+        if (svd_large_gesvdj)  // Jacobi eigenvectors of the Gram matrix, then QR (rocSOLVER's gesvdj)
+          st2 = rocsolver_dgesvdj(hnd, rocblas_svect_singular, rocblas_svect_singular, c.dLt, c.dRt, M.p + c.moff, c.dLt,
+                                  0.0, jres.p + ci, 100, jsw.p + ci, Sg.p + c.soff, U.p + c.uoff, c.dLt, VT.p + c.voff,
+                                  c.cd, info.p + ci);
+        else
+          st2 = rocsolver_dgesvd(hnd, rocblas_svect_singular, rocblas_svect_singular, c.dLt, c.dRt, M.p + c.moff, c.dLt,
+                                 Sg.p + c.soff, U.p + c.uoff, c.dLt, VT.p + c.voff, c.cd, E.p + c.soff,
+                                 rocblas_outofplace, info.p + ci);
+        if (st2 != rocblas_status_success) throw Error{ERR_HIP, "rocSOLVER SVD failed"};
       }
       ci++;
     }

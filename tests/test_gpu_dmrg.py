@@ -142,8 +142,9 @@ def test_reference_test3_ch4_on_gpu():
 
 @pytest.mark.parametrize("mr,D", [(True, 10), (False, 1000)])
 def test_split_svd_paths_agree(oracle_lib, monkeypatch, mr, D):
-    # batched Jacobi kernel (default) and rocSOLVER (CHM_SVD_JACOBI_MAXN=0) give the
-    # same truncation, discarded weight and re-joined two-site object
+    # batched Jacobi kernel (default), rocSOLVER's dgesvd (CHM_SVD_JACOBI_MAXN=0) and its
+    # gesvdj (CHM_SVD_GESVDJ=1) give the same truncation, discarded weight and re-joined
+    # two-site object
     print(f"case svd paths mr {mr} D {D}", flush=True)
     irr, T, V = random_abinitio(8, 5, group=7)
     o = oracle_lib.OracleDMRG(8, 7, irr, 8, 0, 0, T, V)
@@ -152,8 +153,9 @@ def test_split_svd_paths_agree(oracle_lib, monkeypatch, mr, D):
     o.site(3)
     s_prog = symm_to_prog(o, o.davidson(rtol=1e-8)[1])
     out = []
-    for maxn in ("64", "0"):
+    for maxn, gesvdj in (("64", "0"), ("0", "0"), ("0", "1")):
         monkeypatch.setenv("CHM_SVD_JACOBI_MAXN", maxn)
+        monkeypatch.setenv("CHM_SVD_GESVDJ", gesvdj)
         c = chemps2_amd.Context(0)
         load_state(c, o, 20)
         c.join(3)
@@ -163,10 +165,11 @@ def test_split_svd_paths_agree(oracle_lib, monkeypatch, mr, D):
         c.join(3)
         out.append((dw, sec, c.s_get()))
         c.close()
-    (dw0, sec0, s0), (dw1, sec1, s1) = out
-    assert np.array_equal(sec0, sec1)
-    assert abs(dw0 - dw1) <= 1e-12 * max(1.0, dw1)
-    assert np.abs(s0 - s1).max() <= 1e-11 * np.abs(s1).max()
+    dw0, sec0, s0 = out[0]
+    for dw1, sec1, s1 in out[1:]:  # rocSOLVER's dgesvd, then its gesvdj
+        assert np.array_equal(sec0, sec1)
+        assert abs(dw0 - dw1) <= 1e-12 * max(1.0, dw1)
+        assert np.abs(s0 - s1).max() <= 1e-11 * np.abs(s1).max()
 
 
 def test_hubbard_l12_against_fci():
