@@ -5,9 +5,12 @@ D = 2000 with synthetic operators of the reference's shapes and applies Heff ten
 
   python -m chemps2_amd.site_probe [site]      (default 8: chained terms dominate)
 """
+import os
 import sys
 
 import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
 
 def main():
