@@ -463,122 +463,201 @@ void Ctx::prepare(int st) {
   // CHAIN_MAXD run in the output-stationary kernel (two launches: block products up to
   // CHAIN_SMALLP, and larger ones when the device grants the LDS); the rest as GEMM tiles
   {
-    std::vector<DevCBlock> cb, cbl, cbm[CHAIN_MFMA_NT];
-    std::vector<DevCGroup> cg;
-    std::vector<DevCTerm> ct;
-    std::vector<DevBRed> cr;
-    int64_t cpart_n = 0;
-    std::vector<const Term*> rest;
-    int Pmax = 1, PmaxL = 1;
+    // output blocks in chunks on the host threads, each with local records; the chunks
+    // are then concatenated in order (group, term and partial offsets shifted), so the
+    // lists do not depend on the thread count
+    struct Part {
+      std::vector<DevCBlock> cb, cbl, cbm[CHAIN_MFMA_NT];
+      std::vector<DevCGroup> cg;
+      std::vector<DevCTerm> ct;
+      std::vector<DevBRed> cr;
+      std::vector<const Term*> rest;
+      int64_t cpart_n = 0;
+      int Pmax = 1, PmaxL = 1;
+    };
+    std::vector<size_t> bstart;  // first chain index of every output block
+    for (size_t q = 0; q < chain.size(); q++)
+      if (q == 0 || chain[q]->out != chain[q - 1]->out) bstart.push_back(q);
+    bstart.push_back(chain.size());
+    const int64_t nblk = (int64_t)bstart.size() - 1;
+    const int64_t nparts = std::max<int64_t>(1, std::min<int64_t>(256, (nblk + 15) / 16));
+    std::vector<Part> parts(nparts);
     const int maxd = chain_large_ok ? CHAIN_MAXD : 36;
-    for (size_t q0 = 0; q0 < chain.size();) {
-      const int k = chain[q0]->out;
-      size_t q1 = q0;
-      bool small = use_chain_kernel && dLof(k) <= maxd && dRof(k) <= maxd;
-      while (q1 < chain.size() && chain[q1]->out == k) {
-        if (dLof(chain[q1]->in) > maxd || dRof(chain[q1]->in) > maxd) small = false;
-        q1++;
-      }
-      if (!small) {
-        for (size_t q = q0; q < q1; q++) rest.push_back(chain[q]);
-        q0 = q1;
-        continue;
-      }
-      DevCBlock B{};
-      B.dL = dLof(k); B.dR = dRof(k);
-      const int g_first = (int)cg.size();
-      int Pk = B.dL * B.dR;
-      int dmax = std::max(B.dL, B.dR);
-      for (size_t a = q0; a < q1;) {
-        const int in = chain[a]->in;
-        size_t e = a;
-        while (e < q1 && chain[e]->in == in) e++;
-        DevCGroup G{};
-        G.xoff = S.k2i[in]; G.dLi = dLof(in); G.dRi = dRof(in); G.t0 = (int)ct.size();
-        Pk = std::max({Pk, G.dLi * G.dRi, B.dL * G.dRi, G.dLi * B.dR, B.dL * G.dLi, G.dRi * B.dR});
-        dmax = std::max({dmax, G.dLi, G.dRi});
-        for (size_t q = a; q < e; q++) {
-          const Term& t = *chain[q];
-          const int dL = B.dL, dR = B.dR, dLi = G.dLi, dRi = G.dRi;
-          DevCTerm T{};
-          T.alpha = t.alpha;
-          if (t.tl != 'I') { T.A = resolve(plan->ref(t, t.Lb)); T.lda = (t.tl == 'N') ? dL : dLi; T.ta = (t.tl == 'T'); }
-          if (t.tr != 'I') { T.B = resolve(plan->ref(t, t.Rb)); T.ldb = (t.tr == 'N') ? dRi : dR; T.tb = (t.tr == 'T'); }
-          if (t.tl == 'I' && t.tr == 'I') T.mode = 0;
-          else if (t.tr == 'I') T.mode = 1;
-          else if (t.tl == 'I') T.mode = 2;
-          else {
-            const double c3 = (double)dL * dRi * dLi + (double)dL * dR * dRi, c4 = (double)dLi * dR * dRi + (double)dL * dR * dLi;
-            T.mode = c3 <= c4 ? 3 : 4;
+    parallel_for(nparts, nthr, [&](int64_t pi, int) {
+      Part& P = parts[pi];
+      std::vector<DevCBlock>&cb = P.cb, &cbl = P.cbl;
+      std::vector<DevCBlock>* cbm = P.cbm;
+      std::vector<DevCGroup>& cg = P.cg;
+      std::vector<DevCTerm>& ct = P.ct;
+      std::vector<DevBRed>& cr = P.cr;
+      std::vector<const Term*>& rest = P.rest;
+      int64_t& cpart_n = P.cpart_n;
+      int &Pmax = P.Pmax, &PmaxL = P.PmaxL;
+      const size_t qb = bstart[nblk * pi / nparts], qe = bstart[nblk * (pi + 1) / nparts];
+      for (size_t q0 = qb; q0 < qe;) {
+        const int k = chain[q0]->out;
+        size_t q1 = q0;
+        bool small = use_chain_kernel && dLof(k) <= maxd && dRof(k) <= maxd;
+        while (q1 < qe && chain[q1]->out == k) {
+          if (dLof(chain[q1]->in) > maxd || dRof(chain[q1]->in) > maxd) small = false;
+          q1++;
+        }
+        if (!small) {
+          for (size_t q = q0; q < q1; q++) rest.push_back(chain[q]);
+          q0 = q1;
+          continue;
+        }
+        DevCBlock B{};
+        B.dL = dLof(k); B.dR = dRof(k);
+        const int g_first = (int)cg.size();
+        int Pk = B.dL * B.dR;
+        int dmax = std::max(B.dL, B.dR);
+        for (size_t a = q0; a < q1;) {
+          const int in = chain[a]->in;
+          size_t e = a;
+          while (e < q1 && chain[e]->in == in) e++;
+          DevCGroup G{};
+          G.xoff = S.k2i[in]; G.dLi = dLof(in); G.dRi = dRof(in); G.t0 = (int)ct.size();
+          Pk = std::max({Pk, G.dLi * G.dRi, B.dL * G.dRi, G.dLi * B.dR, B.dL * G.dLi, G.dRi * B.dR});
+          dmax = std::max({dmax, G.dLi, G.dRi});
+          for (size_t q = a; q < e; q++) {
+            const Term& t = *chain[q];
+            const int dL = B.dL, dR = B.dR, dLi = G.dLi, dRi = G.dRi;
+            DevCTerm T{};
+            T.alpha = t.alpha;
+            if (t.tl != 'I') { T.A = resolve(plan->ref(t, t.Lb)); T.lda = (t.tl == 'N') ? dL : dLi; T.ta = (t.tl == 'T'); }
+            if (t.tr != 'I') { T.B = resolve(plan->ref(t, t.Rb)); T.ldb = (t.tr == 'N') ? dRi : dR; T.tb = (t.tr == 'T'); }
+            if (t.tl == 'I' && t.tr == 'I') T.mode = 0;
+            else if (t.tr == 'I') T.mode = 1;
+            else if (t.tl == 'I') T.mode = 2;
+            else {
+              const double c3 = (double)dL * dRi * dLi + (double)dL * dR * dRi, c4 = (double)dLi * dR * dRi + (double)dL * dR * dLi;
+              T.mode = c3 <= c4 ? 3 : 4;
+            }
+            ct.push_back(T);
           }
-          ct.push_back(T);
+          // groups of at most CHAIN_GROUP_TERMS terms (x is staged once per group)
+          const int t_end = (int)ct.size();
+          for (int t0 = G.t0; t0 < t_end; t0 += CHAIN_GROUP_TERMS) {
+            DevCGroup Gs = G;
+            Gs.t0 = t0;
+            Gs.nt = std::min(CHAIN_GROUP_TERMS, t_end - t0);
+            cg.push_back(Gs);
+          }
+          a = e;
         }
-        // groups of at most CHAIN_GROUP_TERMS terms (x is staged once per group)
-        const int t_end = (int)ct.size();
-        for (int t0 = G.t0; t0 < t_end; t0 += CHAIN_GROUP_TERMS) {
-          DevCGroup Gs = G;
-          Gs.t0 = t0;
-          Gs.nt = std::min(CHAIN_GROUP_TERMS, t_end - t0);
-          cg.push_back(Gs);
+        // work items of about CHAIN_ITEM_TERMS terms; a block with several items sums
+        // their partials in k_chain_reduce
+        std::vector<DevCBlock> items;
+        for (int g = g_first; g < (int)cg.size();) {
+          DevCBlock it = B;
+          it.g0 = g;
+          int nt = 0;
+          while (g < (int)cg.size() && (nt == 0 || nt + cg[g].nt <= CHAIN_ITEM_TERMS)) nt += cg[g++].nt;
+          it.ng = g - it.g0;
+          items.push_back(it);
         }
-        a = e;
-      }
-      // work items of about CHAIN_ITEM_TERMS terms; a block with several items sums
-      // their partials in k_chain_reduce
-      std::vector<DevCBlock> items;
-      for (int g = g_first; g < (int)cg.size();) {
-        DevCBlock it = B;
-        it.g0 = g;
-        int nt = 0;
-        while (g < (int)cg.size() && (nt == 0 || nt + cg[g].nt <= CHAIN_ITEM_TERMS)) nt += cg[g++].nt;
-        it.ng = g - it.g0;
-        items.push_back(it);
-      }
-      const int64_t nel = (int64_t)B.dL * B.dR;
-      if (items.size() == 1) {
-        items[0].direct = 1;
-        items[0].out = S.k2i[k];
-      } else {
-        cr.push_back({S.k2i[k], cpart_n, (int)nel, (int)items.size()});
-        for (auto& it : items) { it.direct = 0; it.out = cpart_n; cpart_n += nel; }
-      }
-      const int nt = (dmax + 15) / 16;
-      if (use_chain_mfma && nt <= CHAIN_MFMA_NT) {
-        for (auto& it : items) cbm[nt - 1].push_back(it);
-      } else {
-        for (auto& it : items) {
-          if (Pk <= CHAIN_SMALLP) cb.push_back(it);
-          else cbl.push_back(it);
+        const int64_t nel = (int64_t)B.dL * B.dR;
+        if (items.size() == 1) {
+          items[0].direct = 1;
+          items[0].out = S.k2i[k];
+        } else {
+          cr.push_back({S.k2i[k], cpart_n, (int)nel, (int)items.size()});
+          for (auto& it : items) { it.direct = 0; it.out = cpart_n; cpart_n += nel; }
         }
-        if (Pk <= CHAIN_SMALLP) Pmax = std::max(Pmax, Pk);
-        else PmaxL = std::max(PmaxL, Pk);
+        const int nt = (dmax + 15) / 16;
+        if (use_chain_mfma && nt <= CHAIN_MFMA_NT) {
+          for (auto& it : items) cbm[nt - 1].push_back(it);
+        } else {
+          for (auto& it : items) {
+            if (Pk <= CHAIN_SMALLP) cb.push_back(it);
+            else cbl.push_back(it);
+          }
+          if (Pk <= CHAIN_SMALLP) Pmax = std::max(Pmax, Pk);
+          else PmaxL = std::max(PmaxL, Pk);
+        }
+        q0 = q1;
       }
-      q0 = q1;
+    });
+    // offsets of every part in the concatenated lists (cbm: all parts of tile count 1,
+    // then of 2, ...), then a parallel copy into page-locked staging with the indices
+    // shifted, and asynchronous uploads (complete at the synchronisation ending prepare)
+    enum { L_CG, L_CT, L_CR, L_CB, L_CBL, L_REST, L_M0 };
+    const int NL = L_M0 + CHAIN_MFMA_NT;
+    std::vector<int64_t> off((size_t)NL * (nparts + 1), 0), poff(nparts + 1, 0);
+    auto O = [&](int l, int64_t pi) -> int64_t& { return off[(size_t)l * (nparts + 1) + pi]; };
+    int Pmax = 1, PmaxL = 1;
+    for (int64_t pi = 0; pi < nparts; pi++) {
+      const Part& P = parts[pi];
+      const size_t sz[NL - CHAIN_MFMA_NT] = {P.cg.size(), P.ct.size(), P.cr.size(), P.cb.size(), P.cbl.size(), P.rest.size()};
+      for (int l = 0; l < L_M0; l++) O(l, pi + 1) = O(l, pi) + (int64_t)sz[l];
+      for (int n = 0; n < CHAIN_MFMA_NT; n++) O(L_M0 + n, pi + 1) = O(L_M0 + n, pi) + (int64_t)P.cbm[n].size();
+      poff[pi + 1] = poff[pi] + P.cpart_n;
+      Pmax = std::max(Pmax, P.Pmax);
+      PmaxL = std::max(PmaxL, P.PmaxL);
     }
-    if ((int64_t)ct.size() > INT32_MAX || (int64_t)cg.size() > INT32_MAX) throw Error{ERR_SIZE, "too many chained terms"};
-    cblocks.upload(cb, stream);
-    {
-      std::vector<DevCBlock> all;
-      cblocks_m_offs[0] = 0;
+    const int64_t n_cg = O(L_CG, nparts), n_ct = O(L_CT, nparts), n_cr = O(L_CR, nparts), n_cb = O(L_CB, nparts),
+                  n_cbl = O(L_CBL, nparts), n_rest = O(L_REST, nparts), cpart_n = poff[nparts];
+    int64_t n_m[CHAIN_MFMA_NT], n_mall = 0;
+    for (int n = 0; n < CHAIN_MFMA_NT; n++) { n_m[n] = O(L_M0 + n, nparts); n_mall += n_m[n]; }
+    if (n_ct > INT32_MAX || n_cg > INT32_MAX) throw Error{ERR_SIZE, "too many chained terms"};
+    cstage.reset();
+    cstage.reserve(8 * 256 + n_cg * sizeof(DevCGroup) + n_ct * sizeof(DevCTerm) + n_cr * sizeof(DevBRed) +
+                   (n_cb + n_cbl + n_mall) * sizeof(DevCBlock));
+    DevCGroup* hcg = cstage.take<DevCGroup>(n_cg);
+    DevCTerm* hct = cstage.take<DevCTerm>(n_ct);
+    DevBRed* hcr = cstage.take<DevBRed>(n_cr);
+    DevCBlock* hcb = cstage.take<DevCBlock>(n_cb);
+    DevCBlock* hcbl = cstage.take<DevCBlock>(n_cbl);
+    DevCBlock* hcbm = cstage.take<DevCBlock>(n_mall);
+    std::vector<const Term*> rest(n_rest);
+    parallel_for(nparts, nthr, [&](int64_t pi, int) {
+      const Part& P = parts[pi];
+      const int g_off = (int)O(L_CG, pi), t_off = (int)O(L_CT, pi);
+      const int64_t p_off = poff[pi];
+      DevCGroup* g = hcg + O(L_CG, pi);
+      for (DevCGroup x : P.cg) { x.t0 += t_off; *g++ = x; }
+      if (!P.ct.empty()) std::memcpy(hct + O(L_CT, pi), P.ct.data(), P.ct.size() * sizeof(DevCTerm));
+      DevBRed* r = hcr + O(L_CR, pi);
+      for (DevBRed x : P.cr) { x.p0 += p_off; *r++ = x; }
+      auto shift = [&](DevCBlock* dst, const std::vector<DevCBlock>& src) {
+        for (DevCBlock b : src) {
+          b.g0 += g_off;
+          if (!b.direct) b.out += p_off;
+          *dst++ = b;
+        }
+      };
+      shift(hcb + O(L_CB, pi), P.cb);
+      shift(hcbl + O(L_CBL, pi), P.cbl);
+      int64_t mbase = 0;
       for (int n = 0; n < CHAIN_MFMA_NT; n++) {
-        all.insert(all.end(), cbm[n].begin(), cbm[n].end());
-        cblocks_m_offs[n + 1] = (int)all.size();
+        shift(hcbm + mbase + O(L_M0 + n, pi), P.cbm[n]);
+        mbase += n_m[n];
       }
-      cblocks_m.upload(all, stream);
-    }
-    cgroups.upload(cg, stream);
-    cterms.upload(ct, stream);
-    ncblocks = (int)cb.size();
+      std::copy(P.rest.begin(), P.rest.end(), rest.begin() + O(L_REST, pi));
+    });
+    auto up = [&](auto& dbuf, const auto* h, int64_t n) {
+      dbuf.alloc(std::max<int64_t>(n, 1));
+      if (n > 0) hip_check(hipMemcpyAsync(dbuf.p, h, n * sizeof(*h), hipMemcpyHostToDevice, stream), "chain lists");
+    };
+    up(cblocks, hcb, n_cb);
+    up(cblocks_m, hcbm, n_mall);
+    cblocks_m_offs[0] = 0;
+    for (int n = 0; n < CHAIN_MFMA_NT; n++) cblocks_m_offs[n + 1] = cblocks_m_offs[n] + (int)n_m[n];
+    up(cgroups, hcg, n_cg);
+    up(cterms, hct, n_ct);
+    ncblocks = (int)n_cb;
     chain_P = std::min((Pmax + 31) / 32 * 32, CHAIN_SMALLP);
-    cblocks_l.upload(cbl, stream);
-    ncblocks_l = (int)cbl.size();
-    creds.upload(cr, stream);
-    ncreds = (int)cr.size();
+    up(cblocks_l, hcbl, n_cbl);
+    ncblocks_l = (int)n_cbl;
+    up(creds, hcr, n_cr);
+    ncreds = (int)n_cr;
     cpart.alloc(std::max<int64_t>(cpart_n, 1));
     chain_P_l = std::min((PmaxL + 31) / 32 * 32, CHAIN_MAXD * CHAIN_MAXD);
-    stats[15] = (int64_t)ct.size();
+    stats[15] = n_ct;
     chain.swap(rest);
   }
+  const double t_route = now_s();
   // ---- chained GEMM tiles for the remaining terms (in (out, in) order)
   std::vector<DevGemmOp> o1, o2;
   std::vector<DevTile> t1, t2;
@@ -715,6 +794,7 @@ void Ctx::prepare(int st) {
   stats[6] = tot.pairs;
   stats[7] = (int64_t)chain.size() + stats[15];
 
+  const double t_tiles = now_s();
   // ---- diagonal
   {
     std::vector<DevDiagBlock> blocks(S.nk());
@@ -742,6 +822,7 @@ void Ctx::prepare(int st) {
     stats[10] = (int64_t)terms.size();
   }
 
+  const double t_diag = now_s();
   // ---- program <-> symmetric convention factors (Sobject.cpp:624-650)
   {
     std::vector<double> fs(S.nk()), fp(S.nk());
@@ -755,6 +836,7 @@ void Ctx::prepare(int st) {
   partial.alloc((size_t)dot_parts(V) * 64);
   scal.alloc(256);
   hip_check(hipStreamSynchronize(stream), "prepare");
+  const double t_sync = now_s();
 
   // ---- algorithmic bytes: the vectors and every operator tensor the step reads
   {
@@ -777,9 +859,10 @@ void Ctx::prepare(int st) {
     {
       int maxd = 0;
       for (int k = 0; k < S.nk(); k++) maxd = std::max({maxd, S.dL[k], S.dR[k]});
-      std::fprintf(stderr, "prepare site %d: plan %.3f s (generate %.3f, merge %.3f, finish %.3f), combinations %.3f s, lists %.3f s, assembly %.3f s, rest %.3f s, total %.3f s; "
+      std::fprintf(stderr, "prepare site %d: plan %.3f s (generate %.3f, merge %.3f, finish %.3f), combinations %.3f s, lists %.3f s, assembly %.3f s, rest %.3f s (chain %.3f, tiles %.3f, diagonal %.3f, sync %.3f, bytes %.3f), total %.3f s; "
                    "max block dim %d, terms explicit %lld / chain kernel %lld / tiles %lld\n",
-                   st, ph[0], plan->phase_s[0], plan->phase_s[1], plan->phase_s[2], ph[1], ph[2], ph[3], ph[4], ph[5], maxd,
+                   st, ph[0], plan->phase_s[0], plan->phase_s[1], plan->phase_s[2], ph[1], ph[2], ph[3], ph[4],
+                   t_route - t_assemble, t_tiles - t_route, t_diag - t_tiles, t_sync - t_diag, t_end - t_sync, ph[5], maxd,
                    (long long)(plan->nterms - stats[7]), (long long)stats[15], (long long)(stats[7] - stats[15]));
     }
   }

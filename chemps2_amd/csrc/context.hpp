@@ -182,6 +182,7 @@ struct Ctx : HostState {
   std::vector<std::vector<Term>> term_pool;
   // update executor: staging and scratch kept across calls (no page faults per call)
   HostStage ustage;
+  HostStage cstage;  // chained-kernel records of the prepared site
   std::vector<int> u_qof, u_seg;
   std::vector<int64_t> u_woff, u_order, u_hist;
   DBuf<DevGemmOp> u_d1, u_d2;
