@@ -974,11 +974,6 @@ void Ctx::run_terms(const std::vector<UTerm>& terms, const std::vector<std::pair
 }
 
 // Executes update terms: every output block is overwritten by the sum of its terms.
-// Three-operand terms first form W = alpha op(A) op(B) (tile phase 1); phase 2 sums,
-// per output block, the two-operand products, W op(C) and the transposed axpys. The
-// lists are built on the host threads; positions come from prefix sums, so they do
-// not depend on the thread count.
-// Executes update terms: every output block is overwritten by the sum of its terms.
 // Three-operand terms first form W = alpha op(A) op(B); then every output block sums its
 // two-operand products, W op(C) and the transposed axpys. Products whose blocks are
 // small run in k_ops (2 x 2 register tiles, output blocks with many products split
@@ -991,13 +986,34 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
   const int nthr = host_threads();
   const int64_t nseg = (int64_t)segs.size();
   const int64_t nq = (int64_t)outs.size();
-  auto key = [](const Ref& r) { return ((uint64_t)(uint32_t)r.buf << 40) ^ (uint64_t)r.off; };
-  FlatMap where;
-  {
-    std::vector<std::pair<uint64_t, int>> kv(outs.size());
-    for (size_t q = 0; q < outs.size(); q++) kv[q] = {key(outs[q].first), (int)q};
-    where.build(kv);
+  // output index of a (buffer, offset) reference: the outputs come grouped by buffer in
+  // increasing offset, so a buffer's range of outputs and a binary search find it (the
+  // last of equal offsets: an empty block shares its offset with the block after it)
+  std::vector<int> qfirst(bufs.size() + 1, -1), qcount(bufs.size() + 1, 0);
+  for (int64_t q = 0; q < nq; q++) {
+    const Ref& r = outs[q].first;
+    if (r.buf < 0 || r.buf >= (int)qfirst.size()) throw Error{ERR_STATE, "operator update output outside the buffers"};
+    if (qfirst[r.buf] < 0) qfirst[r.buf] = (int)q;
+    else if (qfirst[r.buf] + qcount[r.buf] != q || outs[q - 1].first.off > r.off)
+      throw Error{ERR_STATE, "operator update outputs not grouped by buffer in offset order"};
+    qcount[r.buf]++;
   }
+  struct Where {
+    const std::vector<std::pair<Ref, std::pair<int, int>>>& outs;
+    const std::vector<int>& first;
+    const std::vector<int>& count;
+    int get(const Ref& r) const {
+      if (r.buf < 0 || r.buf >= (int)first.size() || first[r.buf] < 0) return -1;
+      int lo = first[r.buf], hi = lo + count[r.buf] - 1;
+      if (outs[lo].first.off > r.off) return -1;
+      while (lo < hi) {  // the last output with offset <= r.off
+        const int mid = (lo + hi + 1) >> 1;
+        if (outs[mid].first.off <= r.off) lo = mid; else hi = mid - 1;
+      }
+      return outs[lo].first.off == r.off ? lo : -1;
+    }
+  } where{outs, qfirst, qcount};
+  const double t_map = now_s();
   std::vector<int64_t> s0(nseg + 1, 0);
   for (int64_t s = 0; s < nseg; s++) s0[s + 1] = s0[s] + (int64_t)segs[s].terms.size();
   const int64_t n = s0[nseg];
@@ -1022,7 +1038,7 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
     P1 c;
     for (size_t j = 0; j < segs[s].terms.size(); j++) {
       const UTerm& u = segs[s].terms[j];
-      const int qq = where.get(key(u.out));
+      const int qq = where.get(u.out);
       if (qq < 0) throw Error{ERR_STATE, "operator update term writes outside the new tensors"};
       qof[s0[s] + j] = qq;
       seg_of[s0[s] + j] = (int)s;
@@ -1045,28 +1061,42 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
   std::vector<int64_t> qstart(nq + 1, 0);
   {
     const int R = (int)std::max<int64_t>(1, std::min<int64_t>(nthr, (n + 65535) / 65536));
-    u_hist.assign((size_t)R * nq, 0);
+    if (u_hist.size() < (size_t)R * nq) u_hist.resize((size_t)R * nq);
     int64_t* hist = u_hist.data();
     auto range = [&](int r) { return std::make_pair(n * r / R, n * (r + 1) / R); };
     parallel_for(R, nthr, [&](int64_t r, int) {
+      std::fill(hist + r * nq, hist + (r + 1) * nq, (int64_t)0);
       const auto rg = range((int)r);
       for (int64_t g = rg.first; g < rg.second; g++) hist[r * nq + qof[g]]++;
     });
-    int64_t pos = 0;
-    for (int64_t q = 0; q < nq; q++) {
-      qstart[q] = pos;
-      for (int r = 0; r < R; r++) {
-        const int64_t c = hist[r * nq + q];
-        hist[r * nq + q] = pos;
-        pos += c;
+    // exclusive prefix over (q, r) in q-major order: chunk sums, their prefix, then fill
+    const int64_t NC = std::max<int64_t>(1, std::min<int64_t>(64, nq / 4096));
+    std::vector<int64_t> csum(NC + 1, 0);
+    parallel_for(NC, nthr, [&](int64_t c, int) {
+      int64_t t = 0;
+      for (int64_t q = nq * c / NC; q < nq * (c + 1) / NC; q++)
+        for (int r = 0; r < R; r++) t += hist[r * nq + q];
+      csum[c + 1] = t;
+    });
+    for (int64_t c = 0; c < NC; c++) csum[c + 1] += csum[c];
+    parallel_for(NC, nthr, [&](int64_t c, int) {
+      int64_t pos = csum[c];
+      for (int64_t q = nq * c / NC; q < nq * (c + 1) / NC; q++) {
+        qstart[q] = pos;
+        for (int r = 0; r < R; r++) {
+          const int64_t cnt = hist[r * nq + q];
+          hist[r * nq + q] = pos;
+          pos += cnt;
+        }
       }
-    }
-    qstart[nq] = pos;
+    });
+    qstart[nq] = csum[NC];
     parallel_for(R, nthr, [&](int64_t r, int) {
       const auto rg = range((int)r);
       for (int64_t g = rg.first; g < rg.second; g++) order[hist[r * nq + qof[g]]++] = g;
     });
   }
+  const double t_sort = now_s();
   // phase 2 per output block: small path or tiles; counts, then positions
   std::vector<char> qsmall(nq, 0);
   std::vector<int64_t> t2start(nq + 1, 0), i2start(nq + 1, 0), p2start(nq + 1, 0);
@@ -1099,6 +1129,7 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
   }
   const int64_t nt2 = t2start[nq], ni2 = i2start[nq], npart = p2start[nq];
   if (nt1 > INT32_MAX || nt2 > INT32_MAX || ni1 + ni2 > INT32_MAX) throw Error{ERR_SIZE, "too many update tiles"};
+  const double t_counts = now_s();
   ustage.reset();
   ustage.reserve(256 * 8 + (n1 + n) * sizeof(DevGemmOp) + (nt1 + nt2) * sizeof(DevTile) + (ni1 + ni2) * sizeof(DevOItem) +
                  nred * sizeof(DevPRed));
@@ -1230,9 +1261,9 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
   hip_check(hipGetLastError(), "update kernels");
   hip_check(hipStreamSynchronize(stream), "update");
   if (verbose)
-    std::fprintf(stderr, "    terms %lld: lists %.3f s, upload %.3f s, kernels %.3f s (%lld + %lld tiles, %lld + %lld small items)\n",
-                 (long long)n, t_lists - t_begin, t_up - t_lists, now_s() - t_up, (long long)nt1, (long long)nt2,
-                 (long long)ni1, (long long)ni2);
+    std::fprintf(stderr, "    terms %lld: lists %.3f s (map %.3f, sort %.3f, counts %.3f, fill %.3f), upload %.3f s, kernels %.3f s (%lld + %lld tiles, %lld + %lld small items)\n",
+                 (long long)n, t_lists - t_begin, t_map - t_begin, t_sort - t_map, t_counts - t_sort, t_lists - t_counts,
+                 t_up - t_lists, now_s() - t_up, (long long)nt1, (long long)nt2, (long long)ni1, (long long)ni2);
 }
 
 // DMRG::updateMovingRight/Left (DMRGoperators.cpp:243-907) for slot `index`.
