@@ -1036,9 +1036,13 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
   std::vector<P1> p1(nseg + 1);
   parallel_for(nseg, nthr, [&](int64_t s, int) {
     P1 c;
+    int last = -1;  // consecutive terms mostly write the same block
     for (size_t j = 0; j < segs[s].terms.size(); j++) {
       const UTerm& u = segs[s].terms[j];
-      const int qq = where.get(u.out);
+      const int qq = (last >= 0 && outs[last].first.buf == u.out.buf && outs[last].first.off == u.out.off)
+                         ? last
+                         : where.get(u.out);
+      last = qq;
       if (qq < 0) throw Error{ERR_STATE, "operator update term writes outside the new tensors"};
       qof[s0[s] + j] = qq;
       seg_of[s0[s] + j] = (int)s;
