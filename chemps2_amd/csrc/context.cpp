@@ -538,10 +538,10 @@ void Ctx::prepare(int st) {
           }
           // groups of at most CHAIN_GROUP_TERMS terms (x is staged once per group)
           const int t_end = (int)ct.size();
-          for (int t0 = G.t0; t0 < t_end; t0 += CHAIN_GROUP_TERMS) {
+          for (int t0 = G.t0; t0 < t_end; t0 += chain_group_terms) {
             DevCGroup Gs = G;
             Gs.t0 = t0;
-            Gs.nt = std::min(CHAIN_GROUP_TERMS, t_end - t0);
+            Gs.nt = std::min(chain_group_terms, t_end - t0);
             cg.push_back(Gs);
           }
           a = e;
@@ -553,7 +553,7 @@ void Ctx::prepare(int st) {
           DevCBlock it = B;
           it.g0 = g;
           int nt = 0;
-          while (g < (int)cg.size() && (nt == 0 || nt + cg[g].nt <= CHAIN_ITEM_TERMS)) nt += cg[g++].nt;
+          while (g < (int)cg.size() && (nt == 0 || nt + cg[g].nt <= chain_item_terms)) nt += cg[g++].nt;
           it.ng = g - it.g0;
           items.push_back(it);
         }
@@ -1731,6 +1731,8 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_CHAIN_KERNEL")) ctx->c.use_chain_kernel = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_CHAIN_MFMA")) ctx->c.use_chain_mfma = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_TILES_MFMA")) ctx->c.use_tiles_mfma = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CHM_CHAIN_ITEM_TERMS")) ctx->c.chain_item_terms = std::max(1, std::atoi(e));
+  if (const char* e = std::getenv("CHM_CHAIN_GROUP_TERMS")) ctx->c.chain_group_terms = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("CHM_SVD_GESVDJ")) ctx->c.svd_large_gesvdj = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_TILE_W_CAP")) ctx->c.tile_w_cap = std::max(1LL, std::atoll(e));
   if (const char* e = std::getenv("CHM_OPS_KERNEL")) ctx->c.use_ops_kernel = std::atoi(e) != 0;

@@ -159,6 +159,8 @@ struct Ctx : HostState {
   bool use_chain_mfma = false;
   DBuf<DevCBlock> cblocks_m;         // k_chain_mfma items sorted by tile count
   bool svd_large_gesvdj = false;     // CHM_SVD_GESVDJ=1: large Split sectors by rocsolver_dgesvdj instead of dgesvd
+  int chain_item_terms = CHAIN_ITEM_TERMS;    // CHM_CHAIN_ITEM_TERMS
+  int chain_group_terms = CHAIN_GROUP_TERMS;  // CHM_CHAIN_GROUP_TERMS
   bool use_tiles_mfma = true;        // CHM_TILES_MFMA=0: GEMM tiles in k_tiles (vector FMA) instead of k_tiles_mfma
   int cblocks_m_offs[CHAIN_MFMA_NT + 1] = {};
   DBuf<DevCBlock> cblocks;
