@@ -201,23 +201,32 @@ void HeffPlanner::build() {  // Heff.cpp:43-248 (the loop over output blocks)
   for (int64_t c = 0; c < nch; c++) local[c].resize(parts[c].hash.size());
   parallel_for(NSH, nt, [&](int64_t s, int) {
     Shard& sh = shards[s];
-    std::unordered_map<uint64_t, int> cache;
+    // open-addressing table hash -> first representative with that hash
+    int64_t cand = 0;
+    for (int64_t c = 0; c < nch; c++) cand += sh_start[c][s + 1] - sh_start[c][s];
+    size_t cap = 16;
+    while (cap < (size_t)cand * 2) cap <<= 1;
+    std::vector<uint64_t> keys(cap);
+    std::vector<int> vals(cap, -1);
+    const size_t mask = cap - 1;
     for (int64_t c = 0; c < nch; c++) {
       const Part& p = parts[c];
       for (int e = sh_start[c][s]; e < sh_start[c][s + 1]; e++) {
         const int q = by_shard[c][e];
         const Combo& cb = p.combos[q];
-        auto it = cache.find(p.hash[q]);
-        if (it != cache.end()) {
-          const auto& r = sh.reps[it->second];
+        const uint64_t h = p.hash[q];
+        size_t slot = (size_t)(h * 0x9E3779B97F4A7C15ULL >> 20) & mask;
+        while (vals[slot] >= 0 && keys[slot] != h) slot = (slot + 1) & mask;
+        if (vals[slot] >= 0) {
+          const auto& r = sh.reps[vals[slot]];
           if (same_combo(parts[r.first].combos[r.second], cb.src, cb.size)) {
-            local[c][q] = ((int64_t)s << 40) | it->second;
+            local[c][q] = ((int64_t)s << 40) | vals[slot];
             continue;
           }
         }
         const int id = (int)sh.reps.size();
         sh.reps.push_back({c, q});
-        if (it == cache.end()) cache.emplace(p.hash[q], id);
+        if (vals[slot] < 0) { keys[slot] = h; vals[slot] = id; }
         local[c][q] = ((int64_t)s << 40) | id;
       }
     }
@@ -230,12 +239,16 @@ void HeffPlanner::build() {  // Heff.cpp:43-248 (the loop over output blocks)
   std::vector<int64_t> sbase(NSH + 1, 0);
   for (int s = 0; s < NSH; s++) sbase[s + 1] = sbase[s] + shards[s].total;
   combo_size = sbase[NSH];
-  for (int s = 0; s < NSH; s++)
+  std::vector<int64_t> cbase(NSH + 1, 0);
+  for (int s = 0; s < NSH; s++) cbase[s + 1] = cbase[s] + (int64_t)shards[s].reps.size();
+  combos.resize(cbase[NSH]);
+  parallel_for(NSH, nt, [&](int64_t s, int) {
     for (size_t i = 0; i < shards[s].reps.size(); i++) {
       Combo& cb = parts[shards[s].reps[i].first].combos[shards[s].reps[i].second];
       cb.out_off = sbase[s] + shards[s].size[i];
-      combos.push_back(std::move(cb));
+      combos[cbase[s] + i] = std::move(cb);
     }
+  });
   remap.assign(nch, {});
   parallel_for(nch, nt, [&](int64_t c, int) {
     remap[c].resize(local[c].size());
