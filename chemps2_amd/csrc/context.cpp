@@ -269,13 +269,10 @@ void Ctx::prepare(int st) {
       }
     });
     if (!dc.empty()) {
-      DBuf<DevCombo> d_dc;
-      DBuf<DevSrc> d_ds;
-      DBuf<int64_t> d_cs;
-      d_dc.upload(dc, stream);
-      d_ds.upload(ds, stream);
-      d_cs.upload(chunk_start, stream);
-      launch_combos(d_dc.p, d_ds.p, (int)dc.size(), d_cs.p, nch, stream);
+      cmb_dc.upload(dc, stream);
+      cmb_ds.upload(ds, stream);
+      cmb_cs.upload(chunk_start, stream);
+      launch_combos(cmb_dc.p, cmb_ds.p, (int)dc.size(), cmb_cs.p, nch, stream);
       hip_check(hipGetLastError(), "combination kernel launch");
       hip_check(hipStreamSynchronize(stream), "combination kernel");
     }
@@ -1307,28 +1304,36 @@ void Ctx::update_operators(int index, bool mr) {
   last_update_flops = up.flops;
   last_update_terms = up.nterms(1) + up.nterms(2);
   // combinations first (they read the previous slot only)
-  DBuf<double> arena;
+  DBuf<double>& arena = u_arena;  // kept between updates (grown as needed)
+  hip_check(hipStreamSynchronize(stream), "update arena");
   arena.alloc(std::max<int64_t>(up.combo_size, 1));
   if (!up.combos.empty()) {
-    std::vector<DevCombo> dc;
-    std::vector<DevSrc> ds;
-    std::vector<int64_t> chunk_start;
+    // records by prefix sums, filled on the host threads
+    const int64_t ncb = (int64_t)up.combos.size();
+    std::vector<DevCombo> dc(ncb);
+    std::vector<int64_t> chunk_start(ncb), src0(ncb + 1, 0);
     int64_t nch = 0;
-    for (const Combo& c : up.combos) {
-      DevCombo d;
-      d.out = arena.p + c.out_off; d.size = c.size; d.src0 = (int)ds.size(); d.nsrc = (int)c.src.size();
-      for (auto& e : c.src) ds.push_back({resolve(e.first), e.second});
-      dc.push_back(d);
-      chunk_start.push_back(nch);
+    for (int64_t i = 0; i < ncb; i++) {
+      const Combo& c = up.combos[i];
+      src0[i + 1] = src0[i] + (int64_t)c.src.size();
+      chunk_start[i] = nch;
       nch += (c.size + 1023) / 1024;
     }
-    DBuf<DevCombo> d_dc;
-    DBuf<DevSrc> d_ds;
-    DBuf<int64_t> d_cs;
-    d_dc.upload(dc, stream);
-    d_ds.upload(ds, stream);
-    d_cs.upload(chunk_start, stream);
-    launch_combos(d_dc.p, d_ds.p, (int)dc.size(), d_cs.p, nch, stream);
+    if (src0[ncb] > INT32_MAX) throw Error{ERR_SIZE, "too many combination sources"};
+    std::vector<DevSrc> ds(src0[ncb]);
+    parallel_for((ncb + 4095) / 4096, host_threads(), [&](int64_t blk, int) {
+      for (int64_t i = blk * 4096; i < std::min<int64_t>(ncb, (blk + 1) * 4096); i++) {
+        const Combo& c = up.combos[i];
+        DevCombo d;
+        d.out = arena.p + c.out_off; d.size = c.size; d.src0 = (int)src0[i]; d.nsrc = (int)c.src.size();
+        for (size_t q = 0; q < c.src.size(); q++) ds[src0[i] + q] = {resolve(c.src[q].first), c.src[q].second};
+        dc[i] = d;
+      }
+    });
+    cmb_dc.upload(dc, stream);
+    cmb_ds.upload(ds, stream);
+    cmb_cs.upload(chunk_start, stream);
+    launch_combos(cmb_dc.p, cmb_ds.p, (int)dc.size(), cmb_cs.p, nch, stream);
     hip_check(hipGetLastError(), "combination kernel launch");
     hip_check(hipStreamSynchronize(stream), "combinations");
   }

@@ -185,6 +185,10 @@ struct Ctx : HostState {
   // update executor: staging and scratch kept across calls (no page faults per call)
   HostStage ustage;
   HostStage cstage;  // chained-kernel records of the prepared site
+  DBuf<DevCombo> cmb_dc;  // combination records (site preparation and operator updates)
+  DBuf<DevSrc> cmb_ds;
+  DBuf<int64_t> cmb_cs;
+  DBuf<double> u_arena;   // combinations of the operator update
   std::vector<int> u_qof, u_seg;
   std::vector<int64_t> u_woff, u_order, u_hist;
   DBuf<DevGemmOp> u_d1, u_d2;
