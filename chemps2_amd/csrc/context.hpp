@@ -265,13 +265,15 @@ struct Ctx : HostState {
   void prepare(int site);
   const double* resolve(const Ref& r) const;
   void apply(const double* dxp, double* dyp);
-  // frees the prepared site's explicit matrix and tile scratch; the site must be
-  // prepared again before the next product
+  // frees the prepared site's explicit matrix and tile scratch (and the update arena,
+  // which the next update allocates again); the site must be prepared again before the
+  // next product
   void release_site_scratch() {
     Mvals.release();
     Mcol.release();
     bpart.release();
     W.release();
+    u_arena.release();
     M_nnz = 0;
     site = -1;
   }
