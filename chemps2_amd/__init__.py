@@ -22,7 +22,7 @@ EXPORTS = [
     "chm_davidson", "chm_flops", "chm_plan_stats", "chm_stream", "chm_timing_enable", "chm_timing_read",
     "chm_mps_upload", "chm_mps_download", "chm_tensor_download", "chm_update_operators",
     "chm_join", "chm_s_get", "chm_s_set", "chm_split", "chm_get_problem", "chm_mps_layout_size", "chm_dmrg_init", "chm_dmrg_presolve", "chm_dmrg_solve", "chm_dmrg_sweep", "chm_dmrg_heff_stats", "chm_dmrg_stats",
-    "chm_set_heff_form", "chm_heff_apply_f32", "chm_mixed_stats",
+    "chm_set_heff_form", "chm_heff_apply_f32", "chm_mixed_stats", "chm_heff_set_lower",
     "chm_precond_mode", "chm_mlp_nparams", "chm_mlp_set", "chm_mlp_get", "chm_mlp_forward", "chm_mlp_loss_grad",
     "chm_mlp_train", "chm_harvest", "chm_harvest_take", "chm_dmrg_sweep_range",
 ]
@@ -80,6 +80,7 @@ def load():
         "chm_heff_apply": (i, [vp, vp, vp, i]),
         "chm_heff_apply_f32": (i, [vp, vp, vp, i]),
         "chm_set_heff_form": (i, [vp, i]),
+        "chm_heff_set_lower": (i, [vp, i, vp, i]),
         "chm_mixed_stats": (i, [vp, C.POINTER(i64), C.POINTER(i64)]),
         "chm_precond_mode": (i, [vp, i]),
         "chm_mlp_nparams": (i, [C.POINTER(i64)]),
@@ -389,6 +390,22 @@ class Context:
             return y
         self._check(fn(self.h, _ptr(x), _ptr(y), 1), name)
         return y
+
+    def set_lower(self, veff):
+        """Lower states' VeffTilde vectors for the prepared site (n_lower x V, host array
+        or device tensor); None or an empty array removes them."""
+        if veff is None or len(veff) == 0:
+            self._check(self.lib.chm_heff_set_lower(self.h, 0, None, 0), "chm_heff_set_lower")
+            return
+        if isinstance(veff, np.ndarray):
+            v = np.ascontiguousarray(np.atleast_2d(veff), dtype=np.float64)
+            if v.shape[1] != self.V:
+                raise ValueError(f"lower-state vectors of length {v.shape[1]}, the site has V = {self.V}")
+            self._check(self.lib.chm_heff_set_lower(self.h, v.shape[0], _ptr(v), 0), "chm_heff_set_lower")
+            return
+        if veff.dim() != 2 or veff.shape[1] != self.V or not veff.is_contiguous():
+            raise ValueError("lower-state vectors must be a contiguous n_lower x V tensor")
+        self._check(self.lib.chm_heff_set_lower(self.h, veff.shape[0], _ptr(veff), 1), "chm_heff_set_lower")
 
     def set_heff_form(self, form):
         """FORM_FP64 or FORM_MIXED; applies from the next prepare."""

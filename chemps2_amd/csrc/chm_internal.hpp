@@ -407,6 +407,7 @@ void launch_multi_dot(const double* const* d_vecs, int nv, const double* b, int6
                       hipStream_t s);
 void launch_axpy_dev(const double* coef, double sign, const double* x, double* y, int64_t n, hipStream_t s);
 void launch_axpy(double a, const double* x, double* y, int64_t n, hipStream_t s);
+void launch_add_squares(const double* v, int nv, double* d, int64_t n, hipStream_t s);
 void launch_scale(double a, double* x, int64_t n, hipStream_t s);
 void launch_lincomb_vecs(const double* const* d_vecs, const double* d_coefs, int nv, double* out, int64_t n,
                          hipStream_t s);

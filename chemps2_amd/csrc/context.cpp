@@ -213,6 +213,7 @@ void Ctx::prepare(int st) {
   if (!have_problem) throw Error{ERR_STATE, "set the problem first"};
   if (st < 0 || st > L - 2) throw Error{ERR_ARG, "site out of range"};
   site = -1;
+  n_lower = 0;
   OpSet* Lo = nullptr;
   OpSet* Ro = nullptr;
   if (st > 0) { validate_set(st - 1, true); Lo = &sets[st - 1]; }
@@ -890,6 +891,10 @@ void Ctx::apply(const double* xp, double* yp) {
   launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, cpart.p, chain_P, false, stream);
   launch_chain(cblocks_l.p, ncblocks_l, cgroups.p, cterms.p, xp, yp, cpart.p, chain_P_l, true, stream);
   launch_chain_reduce(creds.p, ncreds, cpart.p, yp, stream);
+  if (n_lower > 0) {  // lower states, in state order (HeffDiagrams1.cpp:65-86)
+    launch_multi_dot(lower_ptrs.p, n_lower, xp, V, lower_part.p, lower_dot.p, stream);
+    for (int q = 0; q < n_lower; q++) launch_axpy_dev(lower_dot.p + q, 1.0, lower.p + (int64_t)q * V, yp, V, stream);
+  }
   if (timing) hip_check(hipEventRecord(ev[3], stream), "event");
   hip_check(hipGetLastError(), "Heff kernels");
   if (timing) {
@@ -910,6 +915,7 @@ void Ctx::diag(double* d) {
   for (int k = 0; k < S.nk(); k++) blocks[k].out = d + S.k2i[k];
   dblocks.upload(blocks, stream);
   launch_diag(dblocks.p, dterms.p, S.nk(), stream);
+  launch_add_squares(lower.p, n_lower, d, V, stream);  // HeffDiagonal.cpp:621-642
   hip_check(hipGetLastError(), "diag kernel");
   hip_check(hipStreamSynchronize(stream), "diag");
 }
@@ -2034,6 +2040,26 @@ int chm_heff_apply_f32(chm_ctx* ctx, const double* x, double* y, int on_device) 
       throw;
     }
     c.mv_low = false;
+  });
+}
+
+int chm_heff_set_lower(chm_ctx* ctx, int n_lower, const double* veff, int on_device) {
+  return guard(ctx, [&](Ctx& c) {
+    if (c.site < 0) throw Error{CHM_ERR_STATE, "no prepared site"};
+    if (n_lower < 0 || (n_lower > 0 && !veff)) throw Error{CHM_ERR_ARG, "bad lower-state vectors"};
+    (void)hipSetDevice(c.device);
+    c.n_lower = 0;
+    if (n_lower == 0) return;
+    const int64_t n = (int64_t)n_lower * c.V;
+    c.lower.alloc((size_t)n);
+    chm::copy_sync(c.stream, c.lower.p, veff, (size_t)n * sizeof(double),
+                   on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, "lower-state upload");
+    std::vector<const double*> ptrs(n_lower);
+    for (int q = 0; q < n_lower; q++) ptrs[q] = c.lower.p + (int64_t)q * c.V;
+    c.lower_ptrs.upload(ptrs, c.stream);
+    c.lower_part.alloc((size_t)chm::dot_parts(c.V) * n_lower);
+    c.lower_dot.alloc((size_t)n_lower);
+    c.n_lower = n_lower;
   });
 }
 

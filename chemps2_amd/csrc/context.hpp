@@ -136,6 +136,13 @@ struct Ctx : HostState {
   bool M_split = false;              // storage of Mvals at the prepared site
   int64_t M_nnz = 0;
   bool mv_low = false;               // apply() reads the hi floats only
+  // Lower states' projector vectors (VeffTilde, already scaled by sqrt(E_shift)) for the
+  // prepared site: apply() adds sum_q (v_q . x) v_q, diag() adds sum_q v_q^2. Cleared by
+  // prepare().
+  int n_lower = 0;
+  DBuf<double> lower;                // n_lower x V
+  DBuf<const double*> lower_ptrs;
+  DBuf<double> lower_part, lower_dot;
   double mixed_switch = 1e-5;
   int64_t n_low_mults = 0, n_full_mults = 0;
   // Learned preconditioner (learned.hip): precond_mode 1 multiplies the diagonal

@@ -884,6 +884,20 @@ __global__ __launch_bounds__(256) void k_axpy(double a, const double* __restrict
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) y[i] += a * x[i];
 }
 
+// d[i] += sum over the nv vectors of v[q * n + i]^2, vectors in order (the diagonal of the
+// lower states' projector terms, HeffDiagonal.cpp:621-642).
+__global__ __launch_bounds__(256) void k_add_squares(const double* __restrict__ v, int nv, double* __restrict__ d,
+                                                     int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double a = d[i];
+    for (int q = 0; q < nv; q++) {
+      const double w = v[(int64_t)q * n + i];
+      a += w * w;
+    }
+    d[i] = a;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_scale(double a, double* __restrict__ x, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) x[i] *= a;
 }
@@ -1212,6 +1226,10 @@ void launch_axpy_dev(const double* coef, double sign, const double* x, double* y
 
 void launch_axpy(double a, const double* x, double* y, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(256), 0, s, a, x, y, n);
+}
+
+void launch_add_squares(const double* v, int nv, double* d, int64_t n, hipStream_t s) {
+  if (nv > 0 && n > 0) hipLaunchKernelGGL(k_add_squares, dim3(grid_for(n)), dim3(256), 0, s, v, nv, d, n);
 }
 
 void launch_scale(double a, double* x, int64_t n, hipStream_t s) {

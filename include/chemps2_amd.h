@@ -156,6 +156,17 @@ int chm_heff_prepare(chm_ctx* ctx, int site);
 int chm_heff_veclen(const chm_ctx* ctx, int64_t* V);
 /* y = Heff x in the symmetric convention (Heff::makeHeff, Heff.cpp:43-248). */
 int chm_heff_apply(chm_ctx* ctx, const double* x, double* y, int on_device);
+/* Lower states for excited-state DMRG at the prepared site: n_lower vectors of length V
+ * (row-major, n_lower x V, in the block order and symmetric convention of the x of
+ * chm_heff_apply), each the reference's VeffTilde
+ * of one lower state (DMRG::calcVeffTilde, DMRGtechnics.cpp:540-620: the lower state's
+ * two-site tensor contracted with the overlap operators, already scaled by
+ * sqrt(E_shift)). chm_heff_apply then adds sum_q (v_q . x) v_q
+ * (Heff::addDiagramExcitations, HeffDiagrams1.cpp:65-86) and chm_heff_diag adds
+ * sum_q v_q^2 (Heff::addDiagonalExcitations, HeffDiagonal.cpp:621-642); chm_davidson
+ * uses both. The vectors are copied; n_lower = 0 removes them; chm_heff_prepare clears
+ * them. */
+int chm_heff_set_lower(chm_ctx* ctx, int n_lower, const double* veff, int on_device);
 /* Heff forms (no reference counterpart; the reference computes in FP64 only).
  * CHM_FORM_FP64: every product in FP64. CHM_FORM_MIXED: the explicit sector matrix is
  * stored as float pairs (hi = fl32(v), lo = fl32(v - hi)); chm_davidson iterates with
