@@ -87,14 +87,23 @@ def cpu_baseline(L, D, site, seed, irr, T, V, x, y_gpu, budget_s):
         step *= 2
     a, b = k2i[0], k2i[k1]
     err = float(np.abs(ys[a:b] - y_gpu[a:b]).max() / max(1e-300, np.abs(ys[a:b]).max()))
-    log(f"cpu: {k1}/{nk} output blocks in {secs:.1f}s, {flops:.3e} flop; GPU vs oracle on the sample: {err:.2e}")
+    # a whole step cheaper than the budget: repeat whole steps until it is spent
+    passes = 1
+    if k1 == nk:
+        while secs < budget_s:
+            _, f, s = o.apply_range(x, 0, nk)
+            flops += f
+            secs += s
+            passes += 1
+    log(f"cpu: {k1}/{nk} output blocks x {passes} in {secs:.1f}s, {flops:.3e} flop; GPU vs oracle on the sample: {err:.2e}")
+    what = (f"{passes} whole Heff x steps ({nk} output blocks)" if k1 == nk else
+            f"Heff x on output blocks 0..{k1 - 1} of {nk}")
     return {
         "value": flops / secs / 1e12,
         "unit": "TFLOP/s",
         "cores": threads(),
         "kind": "port",
-        "sample": f"oracle Heff x on output blocks 0..{k1 - 1} of {nk} at the same site and data "
-                  f"({flops:.3e} of the step's GEMM flops, {secs:.1f} s, {threads()} threads)",
+        "sample": f"oracle {what} at the same site and data ({flops:.3e} flop, {secs:.1f} s, {threads()} threads)",
     }, err
 
 
