@@ -6,7 +6,8 @@ Bars: Heff x and the diagonal within 1e-12 relative of the plain product plus th
 terms; removing the vectors gives the plain product bit for bit; and, fixed by the
 mathematics, the Davidson on Heff + E_shift w0 w0^T (w0 the ground state of the dense
 Heff built column by column with the oracle) converges to the dense Heff's second
-eigenvalue within 1e-9.
+eigenvalue within 1e-9, in the FP64 and the mixed form and with the learned
+preconditioner.
 """
 import numpy as np
 import pytest
@@ -73,24 +74,36 @@ def test_lower_terms_apply_and_diag(ctx, oracle_lib, case):
         ctx.set_lower(np.zeros((1, V + 1)))
 
 
-def test_lower_state_davidson_gives_second_eigenvalue(ctx, oracle_lib):
+@pytest.mark.parametrize("form,learned", [("fp64", False), ("mixed", False), ("mixed", True)])
+def test_lower_state_davidson_gives_second_eigenvalue(oracle_lib, form, learned):
+    import os
+
     o = d2h_case(oracle_lib, 3)
-    V = load_site(ctx, o, 3)
-    H = np.empty((V, V))
-    e = np.zeros(V)
-    for i in range(V):
-        e[i] = 1.0
-        H[:, i] = o.apply(e)[0]
-        e[i] = 0.0
-    ev, vec = np.linalg.eigh(0.5 * (H + H.T))
-    print(f"V={V}: E0 {ev[0]!r} E1 {ev[1]!r}", flush=True)
-    assert ev[1] - ev[0] > 1e-4
-    shift = 2.0 * (ev[1] - ev[0]) + 1.0
-    ctx.set_lower(np.sqrt(shift) * vec[:, :1].T)
-    guess = np.random.default_rng(2).normal(size=V)
-    e1, nm = ctx.davidson(guess, 1e-10)
-    print(f"Davidson with the lower state: {e1!r} ({nm} products)", flush=True)
-    assert abs(e1 - ev[1]) <= 1e-9
-    ctx.set_lower(None)
-    e0, _ = ctx.davidson(np.random.default_rng(3).normal(size=V), 1e-10)
-    assert abs(e0 - ev[0]) <= 1e-9
+    c = chemps2_amd.Context(0)
+    try:
+        if form == "mixed":
+            c.set_heff_form(chemps2_amd.FORM_MIXED)
+        if learned:
+            path = os.path.join(os.path.dirname(chemps2_amd.__file__), "data", "precond_mlp.npy")
+            c.mlp_set(np.load(path, allow_pickle=False))
+            c.precond_mode(chemps2_amd.PRECOND_LEARNED)
+        V = load_site(c, o, 3)
+        H = np.empty((V, V))
+        e = np.zeros(V)
+        for i in range(V):
+            e[i] = 1.0
+            H[:, i] = o.apply(e)[0]
+            e[i] = 0.0
+        ev, vec = np.linalg.eigh(0.5 * (H + H.T))
+        print(f"{form} learned={learned} V={V}: E0 {ev[0]!r} E1 {ev[1]!r}", flush=True)
+        assert ev[1] - ev[0] > 1e-4
+        shift = 2.0 * (ev[1] - ev[0]) + 1.0
+        c.set_lower(np.sqrt(shift) * vec[:, :1].T)
+        e1, nm = c.davidson(np.random.default_rng(2).normal(size=V), 1e-10)
+        print(f"Davidson with the lower state: {e1!r} ({nm} products)", flush=True)
+        assert abs(e1 - ev[1]) <= 1e-9
+        c.set_lower(None)
+        e0, _ = c.davidson(np.random.default_rng(3).normal(size=V), 1e-10)
+        assert abs(e0 - ev[0]) <= 1e-9
+    finally:
+        c.close()
