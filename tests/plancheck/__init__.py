@@ -31,6 +31,7 @@ def lib():
             "pc_sectors": (None, [vp, i, i, _ip, _ip, _ip, _ip]),
             "pc_sectors_auto": (None, [vp, i]),
             "pc_nsectors": (i, [vp, i, C.c_void_p]),
+            "pc_fci": (i, [vp, i, C.c_void_p]),
             "pc_tensor": (i, [vp, i, i, i, i, i, i, i, i, C.POINTER(C.c_long)]),
             "pc_synthetic_set": (i, [vp, i, i]),
             "pc_plan": (lg, [vp, i]),

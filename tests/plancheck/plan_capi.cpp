@@ -57,6 +57,15 @@ int pc_nsectors(void* h, int b, int* out) {
   return (int)v.size();
 }
 
+// FCI dimensions of boundary b that auto_sectors derived (N, 2S, I, dim per sector).
+int pc_fci(void* h, int b, int* out) {
+  PC* p = (PC*)h;
+  const auto& v = p->c.fci[b];
+  if (out)
+    for (size_t q = 0; q < v.size(); q++) { out[4 * q] = v[q].N; out[4 * q + 1] = v[q].S; out[4 * q + 2] = v[q].I; out[4 * q + 3] = v[q].d; }
+  return (int)v.size();
+}
+
 // Registers a tensor; returns its buffer id and writes its size.
 int pc_tensor(void* h, int cnt, int mr, int kind, int k, int m, int two_j, int n_elec, int n_irrep, long* size) {
   PC* p = (PC*)h;
