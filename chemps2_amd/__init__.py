@@ -25,6 +25,7 @@ EXPORTS = [
     "chm_set_heff_form", "chm_heff_apply_f32", "chm_mixed_stats", "chm_heff_set_lower",
     "chm_precond_mode", "chm_mlp_nparams", "chm_mlp_set", "chm_mlp_get", "chm_mlp_forward", "chm_mlp_loss_grad",
     "chm_mlp_train", "chm_harvest", "chm_harvest_take", "chm_dmrg_sweep_range",
+    "chm_dmrg_new_excitation", "chm_dmrg_nstates",
 ]
 PRECOND_DIAGONAL, PRECOND_LEARNED = 0, 1
 MLP_IN, MLP_H = 8, 128
@@ -68,6 +69,8 @@ def load():
         "chm_s_set": (i, [vp, vp, i64, i]),
         "chm_split": (i, [vp, i, i, i, i, C.POINTER(d)]),
         "chm_dmrg_init": (i, [vp, i, C.c_uint64]),
+        "chm_dmrg_new_excitation": (i, [vp, d, i, C.c_uint64]),
+        "chm_dmrg_nstates": (i, [vp, C.POINTER(i)]),
         "chm_dmrg_solve": (i, [vp, i, _ip, _dp, _ip, _dp, _dp, C.POINTER(d)]),
         "chm_dmrg_sweep": (i, [vp, i, i, d, d, i, C.POINTER(d)]),
         "chm_get_problem": (i, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
@@ -207,12 +210,31 @@ def _ptr(a):
     return a.data_ptr()
 
 
+def _dev(t, device, n, what):
+    """Device address of a torch tensor handed to an on_device call: float64, contiguous,
+    on the context's GPU and holding at least n elements. Work already queued on torch's
+    current stream is waited for, since the library reads on its own stream."""
+    import torch
+
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{what}: a numpy array or a torch tensor is required")
+    if t.dtype != torch.float64:
+        raise TypeError(f"{what}: float64 required, got {t.dtype}")
+    if not t.is_cuda or t.device.index != device:
+        raise ValueError(f"{what}: the tensor must live on cuda:{device}, not {t.device}")
+    if not t.is_contiguous() or t.numel() < n:
+        raise ValueError(f"{what}: a contiguous tensor of at least {n} elements is required")
+    torch.cuda.current_stream(t.device).synchronize()
+    return t.data_ptr()
+
+
 class Context:
     """One library context on one GPU (chm_ctx)."""
 
     def __init__(self, device=0):
         self.lib = load()
         self.h = C.c_void_p()
+        self.device = int(device)
         rc = self.lib.chm_ctx_create(int(device), C.byref(self.h))
         if rc != 0:
             raise ChmError(f"chm_ctx_create failed ({rc}): no usable HIP device {device}")
@@ -336,6 +358,16 @@ class Context:
         self._check(self.lib.chm_get_problem(self.h, *[C.byref(x) for x in v]), "chm_get_problem")
         return dict(zip(["L", "N", "two_s", "irrep", "nirrep"], [x.value for x in v]))
 
+    def new_excitation(self, eshift, D, seed=2):
+        """DMRG::newExcitation: keep the current state as a lower state with level shift
+        eshift and start the next one (SyBookkeeper dimensions at D, random MPS)."""
+        self._check(self.lib.chm_dmrg_new_excitation(self.h, float(eshift), int(D), int(seed)), "chm_dmrg_new_excitation")
+
+    def nstates(self):
+        n = C.c_int(0)
+        self._check(self.lib.chm_dmrg_nstates(self.h, C.byref(n)), "chm_dmrg_nstates")
+        return n.value
+
     def dmrg_solve(self, scheme):
         """scheme: list of (D, econv, max_sweeps, noise, rtol), as ConvergenceScheme."""
         D = np.array([s[0] for s in scheme], dtype=np.int32)
@@ -388,7 +420,10 @@ class Context:
             y = np.zeros(self.V) if y is None else y
             self._check(fn(self.h, _ptr(x), _ptr(y), 0), name)
             return y
-        self._check(fn(self.h, _ptr(x), _ptr(y), 1), name)
+        if y is None:
+            raise ValueError("a device x needs a device y of the same length")
+        xp, yp = _dev(x, self.device, self.V, "x"), _dev(y, self.device, self.V, "y")
+        self._check(fn(self.h, xp, yp, 1), name)
         return y
 
     def set_lower(self, veff):
@@ -403,9 +438,10 @@ class Context:
                 raise ValueError(f"lower-state vectors of length {v.shape[1]}, the site has V = {self.V}")
             self._check(self.lib.chm_heff_set_lower(self.h, v.shape[0], _ptr(v), 0), "chm_heff_set_lower")
             return
-        if veff.dim() != 2 or veff.shape[1] != self.V or not veff.is_contiguous():
-            raise ValueError("lower-state vectors must be a contiguous n_lower x V tensor")
-        self._check(self.lib.chm_heff_set_lower(self.h, veff.shape[0], _ptr(veff), 1), "chm_heff_set_lower")
+        if veff.dim() != 2 or veff.shape[1] != self.V:
+            raise ValueError("lower-state vectors must be an n_lower x V tensor")
+        vp = _dev(veff, self.device, veff.shape[0] * self.V, "lower-state vectors")
+        self._check(self.lib.chm_heff_set_lower(self.h, veff.shape[0], vp, 1), "chm_heff_set_lower")
 
     def set_heff_form(self, form):
         """FORM_FP64 or FORM_MIXED; applies from the next prepare."""
@@ -473,8 +509,13 @@ class Context:
         """x: program-convention guess (numpy, updated in place). Returns (eigenvalue, n_matvec)."""
         e = C.c_double(0.0)
         nm = C.c_int(0)
-        on_dev = 0 if isinstance(x, np.ndarray) else 1
-        self._check(self.lib.chm_davidson(self.h, _ptr(x), on_dev, float(rtol), C.byref(e), C.byref(nm)), "chm_davidson")
+        if isinstance(x, np.ndarray):
+            if x.dtype != np.float64 or not x.flags.c_contiguous or x.size != self.V:
+                raise ValueError(f"x must be a contiguous float64 array of length V = {self.V}")
+            on_dev, xp = 0, _ptr(x)
+        else:
+            on_dev, xp = 1, _dev(x, self.device, self.V, "x")
+        self._check(self.lib.chm_davidson(self.h, xp, on_dev, float(rtol), C.byref(e), C.byref(nm)), "chm_davidson")
         return e.value, nm.value
 
     def flops(self):

@@ -364,6 +364,7 @@ struct DevSvd {
   double* U;
   double* VT;
   double* S;
+  int* info;  // set to 1 when the sweeps end before every pair is orthogonal
   int m, n;
   int wide;
   int pad;

@@ -53,6 +53,8 @@ void Ctx::set_problem(int L_, int group_, const int* irr, int N_, int TwoS_, int
   Sdev.release();
   s_buf = -1;
   s_site = -1;
+  lowers.clear();
+  low_buf = sup_buf = -1;
   sets.clear();
   sets.resize(L - 1);
   mps.assign(L, TLay());
@@ -1366,6 +1368,7 @@ void Ctx::update_operators(int index, bool mr) {
   for (auto& row : o.D) for (OpT* t : row) add_outs(t, out2);
   run_terms(up.segs[1], out1, arena.p);
   run_terms(up.segs[2], out2, arena.p);
+  update_overlaps(index, mr);  // DMRGoperators.cpp:555-567, 888-900
   const size_t ncombos = up.combos.size();
   reap(std::move(upp));
   site = -1;
@@ -1893,6 +1896,7 @@ int chm_dmrg_init(chm_ctx* ctx, int D, uint64_t seed) {
     if (D < 1) throw Error{CHM_ERR_ARG, "D must be positive"};
     (void)hipSetDevice(c.device);
     const double t0 = chm::now_s();
+    c.drop_lowers();  // a new DMRG object has no excitations
     chm::auto_sectors(c, D);
     c.mps_random(seed);
     const double t1 = chm::now_s();
@@ -1902,6 +1906,24 @@ int chm_dmrg_init(chm_ctx* ctx, int D, uint64_t seed) {
     for (double& t : c.tm) t = 0.0;
     c.mv_flops = c.mv_ms = 0.0;
   });
+}
+
+int chm_dmrg_new_excitation(chm_ctx* ctx, double eshift, int D, uint64_t seed) {
+  return guard(ctx, [&](Ctx& c) {
+    if (!c.have_problem) throw Error{CHM_ERR_STATE, "set the problem first"};
+    if (D < 1) throw Error{CHM_ERR_ARG, "D must be positive"};
+    (void)hipSetDevice(c.device);
+    c.new_excitation(eshift, D, seed);
+    c.n_matvec_total = 0;
+    for (double& t : c.tm) t = 0.0;
+    c.mv_flops = c.mv_ms = 0.0;
+  });
+}
+
+int chm_dmrg_nstates(const chm_ctx* ctx, int* n) {
+  if (!ctx || !n) return CHM_ERR_ARG;
+  *n = (int)ctx->c.lowers.size() + 1;
+  return CHM_OK;
 }
 
 int chm_get_problem(const chm_ctx* ctx, int* L, int* N, int* two_s, int* irrep, int* nirrep) {
@@ -2014,6 +2036,7 @@ static void apply_abi(Ctx& c, const double* x, double* y, int on_device) {
   (void)hipSetDevice(c.device);
   if (on_device) {
     c.apply(x, y);
+    chm::hip_check(hipStreamSynchronize(c.stream), "apply");
   } else {
     chm::hip_check(hipMemcpyAsync(c.dx.p, x, c.V * sizeof(double), hipMemcpyHostToDevice, c.stream), "x upload");
     c.apply(c.dx.p, c.dy.p);
@@ -2223,6 +2246,7 @@ int chm_heff_diag(chm_ctx* ctx, double* d, int on_device) {
     (void)hipSetDevice(c.device);
     if (on_device) {
       c.diag(d);
+      chm::hip_check(hipStreamSynchronize(c.stream), "diag");
     } else {
       c.diag(c.dy.p);
       chm::copy_sync(c.stream, d, c.dy.p, c.V * sizeof(double), hipMemcpyDeviceToHost, "diag download");
@@ -2238,6 +2262,7 @@ int chm_davidson(chm_ctx* ctx, double* x, int on_device, double rtol, double* ei
     (void)hipSetDevice(c.device);
     if (on_device) {
       *eigenvalue = c.davidson(x, rtol, n_matvec);
+      chm::hip_check(hipStreamSynchronize(c.stream), "davidson");
     } else {
       chm::DBuf<double> buf;
       buf.alloc(c.V);

@@ -69,6 +69,36 @@ struct HostState {
   void validate_set(int cnt, bool mr) const;
 };
 
+// Overlap operator between the current MPS (up) and one lower state (down) at one
+// boundary: TensorO (TensorO.cpp:27-193), the identity operator (two_j = n_elec =
+// n_irrep = 0) with up dimensions from the current sectors and down dimensions from the
+// lower state's; one block per sector where both are non-zero, in the order of the
+// current sector list.
+struct OvlT {
+  int boundary = -1, dir = 0;  // dir 1 built moving right, 2 moving left
+  int32_t buf = -1;
+  int64_t size = 0;
+  std::vector<int> pos2blk;    // position in the current list of the boundary -> block or -1
+  std::vector<int> N, S, I, du, dd;
+  std::vector<int64_t> k2i;
+  Ref blk(const Sectors& cur, int n, int s, int i) const {
+    const int p = cur.idx[boundary]->find(n, s, i);
+    if (p < 0 || p >= (int)pos2blk.size() || pos2blk[p] < 0) return Ref{};
+    return Ref{buf, k2i[pos2blk[p]]};
+  }
+};
+
+// A converged lower state kept for excited-state DMRG (DMRG::newExcitation,
+// DMRG.cpp:475-505): its energy shift, virtual dimensions, MPS on the device and its
+// overlap operators with the current MPS per slot (Exc_Eshifts, Exc_BKs, Exc_MPSs,
+// Exc_Overlaps).
+struct LowerState {
+  double eshift = 0.0;
+  Sectors sec;
+  std::vector<TLay> mps;
+  std::vector<OvlT> ovl;  // [slot cnt] -> boundary cnt + 1
+};
+
 struct Ctx : HostState {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -144,6 +174,17 @@ struct Ctx : HostState {
   DBuf<const double*> lower_ptrs;
   DBuf<double> lower_part, lower_dot;
   double mixed_switch = 1e-5;
+  // Excited states: the lower states, the buffer id of the lower-state vectors and the
+  // lower two-site object joined at each site.
+  std::vector<LowerState> lowers;
+  int32_t low_buf = -1, sup_buf = -1;
+  // device buffer id with at least n doubles (allocated or grown; owned by bufs)
+  double* own_buf(int32_t& id, int64_t n);
+  void new_excitation(double eshift, int D, uint64_t seed);
+  void update_overlaps(int index, bool mr);
+  void veff_tilde(int index);
+  bool ovl_current(const OvlT& o, int dir) const;
+  void drop_lowers();
   int64_t n_low_mults = 0, n_full_mults = 0;
   // Learned preconditioner (learned.hip): precond_mode 1 multiplies the diagonal
   // preconditioner by (1 + m) with m from the MLP; harvest_per_iter > 0 records, at
@@ -243,6 +284,9 @@ struct Ctx : HostState {
   int64_t n_matvec_total = 0;
   uint64_t noise_seed = 1;
   void join(int site);
+  // Sobject::Join of MPS tensors Tl, Tr laid out on sectors s into buffer id out_buf
+  // (layout Sj); the terms run through run_terms.
+  void join_terms(const Sectors& s, const TLay& Tl, const TLay& Tr, int st, const SLayout& Sj, int32_t out_buf);
   // sites first..last in the sweep direction (-1: the end of the chain); a range that
   // does not start at the chain end continues the sweep begun by the previous range
   double sweep(bool moving_right, int D, double rtol, double noise_prefactor, bool change, int first = -1,

@@ -111,23 +111,26 @@ void Ctx::join(int st) {
   Sdev.alloc(std::max<int64_t>(Sj.size(), 1));
   if (s_buf < 0) { s_buf = (int32_t)bufs.size(); bufs.push_back(Sdev.p); buf_len.push_back(Sj.size()); }
   else { bufs[s_buf] = Sdev.p; buf_len[s_buf] = Sj.size(); }
+  join_terms(sec, mps[st], mps[st + 1], st, Sj, s_buf);
+  s_site = st;
+}
+
+void Ctx::join_terms(const Sectors& sc, const TLay& Tl, const TLay& Tr, int st, const SLayout& Sj, int32_t out_buf) {
   const int I1 = irreps[st];
-  const TLay& Tl = mps[st];
-  const TLay& Tr = mps[st + 1];
   std::vector<UTerm> terms;
   std::vector<std::pair<Ref, std::pair<int, int>>> outs;
   for (int ik = 0; ik < Sj.nk(); ik++) {
     const int nl = Sj.NL[ik], sl = Sj.SL[ik], il = Sj.IL[ik], nr = Sj.NR[ik], sr = Sj.SR[ik], ir = Sj.IR[ik];
     const int tj = Sj.TJ[ik], n1 = Sj.N1[ik];
     const int s1 = (n1 == 1) ? 1 : 0, s2 = (Sj.N2[ik] == 1) ? 1 : 0;
-    const int dl = sec.dim(st, nl, sl, il), dr = sec.dim(st + 2, nr, sr, ir);
-    Ref out{s_buf, Sj.k2i[ik]};
+    const int dl = sc.dim(st, nl, sl, il), dr = sc.dim(st + 2, nr, sr, ir);
+    Ref out{out_buf, Sj.k2i[ik]};
     outs.push_back({out, {dl, dr}});
     const int fase = phase(sl + sr + s1 + s2);
     const int nm = nl + n1, im = (s1 == 1) ? (il ^ I1) : il;
     const int lo = std::max(std::abs(sl - s1), std::abs(sr - s2)), hi = std::min(sl + s1, sr + s2);
     for (int sm = lo; sm <= hi; sm += 2) {
-      const int dm = sec.dim(st + 1, nm, sm, im);
+      const int dm = sc.dim(st + 1, nm, sm, im);
       if (dm <= 0) continue;
       UTerm t{};
       t.out = out; t.m = dl; t.n = dr; t.k1 = dm; t.kind = 0;
@@ -139,7 +142,6 @@ void Ctx::join(int st) {
     }
   }
   run_terms(terms, outs, nullptr);
-  s_site = st;
 }
 
 // Sobject::Split (Sobject.cpp:260-622): per centre sector an SVD of the recoupled
@@ -258,13 +260,16 @@ double Ctx::split(int st, int D, bool mr, bool change) {
     launch_copies(d_items.p, (int)items.size(), stream);
     hip_check(hipGetLastError(), "split assembly");
     std::vector<DevSvd> jobs;
-    for (auto& c : cs)
+    for (size_t q = 0; q < cs.size(); q++) {
+      const auto& c = cs[q];
       if (c.cd > 0 && c.jac) {
         DevSvd j{};
         j.A = M.p + c.moff; j.Q = Qw.p + c.qoff; j.U = U.p + c.uoff; j.VT = VT.p + c.voff; j.S = Sg.p + c.soff;
+        j.info = info.p + q;
         j.m = std::max(c.dLt, c.dRt); j.n = c.cd; j.wide = c.dLt < c.dRt;
         jobs.push_back(j);
       }
+    }
     DBuf<DevSvd> d_jobs;
     d_jobs.upload(jobs, stream);
     launch_svd_jacobi(d_jobs.p, (int)jobs.size(), stream);
@@ -399,6 +404,7 @@ double Ctx::solve_site(int index, double rtol, double noise, int D, bool mr, boo
   double t1 = now();
   tm[0] += t1 - t0;
   prepare(index);
+  veff_tilde(index);  // lower states' projector vectors (DMRG.cpp:434, calcVeffTilde)
   if (plan_next >= 0 && !change) {
     // dimensions stay fixed in this sweep: lay out this step's new operator slot and
     // generate the next site's work lists while the GPU solves this one

@@ -970,10 +970,13 @@ __global__ __launch_bounds__(256) void k_copies(const DevCopy* __restrict__ item
 // One-sided (Hestenes) Jacobi SVD of the small centre-sector matrices of Split, one
 // workgroup per matrix. Each round rotates a set of disjoint column pairs (round-robin
 // tournament order), every pair on a group of SVD_TPP lanes; sweeps end when no pair
-// has |a_p . a_q| > SVD_TOL |a_p| |a_q|.
+// has |a_p . a_q| > sqrt(m) eps |a_p| |a_q|, the rounding floor of an m-term dot product
+// (the stopping rule of one-sided Jacobi in Drmac & Veselic, SIAM J. Matrix Anal. Appl.
+// 29 (2008) 1322, as LAPACK's dgesvj uses it). A job that reaches SVD_MAX_SWEEPS without
+// meeting it sets *info = 1.
 constexpr int SVD_TPP = 16;
 constexpr int SVD_GROUPS = 256 / SVD_TPP;
-constexpr double SVD_TOL = 1e-15;
+constexpr double SVD_EPS = 1.1102230246251565e-16;  // 2^-53
 constexpr int SVD_MAX_SWEEPS = 80;
 
 __device__ inline double group_sum(double v) {
@@ -993,6 +996,8 @@ __global__ __launch_bounds__(256) void k_svd_jacobi(const DevSvd* __restrict__ i
   for (int e = tid; e < n * n; e += 256) Q[e] = (e % n == e / n) ? 1.0 : 0.0;
   __syncthreads();
   const int n2 = n + (n & 1);
+  const double tol = sqrt((double)m) * SVD_EPS;
+  bool converged = n <= 1;
   for (int sweep = 0; sweep < SVD_MAX_SWEEPS && n > 1; sweep++) {
     if (tid == 0) rotated = 0;
     __syncthreads();
@@ -1013,7 +1018,7 @@ __global__ __launch_bounds__(256) void k_svd_jacobi(const DevSvd* __restrict__ i
             al += x * x; be += y * y; ga += x * y;
           }
         al = group_sum(al); be = group_sum(be); ga = group_sum(ga);
-        if (p >= 0 && ga != 0.0 && al > 0.0 && be > 0.0 && fabs(ga) > SVD_TOL * sqrt(al * be)) {
+        if (p >= 0 && ga != 0.0 && al > 0.0 && be > 0.0 && fabs(ga) > tol * sqrt(al * be)) {
           const double zeta = (be - al) / (2.0 * ga);
           const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
           const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
@@ -1035,8 +1040,12 @@ __global__ __launch_bounds__(256) void k_svd_jacobi(const DevSvd* __restrict__ i
     __syncthreads();
     const int any = rotated;
     __syncthreads();
-    if (!any) break;
+    if (!any) {
+      converged = true;
+      break;
+    }
   }
+  if (tid == 0 && it.info) *it.info = converged ? 0 : 1;
   // singular values and their descending order
   for (int j = g; j < n; j += SVD_GROUPS) {
     double s2 = 0.0;

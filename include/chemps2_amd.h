@@ -16,7 +16,12 @@
  *  - Dense blocks are column-major FP64, in exactly the storage order of the
  *    reference objects they replace (TensorOperator.cpp:29-102, Sobject.cpp:36-147).
  *  - Pointers are host pointers unless an on_device flag says otherwise; device
- *    pointers must be valid on the context's device.
+ *    pointers must be valid on the context's device and hold FP64 data of the stated
+ *    length. The library reads and writes them on its own stream (chm_stream), which
+ *    is not ordered against the caller's streams: the caller finishes the work that
+ *    writes an input before the call (or makes chm_stream wait on it), and reads an
+ *    output only after the call returned (every on_device call synchronises its stream
+ *    before returning).
  *  - A context is used from one host thread at a time.
  */
 #ifndef CHEMPS2_AMD_H
@@ -127,6 +132,18 @@ int chm_dmrg_init(chm_ctx* ctx, int D, uint64_t seed);
 int chm_dmrg_presolve(chm_ctx* ctx);
 int chm_dmrg_solve(chm_ctx* ctx, int n, const int* D, const double* econv, const int* max_sweeps, const double* noise,
                    const double* rtol, double* energy);
+/* Excited states (DMRG::activateExcitations / newExcitation, DMRG.cpp:464-505): the
+ * current MPS and its virtual dimensions become the next lower state, with level shift
+ * eshift (> 0); a new state starts from SyBookkeeper(Prob, D) dimensions and a random
+ * MPS from `seed`, and PreSolve builds its operators together with the overlap operators
+ * TensorO to every lower state (TensorO.cpp:39-193, DMRGoperators.cpp:555-567, 888-900).
+ * Every later chm_dmrg_solve / sweep then adds, per two-site step, the projector
+ * sum_q |VeffTilde_q><VeffTilde_q| of the lower states (DMRG::calcVeffTilde,
+ * DMRGtechnics.cpp:540-620) to Heff, so that it converges to the next state
+ * (H + sum_q eshift_q |q><q|). The lower states stay in HBM until chm_dmrg_init.
+ * chm_dmrg_nstates: the number of states (lower states + 1). */
+int chm_dmrg_new_excitation(chm_ctx* ctx, double eshift, int D, uint64_t seed);
+int chm_dmrg_nstates(const chm_ctx* ctx, int* n);
 /* One half sweep, DMRG::sweepleft (moving_right = 0) or DMRG::sweepright
  * (moving_right = 1) of DMRG.cpp:354-417 with one instruction's D, Davidson tolerance
  * and noise prefactor; change = 0 keeps the virtual dimensions (the reference's first

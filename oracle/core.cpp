@@ -65,9 +65,24 @@ double wigner9j(int a, int b, int c, int d, int e, int f, int g, int h, int i) {
 }
 
 // ---------------------------------------------------------------- dgemm ----
+// With ORACLE_BLAS the products go to the host BLAS (the reference's own choice,
+// Lapack.h:23-39) once they are large enough to gain from it; the loops below are the
+// fallback and the path of small products.
+#ifdef ORACLE_BLAS
+extern "C" void dgemm_(const char* ta, const char* tb, const int* m, const int* n, const int* k, const double* alpha,
+                       const double* A, const int* lda, const double* B, const int* ldb, const double* beta, double* C,
+                       const int* ldc);
+#endif
+
 void dgemm(char ta, char tb, int m, int n, int k, double alpha, const double* A, int lda, const double* B,
            int ldb, double beta, double* C, int ldc) {
   if (m <= 0 || n <= 0) return;
+#ifdef ORACLE_BLAS
+  if (k > 0 && (double)m * n * k >= 4096.0) {
+    dgemm_(&ta, &tb, &m, &n, &k, &alpha, A, &lda, B, &ldb, &beta, C, &ldc);
+    return;
+  }
+#endif
   for (int j = 0; j < n; j++) {
     double* c = C + (size_t)ldc * j;
     if (beta == 0.0) {

@@ -30,6 +30,8 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB):
             build()
+        # the oracle parallelises over blocks with OpenMP; MKL runs sequentially inside
+        os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
         l = C.CDLL(_LIB)
         vp, i, d, L_ = C.c_void_p, C.c_int, C.c_double, C.c_long
         sig = {
