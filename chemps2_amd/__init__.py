@@ -13,6 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libchemps2_amd.so")
+TUNING_PATH = os.path.join(HERE, "data", "autotune.json")
 
 KINDS = {"L": 0, "F0": 1, "F1": 2, "S0": 3, "S1": 4, "A": 5, "B": 6, "C": 7, "D": 8, "Q": 9, "X": 10}
 EXPORTS = [
@@ -25,7 +26,8 @@ EXPORTS = [
     "chm_set_heff_form", "chm_heff_apply_f32", "chm_mixed_stats", "chm_heff_set_lower",
     "chm_precond_mode", "chm_mlp_nparams", "chm_mlp_set", "chm_mlp_get", "chm_mlp_forward", "chm_mlp_loss_grad",
     "chm_mlp_train", "chm_harvest", "chm_harvest_take", "chm_dmrg_sweep_range",
-    "chm_dmrg_new_excitation", "chm_dmrg_nstates",
+    "chm_dmrg_new_excitation", "chm_dmrg_nstates", "chm_dmrg_site_log",
+    "chm_set_heff_knobs", "chm_set_tuning", "chm_tuning_used", "chm_precond_stats",
 ]
 PRECOND_DIAGONAL, PRECOND_LEARNED = 0, 1
 MLP_IN, MLP_H = 8, 128
@@ -71,6 +73,11 @@ def load():
         "chm_dmrg_init": (i, [vp, i, C.c_uint64]),
         "chm_dmrg_new_excitation": (i, [vp, d, i, C.c_uint64]),
         "chm_dmrg_nstates": (i, [vp, C.POINTER(i)]),
+        "chm_dmrg_site_log": (i, [vp, C.POINTER(i), vp, vp, vp]),
+        "chm_set_heff_knobs": (i, [vp, i64, i, i, i]),
+        "chm_set_tuning": (i, [vp, i, vp]),
+        "chm_tuning_used": (i, [vp, vp]),
+        "chm_precond_stats": (i, [vp, C.POINTER(i64)]),
         "chm_dmrg_solve": (i, [vp, i, _ip, _dp, _ip, _dp, _dp, C.POINTER(d)]),
         "chm_dmrg_sweep": (i, [vp, i, i, d, d, i, C.POINTER(d)]),
         "chm_get_problem": (i, [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i)]),
@@ -231,13 +238,20 @@ def _dev(t, device, n, what):
 class Context:
     """One library context on one GPU (chm_ctx)."""
 
-    def __init__(self, device=0):
+    def __init__(self, device=0, tuning=True):
+        """tuning: install the autotuner's table (data/autotune.json, written by
+        chemps2_amd.autotune on the GPU) when it exists."""
         self.lib = load()
         self.h = C.c_void_p()
         self.device = int(device)
         rc = self.lib.chm_ctx_create(int(device), C.byref(self.h))
         if rc != 0:
             raise ChmError(f"chm_ctx_create failed ({rc}): no usable HIP device {device}")
+        if tuning and os.path.exists(TUNING_PATH):
+            import json
+
+            with open(TUNING_PATH) as f:
+                self.set_tuning(json.load(f)["rows"])
 
     def _check(self, rc, what):
         if rc != 0:
@@ -399,6 +413,36 @@ class Context:
         self._check(self.lib.chm_dmrg_heff_stats(self.h, C.byref(f), C.byref(ms)), "chm_dmrg_heff_stats")
         return f.value, ms.value
 
+    def set_heff_knobs(self, explicit_max=-1, item_terms=0, group_terms=0, chain_mfma=-1):
+        self._check(self.lib.chm_set_heff_knobs(self.h, int(explicit_max), int(item_terms), int(group_terms),
+                                                int(chain_mfma)), "chm_set_heff_knobs")
+
+    def set_tuning(self, rows):
+        """Autotuner table: rows of (dmax_lo, dmax_hi, V_lo, V_hi, explicit_max, item_terms,
+        group_terms, chain_mfma)."""
+        a = np.ascontiguousarray(np.asarray(rows, dtype=np.int64).reshape(-1, 8))
+        self._check(self.lib.chm_set_tuning(self.h, a.shape[0], a.ctypes.data if a.size else None), "chm_set_tuning")
+
+    def tuning_used(self):
+        out = np.zeros(6, dtype=np.int64)
+        self._check(self.lib.chm_tuning_used(self.h, out.ctypes.data), "chm_tuning_used")
+        return dict(zip(["row", "dmax", "explicit_max", "item_terms", "group_terms", "chain_mfma"], out.tolist()))
+
+    def site_log(self):
+        """Per two-site step since the last read: list of dicts (site, energy, products,
+        seconds, F_H, V, heff_ms); clears the records."""
+        n = C.c_int(0)
+        self._check(self.lib.chm_dmrg_site_log(self.h, C.byref(n), None, None, None), "chm_dmrg_site_log")
+        sites = np.zeros(n.value, dtype=np.int32)
+        prods = np.zeros(n.value, dtype=np.int32)
+        vals = np.zeros(5 * n.value)
+        if n.value:
+            self._check(self.lib.chm_dmrg_site_log(self.h, C.byref(n), sites.ctypes.data, prods.ctypes.data,
+                                                   vals.ctypes.data), "chm_dmrg_site_log")
+        v = vals.reshape(-1, 5)
+        return [{"site": int(sites[q]), "energy": float(v[q, 0]), "products": int(prods[q]), "seconds": float(v[q, 1]),
+                 "F_H": float(v[q, 2]), "V": int(v[q, 3]), "heff_ms": float(v[q, 4])} for q in range(n.value)]
+
     def dmrg_stats(self):
         t = (C.c_double * 6)()
         n = C.c_int64(0)
@@ -450,6 +494,12 @@ class Context:
     # ---- learned preconditioner
     def precond_mode(self, mode):
         self._check(self.lib.chm_precond_mode(self.h, int(mode)), "chm_precond_mode")
+
+    def precond_fallbacks(self):
+        """Davidson solves whose learned correction was switched off by the residual guard."""
+        n = C.c_int64(0)
+        self._check(self.lib.chm_precond_stats(self.h, C.byref(n)), "chm_precond_stats")
+        return n.value
 
     def mlp_set(self, params):
         p = np.ascontiguousarray(params, dtype=np.float32)

@@ -210,6 +210,26 @@ void Ctx::alloc_slot(int index, bool mr) {
   if (prepared >= 0 && index != prepared - 1 && index != prepared + 1) site = prepared;
 }
 
+void Ctx::apply_tuning() {
+  site_dmax = 0;
+  for (int k = 0; k < S.nk(); k++) site_dmax = std::max<int64_t>(site_dmax, std::max(S.dL[k], S.dR[k]));
+  explicit_max = def_explicit_max;
+  chain_item_terms = def_item_terms;
+  chain_group_terms = def_group_terms;
+  use_chain_mfma = def_chain_mfma;
+  tune_row = -1;
+  for (size_t r = 0; r < tune.size(); r++) {
+    const TuneRow& t = tune[r];
+    if (site_dmax < t.dmax_lo || site_dmax > t.dmax_hi || V < t.v_lo || V > t.v_hi) continue;
+    if (t.explicit_max >= 0) explicit_max = t.explicit_max;
+    if (t.item_terms > 0) chain_item_terms = (int)t.item_terms;
+    if (t.group_terms > 0) chain_group_terms = (int)t.group_terms;
+    if (t.chain_mfma >= 0) use_chain_mfma = t.chain_mfma != 0;
+    tune_row = (int)r;
+    break;
+  }
+}
+
 void Ctx::prepare(int st) {
   const double t_start = now_s();
   if (!have_problem) throw Error{ERR_STATE, "set the problem first"};
@@ -242,6 +262,7 @@ void Ctx::prepare(int st) {
   }
   V = S.size();
   if (V <= 0) throw Error{ERR_STATE, "empty two-site object"};
+  apply_tuning();
   F_H = plan->flops;
   const double t_built = now_s();
 
@@ -443,6 +464,7 @@ void Ctx::prepare(int st) {
     nbitems = (int)items.size();
     nbreds = (int)reds.size();
     bpart.alloc(std::max<int64_t>(poff, 1));
+    bpart_need = poff;
     if (tot.pairs > 0) {
       d_pairs.alloc(tot.pairs);
       d_kterms.alloc(tot.kt);
@@ -653,6 +675,7 @@ void Ctx::prepare(int st) {
     up(creds, hcr, n_cr);
     ncreds = (int)n_cr;
     cpart.alloc(std::max<int64_t>(cpart_n, 1));
+    cpart_need = cpart_n;
     chain_P_l = std::min((PmaxL + 31) / 32 * 32, CHAIN_MAXD * CHAIN_MAXD);
     stats[15] = n_ct;
     chain.swap(rest);
@@ -878,10 +901,14 @@ void Ctx::prepare(int st) {
   site = st;
 }
 
-void Ctx::apply(const double* xp, double* yp) {
+void Ctx::apply(const double* xp, double* yp) { apply_with(xp, yp, bpart.p, cpart.p, lower_part.p, lower_dot.p); }
+
+// The product with the given partial-sum scratch: bp (explicit rows split over work
+// items), cp (chained blocks split over work items), lp / ld (lower-state dots).
+void Ctx::apply_with(const double* xp, double* yp, double* bp, double* cp, double* lp, double* ld) {
   if (site < 0) throw Error{ERR_STATE, "no prepared site"};
   if (timing) hip_check(hipEventRecord(ev[0], stream), "event");
-  launch_bspmv(bitems.p, nbitems, breds.p, nbreds, bpart.p, Mvals.p, M_nnz, M_split ? (mv_low ? 1 : 2) : 0, Mcol.p, xp,
+  launch_bspmv(bitems.p, nbitems, breds.p, nbreds, bp, Mvals.p, M_nnz, M_split ? (mv_low ? 1 : 2) : 0, Mcol.p, xp,
                yp, stream);
   if (timing) hip_check(hipEventRecord(ev[1], stream), "event");
   for (size_t b = 0; b + 1 < tb1.size(); b++) {  // GEMM tiles, batch by batch (W is shared)
@@ -889,13 +916,13 @@ void Ctx::apply(const double* xp, double* yp) {
     launch_tiles(tiles2.p + tb2[b], ops2.p, tb2[b + 1] - tb2[b], xp, yp, stream, use_tiles_mfma);
   }
   if (timing) hip_check(hipEventRecord(ev[2], stream), "event");
-  launch_chain_mfma(cblocks_m.p, cblocks_m_offs, cgroups.p, cterms.p, xp, yp, cpart.p, stream);
-  launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, cpart.p, chain_P, false, stream);
-  launch_chain(cblocks_l.p, ncblocks_l, cgroups.p, cterms.p, xp, yp, cpart.p, chain_P_l, true, stream);
-  launch_chain_reduce(creds.p, ncreds, cpart.p, yp, stream);
+  launch_chain_mfma(cblocks_m.p, cblocks_m_offs, cgroups.p, cterms.p, xp, yp, cp, stream);
+  launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, cp, chain_P, false, stream);
+  launch_chain(cblocks_l.p, ncblocks_l, cgroups.p, cterms.p, xp, yp, cp, chain_P_l, true, stream);
+  launch_chain_reduce(creds.p, ncreds, cp, yp, stream);
   if (n_lower > 0) {  // lower states, in state order (HeffDiagrams1.cpp:65-86)
-    launch_multi_dot(lower_ptrs.p, n_lower, xp, V, lower_part.p, lower_dot.p, stream);
-    for (int q = 0; q < n_lower; q++) launch_axpy_dev(lower_dot.p + q, 1.0, lower.p + (int64_t)q * V, yp, V, stream);
+    launch_multi_dot(lower_ptrs.p, n_lower, xp, V, lp, ld, stream);
+    for (int q = 0; q < n_lower; q++) launch_axpy_dev(ld + q, 1.0, lower.p + (int64_t)q * V, yp, V, stream);
   }
   if (timing) hip_check(hipEventRecord(ev[3], stream), "event");
   hip_check(hipGetLastError(), "Heff kernels");
@@ -1545,6 +1572,11 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
   add_new_vec();
   matvec(num_vec);
   nmult++;
+  // learned preconditioner guard: the learned correction is used while every step it
+  // makes lowers the residual norm; the first step that does not switches this solve
+  // back to the plain diagonal correction of Davidson.cpp:322-350
+  bool learned_ok = precond_mode == 1 && mlp.ready, last_learned = false;
+  double prev_rnorm = HUGE_VAL;
   for (int iter = 0; iter < 100000; iter++) {
     // Davidson.cpp:242-320
     {
@@ -1568,8 +1600,14 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
     }
     const double rnorm = std::sqrt(dot(t, t));
     if (!std::isfinite(rnorm)) throw Error{ERR_NUMERIC, "Davidson: residual is not finite"};
+    if (last_learned && !(rnorm < prev_rnorm)) {
+      learned_ok = false;
+      n_learned_fallbacks++;
+    }
+    prev_rnorm = rnorm;
     if (mv_low && rnorm <= std::max(rtol, mixed_switch)) {
       mv_low = false;
+      prev_rnorm = HUGE_VAL;
       hip_check(hipMemcpyAsync(t, u, n * sizeof(double), hipMemcpyDeviceToDevice, stream), "refinement start");
       num_vec = 0;
       add_new_vec();
@@ -1579,7 +1617,8 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
     }
     if (rnorm <= rtol) break;
     // Davidson.cpp:322-350, with the learned factor in precond_mode 1
-    const bool learned = precond_mode == 1 && mlp.ready;
+    const bool learned = learned_ok;
+    last_learned = learned;
     const bool harvest = harvest_per_iter > 0 && !learned;
     const float* mvals = nullptr;
     if (learned || harvest) {
@@ -1651,6 +1690,7 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
         lincomb(vs, cs, vec[v]);
       }
       num_vec = 0;
+      prev_rnorm = HUGE_VAL;
       for (int v = 0; v < KEEP; v++) { matvec(v); nmult++; num_vec++; }
       for (int a = 0; a < KEEP; a++)
         for (int b = a; b < KEEP; b++) M[a + MAXV * b] = M[b + MAXV * a] = dot(vec[a], hv[b]);
@@ -1707,10 +1747,6 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
 using chm::Ctx;
 using chm::Error;
 
-struct chm_ctx {
-  Ctx c;
-};
-
 template <class F>
 static int guard(chm_ctx* ctx, F&& f) {
   if (!ctx) return CHM_ERR_ARG;
@@ -1750,6 +1786,10 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_SVD_GESVDJ")) ctx->c.svd_large_gesvdj = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_TILE_W_CAP")) ctx->c.tile_w_cap = std::max(1LL, std::atoll(e));
   if (const char* e = std::getenv("CHM_OPS_KERNEL")) ctx->c.use_ops_kernel = std::atoi(e) != 0;
+  ctx->c.def_explicit_max = ctx->c.explicit_max;
+  ctx->c.def_item_terms = ctx->c.chain_item_terms;
+  ctx->c.def_group_terms = ctx->c.chain_group_terms;
+  ctx->c.def_chain_mfma = ctx->c.use_chain_mfma;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return CHM_ERR_HIP;
@@ -1897,6 +1937,7 @@ int chm_dmrg_init(chm_ctx* ctx, int D, uint64_t seed) {
     (void)hipSetDevice(c.device);
     const double t0 = chm::now_s();
     c.drop_lowers();  // a new DMRG object has no excitations
+    c.site_log.clear();
     chm::auto_sectors(c, D);
     c.mps_random(seed);
     const double t1 = chm::now_s();
@@ -1976,6 +2017,54 @@ int chm_dmrg_heff_stats(const chm_ctx* ctx, double* flops, double* kernel_ms) {
   if (!ctx || !flops || !kernel_ms) return CHM_ERR_ARG;
   *flops = ctx->c.mv_flops;
   *kernel_ms = ctx->c.mv_ms;
+  return CHM_OK;
+}
+
+int chm_set_heff_knobs(chm_ctx* ctx, int64_t explicit_max, int item_terms, int group_terms, int chain_mfma) {
+  return guard(ctx, [&](Ctx& c) {
+    if (explicit_max >= 0) c.def_explicit_max = explicit_max;
+    if (item_terms > 0) c.def_item_terms = item_terms;
+    if (group_terms > 0) c.def_group_terms = group_terms;
+    if (chain_mfma >= 0) c.def_chain_mfma = chain_mfma != 0;
+    c.site = -1;
+  });
+}
+
+int chm_set_tuning(chm_ctx* ctx, int n, const int64_t* rows) {
+  return guard(ctx, [&](Ctx& c) {
+    if (n < 0 || (n > 0 && !rows)) throw Error{CHM_ERR_ARG, "bad tuning table"};
+    std::vector<Ctx::TuneRow> t;
+    for (int r = 0; r < n; r++) {
+      const int64_t* v = rows + 8 * r;
+      if (v[0] > v[1] || v[2] > v[3]) throw Error{CHM_ERR_ARG, "tuning row with an empty range"};
+      t.push_back({v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]});
+    }
+    c.tune = std::move(t);
+    c.site = -1;
+  });
+}
+
+int chm_tuning_used(const chm_ctx* ctx, int64_t* out) {
+  if (!ctx || !out) return CHM_ERR_ARG;
+  const Ctx& c = ctx->c;
+  out[0] = c.tune_row; out[1] = c.site_dmax; out[2] = c.explicit_max; out[3] = c.chain_item_terms;
+  out[4] = c.chain_group_terms; out[5] = c.use_chain_mfma ? 1 : 0;
+  return CHM_OK;
+}
+
+int chm_dmrg_site_log(chm_ctx* ctx, int* n, int* sites, int* products, double* values) {
+  if (!ctx || !n) return CHM_ERR_ARG;
+  auto& log = ctx->c.site_log;
+  *n = (int)log.size();
+  if (!sites) return CHM_OK;
+  if (!products || !values) return CHM_ERR_ARG;
+  for (size_t q = 0; q < log.size(); q++) {
+    sites[q] = log[q].site;
+    products[q] = log[q].products;
+    double* v = values + 5 * q;
+    v[0] = log[q].energy; v[1] = log[q].seconds; v[2] = log[q].F_H; v[3] = log[q].V; v[4] = log[q].heff_ms;
+  }
+  log.clear();
   return CHM_OK;
 }
 
@@ -2230,6 +2319,12 @@ int chm_harvest_take(chm_ctx* ctx, int64_t* n, float* X, float* p, float* e, flo
     *n = have;
     c.hv_feats.clear(); c.hv_p.clear(); c.hv_e.clear(); c.hv_w.clear();
   });
+}
+
+int chm_precond_stats(const chm_ctx* ctx, int64_t* fallbacks) {
+  if (!ctx || !fallbacks) return CHM_ERR_ARG;
+  *fallbacks = ctx->c.n_learned_fallbacks;
+  return CHM_OK;
 }
 
 int chm_mixed_stats(const chm_ctx* ctx, int64_t* low_products, int64_t* full_products) {

@@ -186,6 +186,7 @@ struct Ctx : HostState {
   bool ovl_current(const OvlT& o, int dir) const;
   void drop_lowers();
   int64_t n_low_mults = 0, n_full_mults = 0;
+  int64_t n_learned_fallbacks = 0;  // Davidson solves where the learned step was switched off
   // Learned preconditioner (learned.hip): precond_mode 1 multiplies the diagonal
   // preconditioner by (1 + m) with m from the MLP; harvest_per_iter > 0 records, at
   // every preconditioner application, the features, the plain preconditioned residual p
@@ -200,6 +201,20 @@ struct Ctx : HostState {
   std::vector<float> hv_feats, hv_p, hv_e, hv_w;
   uint64_t hv_counter = 0;
   void build_eldims();
+  // Autotuned routing (chm_set_tuning): rows keyed by the site's largest block dimension
+  // and V select explicit_max, chain_item_terms, chain_group_terms and use_chain_mfma
+  // for that site; sites no row covers use the defaults (chm_set_heff_knobs, the
+  // CHM_* environment knobs or the built-in values).
+  struct TuneRow {
+    int64_t dmax_lo, dmax_hi, v_lo, v_hi, explicit_max, item_terms, group_terms, chain_mfma;
+  };
+  std::vector<TuneRow> tune;
+  int64_t def_explicit_max = 4096;
+  int def_item_terms = CHAIN_ITEM_TERMS, def_group_terms = CHAIN_GROUP_TERMS;
+  bool def_chain_mfma = false;
+  int tune_row = -1;  // row used at the last prepare (-1: defaults)
+  int64_t site_dmax = 0;
+  void apply_tuning();
   bool use_chain_kernel = true;      // CHM_CHAIN_KERNEL=0: every chained term through k_tiles
   // CHM_CHAIN_MFMA=1: chained blocks up to 48 on the matrix cores (k_chain_mfma). Off by
   // default: at L = 32, D = 2000 it matches k_chain for blocks of 33-48 and is slower
@@ -282,6 +297,13 @@ struct Ctx : HostState {
   int mv_used = 0;
   void mv_flush();
   int64_t n_matvec_total = 0;
+  // one record per two-site step of the sweeps since chm_dmrg_init: site, energy, Heff
+  // applications, Davidson seconds, F_H, V and the device ms of its Heff applications
+  struct SiteRec {
+    int site, products;
+    double energy, seconds, F_H, V, heff_ms;
+  };
+  std::vector<SiteRec> site_log;
   uint64_t noise_seed = 1;
   void join(int site);
   // Sobject::Join of MPS tensors Tl, Tr laid out on sectors s into buffer id out_buf
@@ -316,6 +338,8 @@ struct Ctx : HostState {
   void prepare(int site);
   const double* resolve(const Ref& r) const;
   void apply(const double* dxp, double* dyp);
+  void apply_with(const double* xp, double* yp, double* bp, double* cp, double* lp, double* ld);
+  int64_t bpart_need = 0, cpart_need = 0;  // partial-sum doubles the prepared site's product uses
   // frees the prepared site's explicit matrix and tile scratch (and the update arena,
   // which the next update allocates again); the site must be prepared again before the
   // next product
@@ -337,3 +361,8 @@ void auto_sectors(HostState& c, int D);
 std::vector<OpShape> opset_shapes(const HostState& c, int cnt, bool mr);
 
 }  // namespace chm
+
+// The opaque handle of the C ABI.
+struct chm_ctx {
+  chm::Ctx c;
+};

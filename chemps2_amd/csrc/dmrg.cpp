@@ -419,6 +419,7 @@ double Ctx::solve_site(int index, double rtol, double noise, int D, bool mr, boo
   n_matvec_total += nm;
   double t3 = now();
   tm[2] += t3 - t2;
+  site_log.push_back({index, nm, E, t3 - t2, F_H, (double)V, mv_ms - mv0});
   {
     // under memory pressure the explicit matrix and the GEMM-tile scratch of this site go
     // back to the pool before Split and the operator update allocate (the next prepare

@@ -160,6 +160,12 @@ int chm_dmrg_sweep_range(chm_ctx* ctx, int moving_right, int first, int last, in
  * and their device time in ms (HIP events around each application). No counterpart
  * in the reference: it is the sweep-averaged Heff.x rate of the benchmark. */
 int chm_dmrg_heff_stats(const chm_ctx* ctx, double* flops, double* kernel_ms);
+/* Per two-site step of the sweeps since chm_dmrg_init (or the last read): with null
+ * arrays *n receives the number of records; otherwise sites[n], products[n] (Heff
+ * applications of its Davidson) and values[5 n] (energy, Davidson seconds, F_H, V,
+ * device ms of its Heff applications) are written and the records cleared. The
+ * reference prints the same per site (Heff.cpp:376, DMRG.cpp:441-445). */
+int chm_dmrg_site_log(chm_ctx* ctx, int* n, int* sites, int* products, double* values);
 /* seconds[6]: join, Heff preparation, Davidson, split, operator updates, all sweeps;
  * n_matvec: Heff applications. */
 int chm_dmrg_stats(const chm_ctx* ctx, double* seconds, int64_t* n_matvec);
@@ -209,6 +215,11 @@ int chm_mixed_stats(const chm_ctx* ctx, int64_t* low_products, int64_t* full_pro
 #define CHM_PRECOND_LEARNED 1
 #define CHM_MLP_IN 8
 int chm_precond_mode(chm_ctx* ctx, int mode);
+/* The learned correction is kept only while each step it makes lowers the Davidson
+ * residual norm; the first step that does not switches that solve back to the plain
+ * diagonal correction. fallbacks: the number of solves switched back since the context
+ * was created. */
+int chm_precond_stats(const chm_ctx* ctx, int64_t* fallbacks);
 /* Parameters: W1 (8 x 128), b1, W2 (128 x 128), b2, W3 (128 x 128), b3, W4 (128 x 1), b4,
  * each W row-major (inputs x outputs); FP32. */
 int chm_mlp_nparams(int64_t* n);
@@ -238,6 +249,18 @@ int chm_heff_diag(chm_ctx* ctx, double* d, int on_device);
  * On return x holds the eigenvector in the program convention. */
 int chm_davidson(chm_ctx* ctx, double* x, int on_device, double rtol, double* eigenvalue, int* n_matvec);
 
+/* Routing and work-item sizes of the Heff kernels (no reference counterpart): the
+ * largest |kappa||kappa'| of the explicit sector matrix, terms per chained work item and
+ * per input group, and the MFMA chained kernel for blocks up to 48 (1) or the LDS one
+ * (0); a negative value (or 0 for the two term counts) keeps the current setting.
+ * chm_set_heff_knobs sets the defaults. chm_set_tuning installs the autotuner's table:
+ * n rows of 8 values (largest block dimension lo, hi; V lo, hi; then the four knobs as
+ * above); chm_heff_prepare uses the first row whose ranges hold the site's largest
+ * block dimension and V, or the defaults. chm_tuning_used: out[6] = row used (-1: none),
+ * largest block dimension, then the four knobs of the last chm_heff_prepare. */
+int chm_set_heff_knobs(chm_ctx* ctx, int64_t explicit_max, int item_terms, int group_terms, int chain_mfma);
+int chm_set_tuning(chm_ctx* ctx, int n, const int64_t* rows);
+int chm_tuning_used(const chm_ctx* ctx, int64_t* out);
 /* F_H: 2mnk summed over every dgemm_ the reference's makeHeff issues at this site;
  * B_H: 8 * (2V + sum of the sizes of the operator tensors it reads), in bytes. */
 int chm_flops(const chm_ctx* ctx, double* F_H, double* B_H);

@@ -45,6 +45,8 @@ TOp* find_op(OpSet& o, int kind, int k, int m) {
 
 extern "C" {
 
+long orc_site(void* p, int index);
+
 // T: L x L hopping, V: L^4 physicist <ij|kl> at V[i + L(j + L(k + L l))], both in the
 // Hamiltonian's orbital order; reorder != 0 applies the D2h ordering (group 7 only).
 void* orc_create(int L, int group, const int* irreps, int N, int TwoS, int Irrep, const double* T, const double* V,
@@ -183,6 +185,28 @@ double orc_site_apply_range(void* p, const double* x, double* y, int k0, int k1,
   auto t1 = std::chrono::steady_clock::now();
   if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
   return h->H->flops;
+}
+
+// The MPS given site by site (TensorT storage for the bookkeeper at D, as
+// chm_mps_download writes it): the bookkeeper at D, every site tensor copied, the
+// operators right of `site` built moving left (slots L-2 .. site+1) and left of it moving
+// right (slots 0 .. site-1) with the oracle's own updates, then the two-site object and
+// Heff at `site`. Returns V, or -1 if a tensor's size does not match.
+long orc_load_mps(void* p, int D, int site, const double* const* data, const long* sizes) {
+  Handle* h = (Handle*)p;
+  DMRG& dm = h->dm;
+  dm.P.build_mx();
+  dm.bk.init(&dm.P, D);
+  dm.mps.assign(dm.P.L, TensorT());
+  for (int s = 0; s < dm.P.L; s++) {
+    dm.mps[s].init(s, &dm.bk);
+    if ((long)dm.mps[s].data.size() != sizes[s]) return -1;
+    std::copy(data[s], data[s] + sizes[s], dm.mps[s].data.begin());
+  }
+  dm.ops.init(&dm.P, &dm.bk);
+  for (int c = 0; c < site; c++) dm.ops.update_moving_right(c, dm.mps);
+  for (int c = dm.P.L - 2; c > site; c--) dm.ops.update_moving_left(c, dm.mps);
+  return orc_site(p, site);
 }
 
 // Builds the two-site object at `index` (joined from the current MPS) and its Heff.

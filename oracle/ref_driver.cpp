@@ -1,8 +1,9 @@
 // Driver for the reference CheMPS2 library built from its own sources by oracle/ref.mk
 // (test infrastructure: it writes golden vectors and times the reference's CPU path; the
-// product never runs it). It uses only the reference's public API.
+// product never runs it). It uses the reference's public API, and two private DMRG
+// members in the steps mode.
 //
-// Problem files (written by tests/refdrv.py), little endian:
+// Problem files (written by oracle/refdrv.py), little endian:
 //   int32 magic 0x50484d43 ("CMHP"), L, group, N, TwoS, Irrep, has_mx
 //   int32 irreps[L]
 //   float64 econst, T[L*L] (row major), V[L^4] (physicist <ij|kl>, row major i,j,k,l)
@@ -14,6 +15,14 @@
 //   dims <problem> <D>
 //       SyBookkeeper(Prob, D) (SyBookkeeper.cpp:29-269): one line per boundary b and
 //       sector with a non-zero FCI dimension: "b N 2S I current fci".
+//   steps <problem> <D> <first> <last> <rtol> <budget_s> [chkpt]
+//       The first steps of the reference's first left sweep (DMRG::sweepleft with
+//       change = false, DMRG.cpp:356-386): DMRG(Prob, scheme) (PreSolve), then for
+//       index = first down to last, until the steps have taken budget_s seconds:
+//       solve_site + updateMovingLeftSafe. Prints
+//       "init seconds <t>" and per step "step <index> energy <E> seconds <t>"
+//       (Davidson products follow in the reference's own "nIt(DAVIDSON)" lines). With
+//       chkpt the MPS comes from CheMPS2_MPS0.h5 in the working directory.
 //   dmrg <problem> <nstates> <eshift> <ninstr> {D econv max_sweeps noise rtol}*ninstr
 //        [rdm <file>] [chkpt]
 //       DMRG(Prob, scheme).Solve() (DMRG.cpp:268-417); for nstates > 1 the excited-state
@@ -33,7 +42,13 @@
 #include <vector>
 
 #include "ConvergenceScheme.h"
+// The steps mode drives DMRG::solve_site and updateMovingLeftSafe (DMRG.cpp:419-446,
+// DMRGoperators.cpp), which are private: the header is read with them public. The
+// library itself is built from the unmodified sources; access specifiers do not change
+// the class layout.
+#define private public
 #include "DMRG.h"
+#undef private
 #include "Hamiltonian.h"
 #include "Initialize.h"
 #include "Problem.h"
@@ -193,6 +208,36 @@ int mode_dmrg(const ProbFile& p, int argc, char** argv) {
   return 0;
 }
 
+int mode_steps(const ProbFile& p, int argc, char** argv) {
+  if (argc < 5) die("steps: <D> <first> <last> <rtol> <budget_s> [chkpt]");
+  const int D = std::atoi(argv[0]), first = std::atoi(argv[1]), last = std::atoi(argv[2]);
+  const double rtol = std::atof(argv[3]), budget = std::atof(argv[4]);
+  const bool chkpt = argc > 5 && !std::strcmp(argv[5], "chkpt");
+  CheMPS2::ConvergenceScheme scheme(1);
+  scheme.set_instruction(0, D, 1e-8, 1, 0.0, rtol);
+  CheMPS2::Hamiltonian* H = make_ham(p);
+  CheMPS2::Problem prob(H, p.TwoS, p.N, p.Irrep);
+  if (!prob.checkConsistency()) die("inconsistent problem");
+  double t0 = now();
+  auto* dmrg = new CheMPS2::DMRG(&prob, &scheme, chkpt);
+  std::printf("init seconds %.6f\n", now() - t0);
+  std::fflush(stdout);
+  double spent = 0.0;
+  for (int index = first; index >= last && index > 0 && spent < budget; index--) {
+    const double t1 = now();
+    const double E = dmrg->solve_site(index, rtol, 0.0, D, true, false, false);
+    dmrg->updateMovingLeftSafe(index);
+    const double dt = now() - t1;
+    spent += dt;
+    std::printf("step %d energy %.15f seconds %.6f\n", index, E, dt);
+    std::fflush(stdout);
+  }
+  dmrg->deleteStoredOperators();
+  delete dmrg;
+  delete H;
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -204,6 +249,7 @@ int main(int argc, char** argv) {
     return mode_dims(p, std::atoi(argv[3]));
   }
   if (!std::strcmp(argv[1], "dmrg")) return mode_dmrg(p, argc - 3, argv + 3);
+  if (!std::strcmp(argv[1], "steps")) return mode_steps(p, argc - 3, argv + 3);
   die("unknown mode");
   return 2;
 }

@@ -91,3 +91,32 @@ def dmrg(prob_path, schedule, nstates=1, eshift=0.0, rdm=False, threads=None, ti
             res["rdm"] = (a[:n].copy(), a[n:].copy())
         res["stdout"] = out
     return res
+
+
+def steps(prob_path, D, first, last, rtol=1e-5, budget_s=1e30, threads=None, chkpt_file=None, timeout=None):
+    """The reference's first left-sweep steps (ref_driver steps), until they have taken
+    budget_s seconds: {"init_seconds": t, "steps": [(index, energy, seconds, products),
+    ...]}; chkpt_file: an MPS checkpoint in the reference's HDF5 layout to start from."""
+    import shutil
+
+    with tempfile.TemporaryDirectory() as tmp:
+        args = ["steps", prob_path, D, first, last, repr(float(rtol)), repr(float(budget_s))]
+        if chkpt_file:
+            shutil.copy(chkpt_file, os.path.join(tmp, "CheMPS2_MPS0.h5"))
+            args.append("chkpt")
+        out = _run(args, threads=threads, cwd=tmp, timeout=timeout)
+    res = {"init_seconds": None, "steps": []}
+    nits = []
+    for line in out.splitlines():
+        p = line.split()
+        if len(p) == 3 and p[0] == "init" and p[1] == "seconds":
+            res["init_seconds"] = float(p[2])
+        elif len(p) == 6 and p[0] == "step" and p[2] == "energy":
+            res["steps"].append([int(p[1]), float(p[3]), float(p[5])])
+        elif "nIt(DAVIDSON)" in line:
+            nits.append(int(line.split("=")[-1]))
+    # the last nIt lines belong to the steps (one Davidson per step)
+    for q, st in enumerate(res["steps"]):
+        k = len(nits) - len(res["steps"]) + q
+        st.append(nits[k] if 0 <= k < len(nits) else None)
+    return res

@@ -49,6 +49,7 @@ def lib():
             "orc_threads": (i, []),
             "orc_site_apply_range": (d, [vp, _dp, _dp, i, i, C.POINTER(C.c_double)]),
             "orc_site": (L_, [vp, i]),
+            "orc_load_mps": (L_, [vp, i, i, C.POINTER(C.c_void_p), C.POINTER(C.c_long)]),
             "orc_site_x": (None, [vp, _dp]),
             "orc_site_apply": (d, [vp, _dp, _dp]),
             "orc_site_apply_timed": (d, [vp, _dp, _dp, i]),
@@ -135,6 +136,19 @@ class OracleDMRG:
 
     def step_left(self, index, D, rtol=1e-5, change=True):
         return self.lib.orc_step_left(self.h, int(index), int(D), float(rtol), int(change))
+
+    def load_mps(self, D, site, mps):
+        """Bookkeeper at D, the given site tensors, the operators around `site` built by
+        the oracle's updates, and the two-site step at `site` (returns V)."""
+        arrs = [np.ascontiguousarray(m, dtype=np.float64) for m in mps]
+        ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        sizes = (C.c_long * len(arrs))(*[a.size for a in arrs])
+        v = int(self.lib.orc_load_mps(self.h, int(D), int(site), ptrs, sizes))
+        if v < 0:
+            raise ValueError("MPS tensor sizes do not match the bookkeeper at D")
+        self.index = site
+        self.V = v
+        return v
 
     def site(self, index):
         self.index = index
