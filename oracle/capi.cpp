@@ -5,7 +5,7 @@
 
 #include <omp.h>
 
-#include "dmrg.hpp"
+#include "twodm.hpp"
 
 using namespace orc;
 
@@ -349,6 +349,18 @@ double orc_davidson_dense(int n, const double* A, const double* x0, double rtol,
   std::copy(ptr[0], ptr[0] + n, x);
   if (n_mult) *n_mult = dv.n_mult;
   return ptr[1][0];
+}
+
+// The 2-RDM of the current state (DMRG::calc2DMandCorrelations, 2-RDM part), after a
+// solve ending with a right sweep: A and B (L^4 each, DMRG orbital order); returns
+// Econst + 0.5 trace(A h) (TwoDM::energy).
+double orc_2rdm(void* p, double* A, double* B) {
+  DMRG& dm = ((Handle*)p)->dm;
+  TwoDM t;
+  dm.calc_2rdm(t);
+  std::copy(t.A.begin(), t.A.end(), A);
+  std::copy(t.B.begin(), t.B.end(), B);
+  return t.energy();
 }
 
 }  // extern "C"

@@ -32,6 +32,8 @@ struct Davidson {
 // started from the joined two-site object in the symmetric convention.
 double solve_davidson(Heff& H, Sobject& S, double rtol, int* n_mult = nullptr);
 
+struct TwoDM;
+
 struct Instruction { int D; double econv; int max_sweeps; double noise; double rtol; };
 
 // DMRG driver (DMRG.cpp:44-446, DMRGoperators.cpp:233-907, DMRGtechnics.cpp:276-298).
@@ -50,6 +52,8 @@ struct DMRG {
   double sweep_left(bool change, const Instruction& ins);   // DMRG.cpp:356-386
   double sweep_right(bool change, const Instruction& ins);  // DMRG.cpp:387-417
   double solve_site(int index, double rtol, double noise, int D, bool moving_right, bool change);  // DMRG.cpp:419-446
+  // the 2-RDM of the current state, after a right sweep (DMRGtechnics.cpp:40-110)
+  void calc_2rdm(TwoDM& out);
 };
 
 // TensorT::QR restated (TensorT.cpp:188-280): per right sector stack the left blocks

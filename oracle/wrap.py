@@ -68,6 +68,7 @@ def lib():
             "orc_wigner6j": (d, [i] * 6),
             "orc_wigner9j": (d, [i] * 9),
             "orc_davidson_dense": (d, [i, _dp, _dp, d, _dp, C.POINTER(C.c_int)]),
+            "orc_2rdm": (d, [vp, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(l, name)
@@ -220,6 +221,15 @@ class OracleDMRG:
         ptr = C.POINTER(C.c_double)()
         n = self.lib.orc_mps(self.h, int(site), C.byref(ptr))
         return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n > 0 else np.zeros(0)
+
+    def two_rdm(self):
+        """2-RDMs (A, B) of the converged state as L x L x L x L arrays indexed [i, j, k, l]
+        (DMRG orbital order) and the energy 0.5 trace(A h) + Econst. Call after solve()."""
+        L = self.L
+        A = np.zeros(L ** 4)
+        B = np.zeros(L ** 4)
+        e = self.lib.orc_2rdm(self.h, A, B)
+        return A.reshape(L, L, L, L, order="F"), B.reshape(L, L, L, L, order="F"), e
 
     def matvec_stats(self):
         return self.lib.orc_matvec_count(self.h), self.lib.orc_matvec_flops(self.h)
