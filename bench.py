@@ -257,6 +257,12 @@ def main():
         sys.exit(2)
 
     import numpy as np
+    import torch
+
+    # torch's HIP context first: it carries the site benchmark's device vectors, and it
+    # must not start after the library has taken most of the device memory
+    torch.cuda.init()
+    torch.empty(1, device="cuda:0")
 
     import chemps2_amd
     from chemps2_amd.problems import mx_table, random_abinitio

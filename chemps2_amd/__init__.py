@@ -26,7 +26,7 @@ EXPORTS = [
     "chm_set_heff_form", "chm_heff_apply_f32", "chm_mixed_stats", "chm_heff_set_lower",
     "chm_precond_mode", "chm_mlp_nparams", "chm_mlp_set", "chm_mlp_get", "chm_mlp_forward", "chm_mlp_loss_grad",
     "chm_mlp_train", "chm_harvest", "chm_harvest_take", "chm_dmrg_sweep_range",
-    "chm_dmrg_new_excitation", "chm_dmrg_nstates", "chm_dmrg_site_log",
+    "chm_dmrg_new_excitation", "chm_dmrg_nstates", "chm_dmrg_site_log", "chm_dmrg_2rdm",
     "chm_set_heff_knobs", "chm_set_tuning", "chm_tuning_used", "chm_precond_stats",
 ]
 PRECOND_DIAGONAL, PRECOND_LEARNED = 0, 1
@@ -73,6 +73,7 @@ def load():
         "chm_dmrg_init": (i, [vp, i, C.c_uint64]),
         "chm_dmrg_new_excitation": (i, [vp, d, i, C.c_uint64]),
         "chm_dmrg_nstates": (i, [vp, C.POINTER(i)]),
+        "chm_dmrg_2rdm": (i, [vp, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
         "chm_dmrg_site_log": (i, [vp, C.POINTER(i), vp, vp, vp]),
         "chm_set_heff_knobs": (i, [vp, i64, i, i, i]),
         "chm_set_tuning": (i, [vp, i, vp]),
@@ -381,6 +382,17 @@ class Context:
         n = C.c_int(0)
         self._check(self.lib.chm_dmrg_nstates(self.h, C.byref(n)), "chm_dmrg_nstates")
         return n.value
+
+    def two_rdm(self):
+        """TwoDM of the current state (DMRG::calc_rdms_and_correlations, 2-RDM part): the
+        spin-summed 2-RDMs A and B as L x L x L x L arrays indexed [i, j, k, l] (DMRG
+        orbital order) and Econst + 0.5 sum A h. Call after dmrg_solve."""
+        L = self.get_problem()["L"]
+        A = np.zeros(L ** 4)
+        B = np.zeros(L ** 4)
+        e = C.c_double(0.0)
+        self._check(self.lib.chm_dmrg_2rdm(self.h, A.ctypes.data, B.ctypes.data, C.byref(e)), "chm_dmrg_2rdm")
+        return A.reshape(L, L, L, L, order="F"), B.reshape(L, L, L, L, order="F"), e.value
 
     def dmrg_solve(self, scheme):
         """scheme: list of (D, econv, max_sweeps, noise, rtol), as ConvergenceScheme."""

@@ -372,7 +372,16 @@ struct DevSvd {
 constexpr int SVD_JACOBI_MAXN = 256;  // largest n of the batched Jacobi kernel
 
 // Launchers (kernels.hip).
+// One dot product of two device arrays (the 2-RDM's P tensor with an operator).
+struct DevDot {
+  const double* a;
+  const double* b;
+  int64_t n;
+};
+
 void launch_copies(const DevCopy* items, int n, hipStream_t s);
+// out[q] = a_q . b_q for every item q (one workgroup per item, fixed summation order)
+void launch_dots(const DevDot* items, int n, double* out, hipStream_t s);
 void launch_svd_jacobi(const DevSvd* items, int n, hipStream_t s);
 void launch_add_noise(double* x, int64_t n, uint64_t seed, double level, hipStream_t s);
 // mfma: k_tiles_mfma (matrix cores, prefetched double-buffered staging), else k_tiles

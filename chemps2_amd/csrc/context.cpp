@@ -1961,6 +1961,14 @@ int chm_dmrg_new_excitation(chm_ctx* ctx, double eshift, int D, uint64_t seed) {
   });
 }
 
+int chm_dmrg_2rdm(chm_ctx* ctx, double* A, double* B, double* energy) {
+  return guard(ctx, [&](Ctx& c) {
+    (void)hipSetDevice(c.device);
+    const double e = c.two_rdm(A, B);
+    if (energy) *energy = e;
+  });
+}
+
 int chm_dmrg_nstates(const chm_ctx* ctx, int* n) {
   if (!ctx || !n) return CHM_ERR_ARG;
   *n = (int)ctx->c.lowers.size() + 1;

@@ -185,6 +185,14 @@ struct Ctx : HostState {
   void veff_tilde(int index);
   bool ovl_current(const OvlT& o, int dir) const;
   void drop_lowers();
+  // 2-RDM (twodm.cpp): the P tensors of the site being filled, the dot-product records
+  // and their results, and the seconds of the last 2-RDM sweep
+  int32_t rdm_pbuf = -1;
+  DBuf<DevDot> rdm_items;
+  DBuf<double> rdm_res;
+  double rdm_seconds = 0.0;
+  void rdm_fill_site(int site, std::vector<double>& A, std::vector<double>& B);
+  double two_rdm(double* A, double* B);
   int64_t n_low_mults = 0, n_full_mults = 0;
   int64_t n_learned_fallbacks = 0;  // Davidson solves where the learned step was switched off
   // Learned preconditioner (learned.hip): precond_mode 1 multiplies the diagonal

@@ -1109,6 +1109,30 @@ __global__ __launch_bounds__(256) void k_svd_jacobi(const DevSvd* __restrict__ i
 }
 
 // x += level * (u - 1/2) with u uniform in [0, 1) from a splitmix64 hash of the index.
+// Dot products of array pairs, one workgroup per pair: four strided partial sums per
+// thread, then a fixed-order tree over the workgroup.
+__global__ __launch_bounds__(256) void k_dots(const DevDot* __restrict__ items, double* __restrict__ out) {
+  __shared__ double red[256];
+  const DevDot d = items[blockIdx.x];
+  const int t = threadIdx.x;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int64_t e = t;
+  for (; e + 768 < d.n; e += 1024) {
+    s0 = fma(d.a[e], d.b[e], s0);
+    s1 = fma(d.a[e + 256], d.b[e + 256], s1);
+    s2 = fma(d.a[e + 512], d.b[e + 512], s2);
+    s3 = fma(d.a[e + 768], d.b[e + 768], s3);
+  }
+  for (; e < d.n; e += 256) s0 = fma(d.a[e], d.b[e], s0);
+  red[t] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
+    __syncthreads();
+  }
+  if (t == 0) out[blockIdx.x] = red[0];
+}
+
 __global__ __launch_bounds__(256) void k_add_noise(double* __restrict__ x, int64_t n, uint64_t seed, double level) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
     uint64_t z = seed * 0x9E3779B97F4A7C15ULL + (uint64_t)i;
@@ -1213,6 +1237,10 @@ int dot_parts(int64_t n) {
   if (g > 1024) g = 1024;
   if (g < 1) g = 1;
   return (int)g;
+}
+
+void launch_dots(const DevDot* items, int n, double* out, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_dots, dim3(n), dim3(256), 0, s, items, out);
 }
 
 void launch_dot(const double* a, const double* b, int64_t n, double* partial, double* out, hipStream_t s) {

@@ -144,6 +144,18 @@ int chm_dmrg_solve(chm_ctx* ctx, int n, const int* D, const double* econv, const
  * chm_dmrg_nstates: the number of states (lower states + 1). */
 int chm_dmrg_new_excitation(chm_ctx* ctx, double eshift, int D, uint64_t seed);
 int chm_dmrg_nstates(const chm_ctx* ctx, int* n);
+/* The 2-RDM of the current state, DMRG::calc_rdms_and_correlations without the 3-RDM
+ * and correlations (DMRGtechnics.cpp:40-110; TwoDM, TwoDM.cpp:40-640): after a solve
+ * (the state a right sweep leaves), the MPS is brought into left-canonical form and one
+ * left sweep fills the spin-summed 2-RDMs site by site (TwoDM::FillSite) with
+ * right-normalisation and left-moving operator updates. A and B (host arrays of L^4
+ * doubles, or null) receive TwoDM's two_rdm_A / two_rdm_B in DMRG orbital order, element
+ * (i, j, k, l) at i + L (j + L (k + L l)), divided by 2S + 1 as
+ * TwoDM::correct_higher_multiplicities does; *energy (or null) = Econst + 0.5 sum A h
+ * (TwoDM::energy). Afterwards the MPS is right-canonical and the operators are those of
+ * a finished left sweep (the next sweep moves right, or call chm_dmrg_presolve after a
+ * left-normalising sweep). */
+int chm_dmrg_2rdm(chm_ctx* ctx, double* A, double* B, double* energy);
 /* One half sweep, DMRG::sweepleft (moving_right = 0) or DMRG::sweepright
  * (moving_right = 1) of DMRG.cpp:354-417 with one instruction's D, Davidson tolerance
  * and noise prefactor; change = 0 keeps the virtual dimensions (the reference's first

@@ -112,9 +112,28 @@ def test_config_cylinder_D1000_heff():
         e64 = rel(c.apply(x), y0)
         ed = rel(c.diag(), o.diag())
         print(f"  V={n} F_H={f0:.3e} tiles={st['tiles1']}+{st['tiles2']}: product {e64:.2e}, diag {ed:.2e}", flush=True)
-        assert st["tiles1"] + st["tiles2"] > 0  # blocks above 64 go through the GEMM tiles
         assert c.flops()[0] == f0
         assert e64 <= 1e-12 and ed <= 1e-12
+    finally:
+        c.close()
+    # the same product with every chained term on the MFMA GEMM tiles (CHM_CHAIN_KERNEL=0,
+    # read when a context is created): the tile path at this configuration
+    os.environ["CHM_CHAIN_KERNEL"] = "0"
+    try:
+        c = chemps2_amd.Context(0)
+    finally:
+        del os.environ["CHM_CHAIN_KERNEL"]
+    try:
+        c.set_problem(32, 0, irr, 28, 0, 0, mx_table(T, V, 28))
+        c.set_sectors_auto(1000)
+        c.fill_synthetic(14, 1, 1)
+        c.fill_synthetic(16, 0, 2)
+        assert c.prepare(15) == n
+        st = c.plan_stats()
+        et = rel(c.apply(x), y0)
+        print(f"  chained terms on the GEMM tiles: tiles={st['tiles1']}+{st['tiles2']}, product {et:.2e}", flush=True)
+        assert st["tiles1"] + st["tiles2"] > 0
+        assert et <= 1e-12
     finally:
         c.close()
 
