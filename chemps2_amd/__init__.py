@@ -26,7 +26,7 @@ EXPORTS = [
     "chm_set_heff_form", "chm_heff_apply_f32", "chm_mixed_stats", "chm_heff_set_lower",
     "chm_precond_mode", "chm_mlp_nparams", "chm_mlp_set", "chm_mlp_get", "chm_mlp_forward", "chm_mlp_loss_grad",
     "chm_mlp_train", "chm_harvest", "chm_harvest_take", "chm_dmrg_sweep_range",
-    "chm_dmrg_new_excitation", "chm_dmrg_nstates", "chm_dmrg_site_log", "chm_dmrg_2rdm",
+    "chm_dmrg_new_excitation", "chm_dmrg_nstates", "chm_dmrg_site_log", "chm_dmrg_2rdm", "chm_heff_apply_noalloc",
     "chm_set_heff_knobs", "chm_set_tuning", "chm_tuning_used", "chm_precond_stats",
 ]
 PRECOND_DIAGONAL, PRECOND_LEARNED = 0, 1
@@ -74,6 +74,7 @@ def load():
         "chm_dmrg_new_excitation": (i, [vp, d, i, C.c_uint64]),
         "chm_dmrg_nstates": (i, [vp, C.POINTER(i)]),
         "chm_dmrg_2rdm": (i, [vp, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
+        "chm_heff_apply_noalloc": (i, [vp, C.c_void_p, C.c_void_p]),
         "chm_dmrg_site_log": (i, [vp, C.POINTER(i), vp, vp, vp]),
         "chm_set_heff_knobs": (i, [vp, i64, i, i, i]),
         "chm_set_tuning": (i, [vp, i, vp]),
@@ -480,6 +481,13 @@ class Context:
             raise ValueError("a device x needs a device y of the same length")
         xp, yp = _dev(x, self.device, self.V, "x"), _dev(y, self.device, self.V, "y")
         self._check(fn(self.h, xp, yp, 1), name)
+        return y
+
+    def apply_noalloc(self, x, y):
+        """y = Heff x through chm_heff_apply_noalloc (device tensors of length V; partial
+        sums in the library's static device scratch, no device allocation)."""
+        xp, yp = _dev(x, self.device, self.V, "x"), _dev(y, self.device, self.V, "y")
+        self._check(self.lib.chm_heff_apply_noalloc(self.h, xp, yp), "chm_heff_apply_noalloc")
         return y
 
     def set_lower(self, veff):

@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libchemps2_amd.so")
-SOURCES = ["kernels.hip", "learned.hip", "layout.cpp", "bookkeeper.cpp", "heff_plan.cpp", "update_plan.cpp", "dmrg.cpp", "excited.cpp", "twodm.cpp", "context.cpp"]
+SOURCES = ["kernels.hip", "learned.hip", "layout.cpp", "bookkeeper.cpp", "heff_plan.cpp", "update_plan.cpp", "dmrg.cpp", "excited.cpp", "twodm.cpp", "context.cpp", "noalloc.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -191,6 +191,15 @@ int chm_heff_prepare(chm_ctx* ctx, int site);
 int chm_heff_veclen(const chm_ctx* ctx, int64_t* V);
 /* y = Heff x in the symmetric convention (Heff::makeHeff, Heff.cpp:43-248). */
 int chm_heff_apply(chm_ctx* ctx, const double* x, double* y, int on_device);
+/* chm_heff_apply (on_device = 1) for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. x and y are device arrays of V doubles; the partial sums
+ * of the product live in a static device array of the library (256 MiB per GPU, held
+ * until the process ends) instead of the context's buffers, and the call makes no
+ * hipMalloc / hipFree. A prepared site whose partial sums exceed the array is refused
+ * with CHM_ERR_SIZE. The array is shared by every call on a GPU: calls that overlap on
+ * one GPU race. Returns after the product is complete on the context's stream. */
+int chm_heff_apply_noalloc(chm_ctx* ctx, const double* x, double* y);
 /* Lower states for excited-state DMRG at the prepared site: n_lower vectors of length V
  * (row-major, n_lower x V, in the block order and symmetric convention of the x of
  * chm_heff_apply), each the reference's VeffTilde
