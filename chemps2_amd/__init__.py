@@ -26,7 +26,7 @@ EXPORTS = [
     "chm_set_heff_form", "chm_heff_apply_f32", "chm_mixed_stats", "chm_heff_set_lower",
     "chm_precond_mode", "chm_mlp_nparams", "chm_mlp_set", "chm_mlp_get", "chm_mlp_forward", "chm_mlp_loss_grad",
     "chm_mlp_train", "chm_harvest", "chm_harvest_take", "chm_dmrg_sweep_range",
-    "chm_dmrg_new_excitation", "chm_dmrg_nstates", "chm_dmrg_site_log", "chm_dmrg_2rdm", "chm_heff_apply_noalloc",
+    "chm_dmrg_new_excitation", "chm_dmrg_nstates", "chm_dmrg_site_log", "chm_dmrg_2rdm", "chm_heff_apply_noalloc", "chm_davidson_stats",
     "chm_set_heff_knobs", "chm_set_tuning", "chm_tuning_used", "chm_precond_stats",
 ]
 PRECOND_DIAGONAL, PRECOND_LEARNED = 0, 1
@@ -75,8 +75,9 @@ def load():
         "chm_dmrg_nstates": (i, [vp, C.POINTER(i)]),
         "chm_dmrg_2rdm": (i, [vp, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
         "chm_heff_apply_noalloc": (i, [vp, C.c_void_p, C.c_void_p]),
+        "chm_davidson_stats": (i, [vp, C.POINTER(C.c_int64)]),
         "chm_dmrg_site_log": (i, [vp, C.POINTER(i), vp, vp, vp]),
-        "chm_set_heff_knobs": (i, [vp, i64, i, i, i]),
+        "chm_set_heff_knobs": (i, [vp, i64, i, i, i, i]),
         "chm_set_tuning": (i, [vp, i, vp]),
         "chm_tuning_used": (i, [vp, vp]),
         "chm_precond_stats": (i, [vp, C.POINTER(i64)]),
@@ -426,20 +427,21 @@ class Context:
         self._check(self.lib.chm_dmrg_heff_stats(self.h, C.byref(f), C.byref(ms)), "chm_dmrg_heff_stats")
         return f.value, ms.value
 
-    def set_heff_knobs(self, explicit_max=-1, item_terms=0, group_terms=0, chain_mfma=-1):
+    def set_heff_knobs(self, explicit_max=-1, item_terms=0, group_terms=0, chain_mfma=-1, fchain_min=-1):
         self._check(self.lib.chm_set_heff_knobs(self.h, int(explicit_max), int(item_terms), int(group_terms),
-                                                int(chain_mfma)), "chm_set_heff_knobs")
+                                                int(chain_mfma), int(fchain_min)), "chm_set_heff_knobs")
 
     def set_tuning(self, rows):
         """Autotuner table: rows of (dmax_lo, dmax_hi, V_lo, V_hi, explicit_max, item_terms,
-        group_terms, chain_mfma)."""
-        a = np.ascontiguousarray(np.asarray(rows, dtype=np.int64).reshape(-1, 8))
+        group_terms, chain_mfma, fchain_min)."""
+        a = np.ascontiguousarray(np.asarray(rows, dtype=np.int64).reshape(-1, 9))
         self._check(self.lib.chm_set_tuning(self.h, a.shape[0], a.ctypes.data if a.size else None), "chm_set_tuning")
 
     def tuning_used(self):
-        out = np.zeros(6, dtype=np.int64)
+        out = np.zeros(7, dtype=np.int64)
         self._check(self.lib.chm_tuning_used(self.h, out.ctypes.data), "chm_tuning_used")
-        return dict(zip(["row", "dmax", "explicit_max", "item_terms", "group_terms", "chain_mfma"], out.tolist()))
+        return dict(zip(["row", "dmax", "explicit_max", "item_terms", "group_terms", "chain_mfma", "fchain_min"],
+                        out.tolist()))
 
     def site_log(self):
         """Per two-site step since the last read: list of dicts (site, energy, products,
@@ -489,6 +491,12 @@ class Context:
         xp, yp = _dev(x, self.device, self.V, "x"), _dev(y, self.device, self.V, "y")
         self._check(self.lib.chm_heff_apply_noalloc(self.h, xp, yp), "chm_heff_apply_noalloc")
         return y
+
+    def davidson_stats(self):
+        """(deflations, iteration graphs captured, graph launches) of this context's Davidson solves."""
+        s = (C.c_int64 * 3)()
+        self._check(self.lib.chm_davidson_stats(self.h, s), "chm_davidson_stats")
+        return tuple(s)
 
     def set_lower(self, veff):
         """Lower states' VeffTilde vectors for the prepared site (n_lower x V, host array

@@ -6,11 +6,13 @@ For two-site steps of real DMRG states (the L = 32, D = 2000 and L = 24, D = 100
 random D2h Hamiltonians after a warm-up schedule, and the 4 x 8 Hubbard cylinder at
 D = 1000), every candidate setting of the knobs that chm_heff_prepare routes by
 (explicit_max: the largest |kappa||kappa'| stored as explicit sector matrix;
-item_terms / group_terms: terms per work item and per input group of the chained
-kernel; chain_mfma: the MFMA chained kernel for blocks up to 48) is prepared and
-timed with HIP events (chm_timing_*). Sites are grouped by their largest block
-dimension; each group keeps the setting with the least summed product time over its
-sites (a coordinate search: explicit_max first, then item_terms, then chain_mfma).
+fchain_min: chained blocks whose largest dimension is above it go to the fused MFMA
+kernel k_fchain; item_terms / group_terms: terms per work item and per input group of
+the chained kernels; chain_mfma: the one-wavefront MFMA chained kernel for blocks up to
+48) is prepared and timed with HIP events (chm_timing_*). Sites are grouped by their
+largest block dimension; each group keeps the setting with the least summed product
+time over its sites (a coordinate search: explicit_max first, then fchain_min, then
+item_terms, then chain_mfma).
 The table, with the measurements it came from, goes to data/autotune.json; Context
 loads it at creation (chm_set_tuning), and chm_heff_prepare looks up each site in it.
 """
@@ -29,6 +31,7 @@ DEFAULT_OUT = os.path.join(HERE, "data", "autotune.json")
 EXPLICIT = [0, 1024, 4096, 16384]
 ITEMS = [32, 64, 128]
 MFMA = [0, 1]
+FCHAIN = [0, 16, 32, 48, 64]
 # largest block dimension classes of the table
 DMAX_EDGES = [1, 8, 16, 24, 32, 48, 64, 128, 1 << 30]
 
@@ -60,7 +63,7 @@ def tune_state(c, L, D, sites, reps, records, label):
             c.dmrg_sweep_range(False, first, s + 1, D, 1e-5, 0.0, False)
         first = s
         c.join(s)
-        base = [4096, 64, 32, 0]
+        base = [4096, 64, 32, 0, 32]
         res = {}
 
         def run(k):
@@ -69,22 +72,27 @@ def tune_state(c, L, D, sites, reps, records, label):
                 res[key] = time_site(c, s, k, reps)
             return res[key]
 
-        for e in EXPLICIT:
-            run([e] + base[1:])
-        base[0] = min(EXPLICIT, key=lambda e: res[(e, 64, 32, 0)])
+        def best(cands, at):
+            for v in cands:
+                k = list(base)
+                k[at] = v
+                run(k)
+            return min(cands, key=lambda v: res[tuple(base[:at] + [v] + base[at + 1:])])
+
+        base[0] = best(EXPLICIT, 0)
+        base[4] = best(FCHAIN, 4)
         for it in ITEMS:
-            run([base[0], it, max(8, it // 2), 0])
-        best_it = min(ITEMS, key=lambda it: res[(base[0], it, max(8, it // 2), 0)])
+            run([base[0], it, max(8, it // 2), base[3], base[4]])
+        best_it = min(ITEMS, key=lambda it: res[(base[0], it, max(8, it // 2), base[3], base[4])])
         base[1], base[2] = best_it, max(8, best_it // 2)
-        for m in MFMA:
-            run(base[:3] + [m])
+        base[3] = best(MFMA, 3)
         used = c.tuning_used()
         rec = {"workload": label, "site": s, "V": int(c.V), "dmax": int(used["dmax"]),
                "times_ms": [[list(k), round(v, 4)] for k, v in res.items()]}
         records.append(rec)
-        best = min(res, key=res.get)
-        log(f"{label} site {s}: V={c.V} dmax={used['dmax']}: best {best} {res[best]:.3f} ms "
-            f"(default {res.get((4096, 64, 32, 0), float('nan')):.3f} ms)")
+        top = min(res, key=res.get)
+        log(f"{label} site {s}: V={c.V} dmax={used['dmax']}: best {top} {res[top]:.3f} ms "
+            f"(default {res.get((4096, 64, 32, 0, 32), float('nan')):.3f} ms)")
 
 
 def build_table(records):
@@ -102,7 +110,7 @@ def build_table(records):
             continue
         tot = {k: sum(dict((tuple(kk), v) for kk, v in r["times_ms"])[k] for r in grp) for k in keys}
         best = min(tot, key=tot.get)
-        rows.append([lo, hi - 1, 0, 1 << 62, best[0], best[1], best[2], best[3]])
+        rows.append([lo, hi - 1, 0, 1 << 62, best[0], best[1], best[2], best[3], best[4]])
     return rows
 
 
@@ -154,7 +162,7 @@ def main():
         c.close()
     rows = build_table(records)
     out = {"columns": ["dmax_lo", "dmax_hi", "V_lo", "V_hi", "explicit_max", "item_terms", "group_terms",
-                       "chain_mfma"],
+                       "chain_mfma", "fchain_min"],
            "rows": rows, "measurements": records, "seconds": round(time.time() - t0, 1)}
     with open(args.out, "w") as f:
         json.dump(out, f, indent=1)

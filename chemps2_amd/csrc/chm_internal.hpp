@@ -400,6 +400,17 @@ void launch_chain_reduce(const DevBRed* reds, int nreds, const double* part, dou
 // (every dimension of the item's blocks at most 16 nt); items [offs[nt-1], offs[nt]).
 // offs is a host array. Larger blocks run in k_chain.
 constexpr int CHAIN_MFMA_NT = 3;
+// k_fchain: chained products of blocks up to FCHAIN_MAXD on the matrix cores, one
+// workgroup per work item (x and one intermediate in LDS); FCHAIN_NT output tiles of
+// 16 x 16 per wavefront
+constexpr int FCHAIN_MAXD = 96;
+constexpr int FCHAIN_NT = ((FCHAIN_MAXD / 16) * (FCHAIN_MAXD / 16) + 3) / 4;
+// dmax: the largest block dimension of the launch (sets the LDS of a workgroup)
+// grants k_fchain the LDS of its largest blocks (false: the device refuses; the blocks
+// then run as GEMM tiles)
+bool fchain_kernel_setup();
+void launch_fchain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups, const DevCTerm* terms, const double* x,
+                   double* y, double* part, int dmax, hipStream_t s);
 void launch_chain_mfma(const DevCBlock* blocks, const int* offs, const DevCGroup* groups, const DevCTerm* terms,
                        const double* x, double* y, double* part, hipStream_t s);
 void launch_ops(const DevOItem* items, int nitems, const DevGemmOp* ops, int P, hipStream_t s);
@@ -424,6 +435,36 @@ void launch_lincomb_vecs(const double* const* d_vecs, const double* d_coefs, int
 // Davidson.cpp:322-350: stage 0 work = g u, stage 1 t = -g (t + alpha u), with
 // g = 1 / max(|d - shift|, cutoff) (signed) times 1 + m[i], clamped to [0.1, 10], when m
 // is given.
+// Scalars and small matrices of one Davidson solve on the device (Davidson.cpp:105-191,
+// 214-350): basis size, Ritz value, residual norm, Olsen coefficient, V^T H V, its
+// eigenvectors, projections of a new vector, and a wall-clock sum around the products.
+constexpr int DAV_MAXV = 32;
+struct DevDav {
+  int nv;    // basis vectors held
+  int bad;   // a new vector had a zero or invalid norm
+  double theta, rn2, alpha, nrm2;
+  double wr[2];
+  unsigned long long t0, heff_ticks;
+  double M[DAV_MAXV * DAV_MAXV];     // column-major, leading dimension DAV_MAXV
+  double evec[DAV_MAXV * DAV_MAXV];  // eigenvectors of the leading nv x nv block, ascending
+  double eig[DAV_MAXV];
+  double proj[DAV_MAXV];
+};
+void launch_dav_store(const DevDav* st, double* const* vec, double* const* hv, const double* x, const double* y, int64_t n,
+                      hipStream_t s);
+// dots a[c] . b for c < nv + add; mode 0 writes the new column of M, mode 1 the projections
+void launch_dav_dots(DevDav* st, int add, const double* const* a, const double* b, int64_t n, double* partial, int parts,
+                     int mode, hipStream_t s);
+void launch_dav_project_out(const DevDav* st, const double* const* vec, double* t, int64_t n, hipStream_t s);
+void launch_dav_normalize(DevDav* st, const double* t, double* out, int64_t n, hipStream_t s);
+void launch_dav_eig(DevDav* st, hipStream_t s);
+void launch_dav_ritz(DevDav* st, const double* const* vec, const double* const* hv, double* u, double* r, int64_t n,
+                     double* partial, int parts, hipStream_t s);
+void launch_dav_alpha(DevDav* st, hipStream_t s);
+void launch_dav_clock(DevDav* st, int which, hipStream_t s);
+// the Davidson correction with the shift and the coefficient read from device memory
+void launch_precond_dev(const double* diag, const double* u, const double* dshift, double cutoff, double* work, double* t,
+                        int64_t n, int stage, const double* dalpha, hipStream_t s, const float* m);
 void launch_precond(const double* diag, const double* u, double shift, double cutoff, double* work, double* t,
                     int64_t n, int stage, double alpha, hipStream_t s, const float* m = nullptr);
 void launch_symm_scale(double* x, const int64_t* d_k2i, const double* d_scale, int nk, hipStream_t s);

@@ -3,6 +3,7 @@
 #include <array>
 #include <atomic>
 #include <chrono>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -28,6 +29,7 @@ Ctx::~Ctx() {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : mv_ev)
     if (e) (void)hipEventDestroy(e);
+  if (dav_host) (void)hipHostFree(dav_host);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -217,6 +219,7 @@ void Ctx::apply_tuning() {
   chain_item_terms = def_item_terms;
   chain_group_terms = def_group_terms;
   use_chain_mfma = def_chain_mfma;
+  fchain_min = def_fchain_min;
   tune_row = -1;
   for (size_t r = 0; r < tune.size(); r++) {
     const TuneRow& t = tune[r];
@@ -225,6 +228,7 @@ void Ctx::apply_tuning() {
     if (t.item_terms > 0) chain_item_terms = (int)t.item_terms;
     if (t.group_terms > 0) chain_group_terms = (int)t.group_terms;
     if (t.chain_mfma >= 0) use_chain_mfma = t.chain_mfma != 0;
+    if (t.fchain_min >= 0) fchain_min = (int)t.fchain_min;
     tune_row = (int)r;
     break;
   }
@@ -489,13 +493,13 @@ void Ctx::prepare(int st) {
     // are then concatenated in order (group, term and partial offsets shifted), so the
     // lists do not depend on the thread count
     struct Part {
-      std::vector<DevCBlock> cb, cbl, cbm[CHAIN_MFMA_NT];
+      std::vector<DevCBlock> cb, cbl, cbm[CHAIN_MFMA_NT], cbf[2];  // cbf: k_fchain, blocks <= 64 / <= FCHAIN_MAXD
       std::vector<DevCGroup> cg;
       std::vector<DevCTerm> ct;
       std::vector<DevBRed> cr;
       std::vector<const Term*> rest;
       int64_t cpart_n = 0;
-      int Pmax = 1, PmaxL = 1;
+      int Pmax = 1, PmaxL = 1, Fmax[2] = {1, 1};
     };
     std::vector<size_t> bstart;  // first chain index of every output block
     for (size_t q = 0; q < chain.size(); q++)
@@ -505,23 +509,28 @@ void Ctx::prepare(int st) {
     const int64_t nparts = std::max<int64_t>(1, std::min<int64_t>(256, (nblk + 15) / 16));
     std::vector<Part> parts(nparts);
     const int maxd = chain_large_ok ? CHAIN_MAXD : 36;
+    // blocks up to FCHAIN_MAXD can take the fused MFMA kernel: those above maxd always,
+    // the others when their largest dimension is above fchain_min
+    const int lim = use_fchain ? std::max(maxd, FCHAIN_MAXD) : maxd;
     parallel_for(nparts, nthr, [&](int64_t pi, int) {
       Part& P = parts[pi];
       std::vector<DevCBlock>&cb = P.cb, &cbl = P.cbl;
       std::vector<DevCBlock>* cbm = P.cbm;
+      std::vector<DevCBlock>* cbf = P.cbf;
       std::vector<DevCGroup>& cg = P.cg;
       std::vector<DevCTerm>& ct = P.ct;
       std::vector<DevBRed>& cr = P.cr;
       std::vector<const Term*>& rest = P.rest;
       int64_t& cpart_n = P.cpart_n;
       int &Pmax = P.Pmax, &PmaxL = P.PmaxL;
+      int* Fmax = P.Fmax;
       const size_t qb = bstart[nblk * pi / nparts], qe = bstart[nblk * (pi + 1) / nparts];
       for (size_t q0 = qb; q0 < qe;) {
         const int k = chain[q0]->out;
         size_t q1 = q0;
-        bool small = use_chain_kernel && dLof(k) <= maxd && dRof(k) <= maxd;
+        bool small = use_chain_kernel && dLof(k) <= lim && dRof(k) <= lim;
         while (q1 < qe && chain[q1]->out == k) {
-          if (dLof(chain[q1]->in) > maxd || dRof(chain[q1]->in) > maxd) small = false;
+          if (dLof(chain[q1]->in) > lim || dRof(chain[q1]->in) > lim) small = false;
           q1++;
         }
         if (!small) {
@@ -588,7 +597,11 @@ void Ctx::prepare(int st) {
           for (auto& it : items) { it.direct = 0; it.out = cpart_n; cpart_n += nel; }
         }
         const int nt = (dmax + 15) / 16;
-        if (use_chain_mfma && nt <= CHAIN_MFMA_NT) {
+        if (use_fchain && (dmax > maxd || dmax > fchain_min)) {
+          const int bk = dmax <= 64 ? 0 : 1;
+          for (auto& it : items) cbf[bk].push_back(it);
+          Fmax[bk] = std::max(Fmax[bk], dmax);
+        } else if (use_chain_mfma && nt <= CHAIN_MFMA_NT) {
           for (auto& it : items) cbm[nt - 1].push_back(it);
         } else {
           for (auto& it : items) {
@@ -604,14 +617,17 @@ void Ctx::prepare(int st) {
     // offsets of every part in the concatenated lists (cbm: all parts of tile count 1,
     // then of 2, ...), then a parallel copy into page-locked staging with the indices
     // shifted, and asynchronous uploads (complete at the synchronisation ending prepare)
-    enum { L_CG, L_CT, L_CR, L_CB, L_CBL, L_REST, L_M0 };
+    enum { L_CG, L_CT, L_CR, L_CB, L_CBL, L_REST, L_F0, L_F1, L_M0 };
     const int NL = L_M0 + CHAIN_MFMA_NT;
     std::vector<int64_t> off((size_t)NL * (nparts + 1), 0), poff(nparts + 1, 0);
     auto O = [&](int l, int64_t pi) -> int64_t& { return off[(size_t)l * (nparts + 1) + pi]; };
-    int Pmax = 1, PmaxL = 1;
+    int Pmax = 1, PmaxL = 1, Fmx[2] = {1, 1};
     for (int64_t pi = 0; pi < nparts; pi++) {
       const Part& P = parts[pi];
-      const size_t sz[NL - CHAIN_MFMA_NT] = {P.cg.size(), P.ct.size(), P.cr.size(), P.cb.size(), P.cbl.size(), P.rest.size()};
+      const size_t sz[NL - CHAIN_MFMA_NT] = {P.cg.size(),  P.ct.size(),     P.cr.size(),    P.cb.size(), P.cbl.size(),
+                                             P.rest.size(), P.cbf[0].size(), P.cbf[1].size()};
+      Fmx[0] = std::max(Fmx[0], P.Fmax[0]);
+      Fmx[1] = std::max(Fmx[1], P.Fmax[1]);
       for (int l = 0; l < L_M0; l++) O(l, pi + 1) = O(l, pi) + (int64_t)sz[l];
       for (int n = 0; n < CHAIN_MFMA_NT; n++) O(L_M0 + n, pi + 1) = O(L_M0 + n, pi) + (int64_t)P.cbm[n].size();
       poff[pi + 1] = poff[pi] + P.cpart_n;
@@ -620,18 +636,20 @@ void Ctx::prepare(int st) {
     }
     const int64_t n_cg = O(L_CG, nparts), n_ct = O(L_CT, nparts), n_cr = O(L_CR, nparts), n_cb = O(L_CB, nparts),
                   n_cbl = O(L_CBL, nparts), n_rest = O(L_REST, nparts), cpart_n = poff[nparts];
+    const int64_t n_f[2] = {O(L_F0, nparts), O(L_F1, nparts)};
     int64_t n_m[CHAIN_MFMA_NT], n_mall = 0;
     for (int n = 0; n < CHAIN_MFMA_NT; n++) { n_m[n] = O(L_M0 + n, nparts); n_mall += n_m[n]; }
     if (n_ct > INT32_MAX || n_cg > INT32_MAX) throw Error{ERR_SIZE, "too many chained terms"};
     cstage.reset();
     cstage.reserve(8 * 256 + n_cg * sizeof(DevCGroup) + n_ct * sizeof(DevCTerm) + n_cr * sizeof(DevBRed) +
-                   (n_cb + n_cbl + n_mall) * sizeof(DevCBlock));
+                   (n_cb + n_cbl + n_mall + n_f[0] + n_f[1]) * sizeof(DevCBlock));
     DevCGroup* hcg = cstage.take<DevCGroup>(n_cg);
     DevCTerm* hct = cstage.take<DevCTerm>(n_ct);
     DevBRed* hcr = cstage.take<DevBRed>(n_cr);
     DevCBlock* hcb = cstage.take<DevCBlock>(n_cb);
     DevCBlock* hcbl = cstage.take<DevCBlock>(n_cbl);
     DevCBlock* hcbm = cstage.take<DevCBlock>(n_mall);
+    DevCBlock* hcbf[2] = {cstage.take<DevCBlock>(n_f[0]), cstage.take<DevCBlock>(n_f[1])};
     std::vector<const Term*> rest(n_rest);
     parallel_for(nparts, nthr, [&](int64_t pi, int) {
       const Part& P = parts[pi];
@@ -651,6 +669,8 @@ void Ctx::prepare(int st) {
       };
       shift(hcb + O(L_CB, pi), P.cb);
       shift(hcbl + O(L_CBL, pi), P.cbl);
+      shift(hcbf[0] + O(L_F0, pi), P.cbf[0]);
+      shift(hcbf[1] + O(L_F1, pi), P.cbf[1]);
       int64_t mbase = 0;
       for (int n = 0; n < CHAIN_MFMA_NT; n++) {
         shift(hcbm + mbase + O(L_M0 + n, pi), P.cbm[n]);
@@ -672,6 +692,11 @@ void Ctx::prepare(int st) {
     chain_P = std::min((Pmax + 31) / 32 * 32, CHAIN_SMALLP);
     up(cblocks_l, hcbl, n_cbl);
     ncblocks_l = (int)n_cbl;
+    for (int bk = 0; bk < 2; bk++) {
+      up(cblocks_f[bk], hcbf[bk], n_f[bk]);
+      ncblocks_f[bk] = (int)n_f[bk];
+      fchain_dmax[bk] = Fmx[bk];
+    }
     up(creds, hcr, n_cr);
     ncreds = (int)n_cr;
     cpart.alloc(std::max<int64_t>(cpart_n, 1));
@@ -919,6 +944,8 @@ void Ctx::apply_with(const double* xp, double* yp, double* bp, double* cp, doubl
   launch_chain_mfma(cblocks_m.p, cblocks_m_offs, cgroups.p, cterms.p, xp, yp, cp, stream);
   launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, cp, chain_P, false, stream);
   launch_chain(cblocks_l.p, ncblocks_l, cgroups.p, cterms.p, xp, yp, cp, chain_P_l, true, stream);
+  for (int bk = 0; bk < 2; bk++)
+    launch_fchain(cblocks_f[bk].p, ncblocks_f[bk], cgroups.p, cterms.p, xp, yp, cp, fchain_dmax[bk], stream);
   launch_chain_reduce(creds.p, ncreds, cp, yp, stream);
   if (n_lower > 0) {  // lower states, in state order (HeffDiagrams1.cpp:65-86)
     launch_multi_dot(lower_ptrs.p, n_lower, xp, V, lp, ld, stream);
@@ -1479,7 +1506,7 @@ void Ctx::build_eldims() {
 // Davidson.cpp:31-536 in eigenvalue mode with the reference's constants
 // (DAVIDSON_NUM_VEC = 32, DAVIDSON_NUM_VEC_KEEP = 3, DAVIDSON_PRECOND_CUTOFF = 1e-12;
 // Options.h:70-74). Vectors live on the device; the projected problem on the host.
-double Ctx::davidson(double* x, double rtol, int* n_matvec) {
+double Ctx::davidson_host(double* x, double rtol, int* n_matvec) {
   if (site < 0) throw Error{ERR_STATE, "no prepared site"};
   const int MAXV = 32, KEEP = 3;
   const double CUT = 1e-12;
@@ -1741,6 +1768,283 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
   return eig[0];
 }
 
+// Davidson eigensolver with its scalars on the device (Heff::SolveDAVIDSON_main,
+// Heff.cpp:330-386, with Davidson(V, 32, 3, rtol, 1e-12), Davidson.cpp:31-536). One
+// iteration is a fixed launch sequence: the Olsen correction (Davidson.cpp:322-350) with
+// the shift and its coefficient read on the device, orthogonalisation against the basis
+// (classical Gram-Schmidt applied twice) and normalisation, the Heff product, the new
+// column of V^T H V, the Rayleigh-Ritz step of the small matrix in one workgroup (cyclic
+// Jacobi, FP64), the Ritz vector and residual, and a copy of the scalars to page-locked
+// host memory. The sequence reads the basis size from the device record, so it is
+// captured once per solve as a HIP graph and replayed every iteration: one launch and
+// one synchronisation per iteration. Deflation at 32 vectors (Davidson.cpp:352-412,
+// 510-536), the mixed form's switch to refined products and convergence are decided on
+// the host from the copied scalars.
+namespace {
+// bytes of DevDav copied to the host after each iteration (the scalars before M)
+constexpr size_t kDavHead = offsetof(DevDav, M);
+}  // namespace
+
+double Ctx::davidson(double* x, double rtol, int* n_matvec) {
+  if (site < 0) throw Error{ERR_STATE, "no prepared site"};
+  if (harvest_per_iter > 0 || timing || !dav_device) return davidson_host(x, rtol, n_matvec);
+  const int MAXV = DAV_MAXV, KEEP = 3;
+  const double CUT = 1e-12;
+  const int64_t n = V;
+  DBuf<double> store;
+  store.alloc((size_t)(2 * MAXV + 6 + KEEP) * n);
+  std::vector<double*> vec(MAXV), hv(MAXV);
+  for (int i = 0; i < MAXV; i++) {
+    vec[i] = store.p + (size_t)i * n;
+    hv[i] = store.p + (size_t)(MAXV + i) * n;
+  }
+  double* t = store.p + (size_t)(2 * MAXV) * n;  // residual, then correction
+  double* u = t + n;                                // Ritz vector
+  double* work = u + n;
+  double* dg = work + n;                            // diagonal of Heff
+  double* xc = dg + n;                              // new basis vector
+  double* yc = xc + n;                              // its product
+  double* E = yc + n;                               // deflation
+  dav_vec.upload(vec, stream);
+  dav_hv.upload(hv, stream);
+  const int parts = dot_parts(n);
+  partial.alloc((size_t)std::max(parts, 1));
+  dav_part.alloc((size_t)MAXV * parts);
+  dav_state.alloc(1);
+  if (!dav_host) hip_check(hipHostMalloc((void**)&dav_host, sizeof(DevDav), hipHostMallocDefault), "hipHostMalloc");
+  DevDav* st = dav_state.p;
+  std::memset(dav_host, 0, sizeof(DevDav));
+  hip_check(hipMemcpyAsync(st, dav_host, sizeof(DevDav), hipMemcpyHostToDevice, stream), "Davidson state");
+  hip_check(hipStreamSynchronize(stream), "Davidson state");
+  double* const* dvec = dav_vec.p;
+  double* const* dhv = dav_hv.p;
+
+  // 'A': guess in the symmetric convention and the diagonal
+  hip_check(hipMemcpyAsync(t, x, n * sizeof(double), hipMemcpyDeviceToDevice, stream), "guess");
+  symm(t, true);
+  diag(dg);
+  {  // 'U': SafetyCheckGuess (Davidson.cpp:202-212)
+    launch_dot(t, t, n, partial.p, scal.p, stream);
+    double q = 0.0;
+    copy_sync(stream, &q, scal.p, sizeof(double), hipMemcpyDeviceToHost, "guess norm");
+    if (q == 0.0) {
+      std::vector<double> h(n);
+      uint64_t s = 0x9E3779B97F4A7C15ULL;
+      for (int64_t i = 0; i < n; i++) {
+        s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+        h[i] = (s >> 11) * (1.0 / 9007199254740992.0);
+      }
+      copy_sync(stream, t, h.data(), n * sizeof(double), hipMemcpyHostToDevice, "random guess");
+    }
+  }
+  struct LowPhase {
+    bool& f;
+    ~LowPhase() { f = false; }
+  } low_phase{mv_low};
+  mv_low = M_split;
+  int nmult = 0, napply = 0;
+  // correction from the residual in t: t = -(r + alpha u) / K with K = (diag - theta),
+  // or K / (1 + m) with the learned factor m, and alpha making t orthogonal to u in the
+  // Olsen sense (Davidson.cpp:322-350)
+  auto enqueue_P = [&](const float* mvals) {
+    launch_precond_dev(dg, u, &st->theta, CUT, work, t, n, 0, nullptr, stream, mvals);
+    launch_dot(work, t, n, partial.p, &st->wr[0], stream);
+    launch_dot(work, u, n, partial.p, &st->wr[1], stream);
+    launch_dav_alpha(st, stream);
+    launch_precond_dev(dg, u, &st->theta, CUT, work, t, n, 1, &st->alpha, stream, mvals);
+  };
+  // new vector from t, its product, and the Ritz step
+  auto enqueue_Q = [&]() {
+    for (int pass = 0; pass < 2; pass++) {
+      launch_dav_dots(st, 0, dvec, t, n, dav_part.p, parts, 1, stream);
+      launch_dav_project_out(st, dvec, t, n, stream);
+    }
+    launch_dot(t, t, n, partial.p, &st->nrm2, stream);
+    launch_dav_normalize(st, t, xc, n, stream);
+    launch_dav_clock(st, 0, stream);
+    apply(xc, yc);
+    launch_dav_clock(st, 1, stream);
+    launch_dav_store(st, dvec, dhv, xc, yc, n, stream);
+    launch_dav_dots(st, 1, dhv, xc, n, dav_part.p, parts, 0, stream);
+    launch_dav_eig(st, stream);
+    launch_dav_ritz(st, dvec, dhv, u, t, n, partial.p, parts, stream);
+    hip_check(hipMemcpyAsync(dav_host, st, kDavHead, hipMemcpyDeviceToHost, stream), "Davidson scalars");
+  };
+  // iteration graphs, captured on first use: [0] plain correction + new vector, [1] new
+  // vector only (after a correction made outside the graph)
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  auto drop_graphs = [&]() {
+    for (int g = 0; g < 2; g++) {
+      if (gexec[g]) (void)hipGraphExecDestroy(gexec[g]);
+      if (graph[g]) (void)hipGraphDestroy(graph[g]);
+      gexec[g] = nullptr;
+      graph[g] = nullptr;
+    }
+  };
+  struct GraphGuard {
+    std::function<void()> f;
+    ~GraphGuard() { f(); }
+  } graph_guard{drop_graphs};
+  auto run = [&](int g) {  // g = 0: P + Q, 1: Q
+    if (!dav_graph) {
+      if (g == 0) enqueue_P(nullptr);
+      enqueue_Q();
+      return;
+    }
+    if (!gexec[g]) {
+      hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "graph capture");
+      try {
+        if (g == 0) enqueue_P(nullptr);
+        enqueue_Q();
+      } catch (...) {
+        hipGraph_t gg = nullptr;
+        (void)hipStreamEndCapture(stream, &gg);
+        if (gg) (void)hipGraphDestroy(gg);
+        throw;
+      }
+      hip_check(hipStreamEndCapture(stream, &graph[g]), "graph capture end");
+      hip_check(hipGraphInstantiate(&gexec[g], graph[g], nullptr, nullptr, 0), "graph instantiate");
+      dav_graphs++;
+    }
+    hip_check(hipGraphLaunch(gexec[g], stream), "graph launch");
+    dav_graph_launches++;
+  };
+  auto count_apply = [&]() {
+    (mv_low ? n_low_mults : n_full_mults)++;
+    napply++;
+    nmult++;
+  };
+  auto sync_read = [&]() {
+    hip_check(hipStreamSynchronize(stream), "Davidson iteration");
+    if (dav_host->bad) throw Error{ERR_NUMERIC, "Davidson: new vector has zero or invalid norm"};
+  };
+  auto dot = [&](const double* a, const double* b) {
+    launch_dot(a, b, n, partial.p, scal.p, stream);
+    double q = 0.0;
+    copy_sync(stream, &q, scal.p, sizeof(double), hipMemcpyDeviceToHost, "dot");
+    return q;
+  };
+
+  enqueue_Q();
+  count_apply();
+  int host_nv = 1;
+  sync_read();
+  // learned preconditioner guard: the learned correction is used while every step it
+  // makes lowers the residual norm; the first step that does not switches this solve
+  // back to the plain diagonal correction of Davidson.cpp:322-350
+  bool learned_ok = precond_mode == 1 && mlp.ready, last_learned = false;
+  double prev_rnorm = HUGE_VAL;
+  for (int iter = 0; iter < 100000; iter++) {
+    const double rnorm = std::sqrt(dav_host->rn2);
+    const double theta = dav_host->theta;
+    if (!std::isfinite(rnorm)) throw Error{ERR_NUMERIC, "Davidson: residual is not finite"};
+    if (last_learned && !(rnorm < prev_rnorm)) {
+      learned_ok = false;
+      n_learned_fallbacks++;
+    }
+    prev_rnorm = rnorm;
+    if (mv_low && rnorm <= std::max(rtol, mixed_switch)) {  // refined products from the Ritz vector on
+      mv_low = false;
+      prev_rnorm = HUGE_VAL;
+      last_learned = false;
+      drop_graphs();
+      hip_check(hipMemcpyAsync(t, u, n * sizeof(double), hipMemcpyDeviceToDevice, stream), "refinement start");
+      DevDav h;
+      copy_sync(stream, &h, st, kDavHead, hipMemcpyDeviceToHost, "Davidson state");
+      h.nv = 0;
+      copy_sync(stream, st, &h, kDavHead, hipMemcpyHostToDevice, "basis reset");
+      enqueue_Q();
+      count_apply();
+      host_nv = 1;
+      sync_read();
+      continue;
+    }
+    if (rnorm <= rtol) break;
+    const bool learned = learned_ok;
+    last_learned = learned;
+    const float* mvals = nullptr;
+    if (learned) {
+      build_eldims();
+      featX.alloc((size_t)n * MLP_IN);
+      launch_features(dg, t, u, eldims.p, theta, std::sqrt((double)n) / rnorm, std::sqrt((double)n), n, featX.p, stream);
+      mlp.forward(featX.p, n, stream);
+      mvals = mlp.out.p;
+    }
+    if (host_nv == MAXV) {  // Davidson.cpp:352-412, 510-536
+      enqueue_P(mvals);
+      DevDav h;
+      copy_sync(stream, &h, st, sizeof(DevDav), hipMemcpyDeviceToHost, "Davidson state");
+      hip_check(hipMemcpyAsync(E, u, n * sizeof(double), hipMemcpyDeviceToDevice, stream), "deflate");
+      auto lincomb = [&](const std::vector<const double*>& vs, const std::vector<double>& cs, double* out) {
+        dav_ptrs.upload(vs, stream);
+        dav_coef.upload(cs, stream);
+        launch_lincomb_vecs(dav_ptrs.p, dav_coef.p, (int)vs.size(), out, n, stream);
+      };
+      for (int c = 1; c < KEEP; c++) {
+        std::vector<const double*> vs(vec.begin(), vec.end());
+        std::vector<double> cs(MAXV);
+        for (int v = 0; v < MAXV; v++) cs[v] = h.evec[v + (size_t)MAXV * c];
+        lincomb(vs, cs, E + (size_t)c * n);
+      }
+      std::vector<double> Sm(KEEP * KEEP);
+      for (int a = 0; a < KEEP; a++)
+        for (int b = a; b < KEEP; b++) Sm[a + KEEP * b] = Sm[b + KEEP * a] = dot(E + (size_t)a * n, E + (size_t)b * n);
+      std::vector<double> se, sv;
+      jacobi_eig(KEEP, Sm, KEEP, se, sv);
+      for (int c = 0; c < KEEP; c++) {
+        const double f = std::pow(se[c], -0.25);
+        for (int r = 0; r < KEEP; r++) sv[r + KEEP * c] *= f;
+      }
+      for (int v = 0; v < KEEP; v++) {
+        std::vector<const double*> vs;
+        std::vector<double> cs;
+        for (int v2 = 0; v2 < KEEP; v2++) {
+          double low = 0.0;
+          for (int c = 0; c < KEEP; c++) low += sv[v2 + KEEP * c] * sv[v + KEEP * c];
+          vs.push_back(E + (size_t)v2 * n);
+          cs.push_back(low);
+        }
+        lincomb(vs, cs, vec[v]);
+      }
+      for (int v = 0; v < KEEP; v++) {
+        launch_dav_clock(st, 0, stream);
+        apply(vec[v], hv[v]);
+        launch_dav_clock(st, 1, stream);
+        count_apply();
+      }
+      copy_sync(stream, &h, st, sizeof(DevDav), hipMemcpyDeviceToHost, "Davidson state");
+      for (int a = 0; a < KEEP; a++)
+        for (int b = a; b < KEEP; b++) h.M[a + MAXV * b] = h.M[b + MAXV * a] = dot(vec[a], hv[b]);
+      h.nv = KEEP;
+      copy_sync(stream, st, &h, sizeof(DevDav), hipMemcpyHostToDevice, "Davidson state");
+      dav_deflations++;
+      prev_rnorm = HUGE_VAL;
+      run(1);
+      host_nv = KEEP;
+    } else if (learned) {
+      enqueue_P(mvals);
+      run(1);
+    } else {
+      run(0);
+    }
+    count_apply();
+    host_nv++;
+    sync_read();
+  }
+  const double theta = dav_host->theta;
+  if (mv_timing) {
+    mv_flops += F_H * napply;
+    mv_ms += (double)dav_host->heff_ticks / wall_khz;
+  }
+  // 'C': copy back and convert to the program convention
+  hip_check(hipMemcpyAsync(x, u, n * sizeof(double), hipMemcpyDeviceToDevice, stream), "solution");
+  symm(x, false);
+  hip_check(hipStreamSynchronize(stream), "davidson");
+  if (n_matvec) *n_matvec = nmult;
+  return theta;
+}
+
 }  // namespace chm
 
 // ================================================================= C ABI =====
@@ -1790,11 +2094,23 @@ int chm_ctx_create(int device, chm_ctx** out) {
   ctx->c.def_item_terms = ctx->c.chain_item_terms;
   ctx->c.def_group_terms = ctx->c.chain_group_terms;
   ctx->c.def_chain_mfma = ctx->c.use_chain_mfma;
+  if (const char* e = std::getenv("CHM_FCHAIN")) ctx->c.use_fchain = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CHM_FCHAIN_MIN")) ctx->c.fchain_min = std::atoi(e);
+  ctx->c.def_fchain_min = ctx->c.fchain_min;
+  if (const char* e = std::getenv("CHM_DAVIDSON_DEVICE")) ctx->c.dav_device = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CHM_DAVIDSON_GRAPH")) ctx->c.dav_graph = std::atoi(e) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return CHM_ERR_HIP;
   }
   ctx->c.chain_large_ok = chm::chain_kernel_setup();
+  ctx->c.fchain_ok = chm::fchain_kernel_setup();
+  (void)hipGetLastError();
+  if (!ctx->c.fchain_ok) ctx->c.use_fchain = false;
+  {
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0) ctx->c.wall_khz = khz;
+  }
   (void)hipGetLastError();  // a refused attribute leaves no error behind
   for (auto& e : ctx->c.ev)
     if (hipEventCreate(&e) != hipSuccess) { delete ctx; return CHM_ERR_HIP; }
@@ -1969,6 +2285,14 @@ int chm_dmrg_2rdm(chm_ctx* ctx, double* A, double* B, double* energy) {
   });
 }
 
+int chm_davidson_stats(const chm_ctx* ctx, int64_t* out) {
+  if (!ctx || !out) return CHM_ERR_ARG;
+  out[0] = ctx->c.dav_deflations;
+  out[1] = ctx->c.dav_graphs;
+  out[2] = ctx->c.dav_graph_launches;
+  return CHM_OK;
+}
+
 int chm_dmrg_nstates(const chm_ctx* ctx, int* n) {
   if (!ctx || !n) return CHM_ERR_ARG;
   *n = (int)ctx->c.lowers.size() + 1;
@@ -2028,12 +2352,14 @@ int chm_dmrg_heff_stats(const chm_ctx* ctx, double* flops, double* kernel_ms) {
   return CHM_OK;
 }
 
-int chm_set_heff_knobs(chm_ctx* ctx, int64_t explicit_max, int item_terms, int group_terms, int chain_mfma) {
+int chm_set_heff_knobs(chm_ctx* ctx, int64_t explicit_max, int item_terms, int group_terms, int chain_mfma,
+                       int fchain_min) {
   return guard(ctx, [&](Ctx& c) {
     if (explicit_max >= 0) c.def_explicit_max = explicit_max;
     if (item_terms > 0) c.def_item_terms = item_terms;
     if (group_terms > 0) c.def_group_terms = group_terms;
     if (chain_mfma >= 0) c.def_chain_mfma = chain_mfma != 0;
+    if (fchain_min >= 0) c.def_fchain_min = fchain_min;
     c.site = -1;
   });
 }
@@ -2043,9 +2369,9 @@ int chm_set_tuning(chm_ctx* ctx, int n, const int64_t* rows) {
     if (n < 0 || (n > 0 && !rows)) throw Error{CHM_ERR_ARG, "bad tuning table"};
     std::vector<Ctx::TuneRow> t;
     for (int r = 0; r < n; r++) {
-      const int64_t* v = rows + 8 * r;
+      const int64_t* v = rows + 9 * r;
       if (v[0] > v[1] || v[2] > v[3]) throw Error{CHM_ERR_ARG, "tuning row with an empty range"};
-      t.push_back({v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]});
+      t.push_back({v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]});
     }
     c.tune = std::move(t);
     c.site = -1;
@@ -2056,7 +2382,7 @@ int chm_tuning_used(const chm_ctx* ctx, int64_t* out) {
   if (!ctx || !out) return CHM_ERR_ARG;
   const Ctx& c = ctx->c;
   out[0] = c.tune_row; out[1] = c.site_dmax; out[2] = c.explicit_max; out[3] = c.chain_item_terms;
-  out[4] = c.chain_group_terms; out[5] = c.use_chain_mfma ? 1 : 0;
+  out[4] = c.chain_group_terms; out[5] = c.use_chain_mfma ? 1 : 0; out[6] = c.fchain_min;
   return CHM_OK;
 }
 

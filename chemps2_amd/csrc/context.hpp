@@ -210,11 +210,11 @@ struct Ctx : HostState {
   uint64_t hv_counter = 0;
   void build_eldims();
   // Autotuned routing (chm_set_tuning): rows keyed by the site's largest block dimension
-  // and V select explicit_max, chain_item_terms, chain_group_terms and use_chain_mfma
-  // for that site; sites no row covers use the defaults (chm_set_heff_knobs, the
+  // and V select explicit_max, chain_item_terms, chain_group_terms, use_chain_mfma and
+  // fchain_min for that site; sites no row covers use the defaults (chm_set_heff_knobs, the
   // CHM_* environment knobs or the built-in values).
   struct TuneRow {
-    int64_t dmax_lo, dmax_hi, v_lo, v_hi, explicit_max, item_terms, group_terms, chain_mfma;
+    int64_t dmax_lo, dmax_hi, v_lo, v_hi, explicit_max, item_terms, group_terms, chain_mfma, fchain_min;
   };
   std::vector<TuneRow> tune;
   int64_t def_explicit_max = 4096;
@@ -239,6 +239,16 @@ struct Ctx : HostState {
   DBuf<DevCTerm> cterms;
   int ncblocks = 0;
   int chain_P = 1;  // doubles per LDS array of the chained kernel
+  // k_fchain (chained products of blocks up to FCHAIN_MAXD on the matrix cores): work
+  // items of output blocks whose largest dimension is at most 64 / above, and that
+  // dimension's maximum over each list (sets the launch's LDS). CHM_FCHAIN=0 turns it
+  // off; blocks whose largest dimension is at most fchain_min (CHM_FCHAIN_MIN) stay on
+  // k_chain.
+  bool use_fchain = true, fchain_ok = false;
+  int fchain_min = 32, def_fchain_min = 32;
+  DBuf<DevCBlock> cblocks_f[2];
+  int ncblocks_f[2] = {0, 0};
+  int fchain_dmax[2] = {1, 1};
   DBuf<DevCBlock> cblocks_l;  // blocks of the large variant (same groups and terms)
   int ncblocks_l = 0;
   int chain_P_l = 1;
@@ -361,7 +371,20 @@ struct Ctx : HostState {
     site = -1;
   }
   void diag(double* d);
+  // Davidson with its scalars on the device and one iteration captured as a HIP graph
+  // (davidson); the host-driven form (davidson_host) serves the learned preconditioner,
+  // sample harvesting and the per-kernel timing mode. CHM_DAVIDSON_DEVICE=0 selects the
+  // host-driven form everywhere, CHM_DAVIDSON_GRAPH=0 runs the device form without graphs.
   double davidson(double* x, double rtol, int* n_matvec);
+  double davidson_host(double* x, double rtol, int* n_matvec);
+  bool dav_device = true, dav_graph = true;
+  DBuf<double*> dav_vec, dav_hv;
+  DBuf<const double*> dav_ptrs;
+  DBuf<double> dav_coef, dav_part;
+  DBuf<DevDav> dav_state;
+  DevDav* dav_host = nullptr;  // page-locked copy of the scalars after each iteration
+  double wall_khz = 1e5;       // device wall clock (wall_clock64) ticks per ms
+  int64_t dav_deflations = 0, dav_graphs = 0, dav_graph_launches = 0;
   void symm(double* x, bool to_symm);
 };
 

@@ -742,6 +742,213 @@ __global__ __launch_bounds__(256) void k_chain_mfma(const DevCBlock* __restrict_
 }
 #undef CHM_MFMA
 
+// Chained products of blocks up to FCHAIN_MAXD on the matrix cores, one workgroup per
+// work item (the records of k_chain: an output block and a range of its input groups),
+// output-stationary. The output block is split into 16 x 16 tiles dealt round-robin to
+// the four wavefronts, each holding its tiles as v_mfma_f64_16x16x4f64 accumulators
+// (lane l = lr + 16 lg supplies A(lr, k0 + lg) and B(k0 + lg, lr) and holds D(4 r + lg, lr)
+// in register r). Per input group x is staged in LDS, zero-padded to whole tiles, with an
+// odd leading dimension (conflict-free reads along both operand directions); a
+// two-operand term first forms its intermediate W = alpha op(A) x or x op(B) in LDS, all
+// waves sharing its tiles, then every wave adds W op(B) or alpha op(A) W to its tiles.
+// Operator blocks are read from global memory (cache-resident across the terms of an
+// item); their reads past a dimension are masked to zero.
+constexpr int FCHAIN_XS = (FCHAIN_MAXD + 1) * FCHAIN_MAXD;  // doubles of one LDS array
+
+__device__ __forceinline__ double gld(const double* __restrict__ p, bool ok, int64_t idx) {
+  const double v = p[ok ? idx : 0];
+  return ok ? v : 0.0;
+}
+
+#define FMFMA(a, b, c) __builtin_amdgcn_mfma_f64_16x16x4f64((a), (b), (c), 0, 0, 0)
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_fchain(const DevCBlock* __restrict__ blocks, const DevCGroup* __restrict__ groups,
+                                               const DevCTerm* __restrict__ terms, const double* __restrict__ x,
+                                               double* __restrict__ y, double* __restrict__ part, int lds_stride) {
+  extern __shared__ double lds[];
+  double* Xs = lds;
+  double* Ws = lds + lds_stride;
+  const DevCBlock b = blocks[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, lr = lane & 15, lg = lane >> 4;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int dL = b.dL, dR = b.dR;
+  const int nL = (dL + 15) >> 4, nR = (dR + 15) >> 4, nt = nL * nR;
+  chm_d4 acc[NT];
+#pragma unroll
+  for (int s = 0; s < NT; s++) acc[s] = chm_d4{0.0, 0.0, 0.0, 0.0};
+  for (int g = b.g0; g < b.g0 + b.ng; g++) {
+    const DevCGroup G = groups[g];
+    const int dLi = G.dLi, dRi = G.dRi;
+    const int nLi = (dLi + 15) >> 4, nRi = (dRi + 15) >> 4;
+    const int ldx = 16 * nLi + 1;
+    __syncthreads();  // the previous group's readers of Xs and Ws are done
+    {
+      const double* __restrict__ X = x + G.xoff;
+      const int rows = 16 * nLi, cnt = rows * 16 * nRi;
+      for (int e = tid; e < cnt; e += 256) {
+        const int i = e % rows, j = e / rows;
+        Xs[i + ldx * j] = (i < dLi && j < dRi) ? X[i + (int64_t)dLi * j] : 0.0;
+      }
+    }
+    __syncthreads();
+    for (int t = G.t0; t < G.t0 + G.nt; t++) {
+      const DevCTerm T = terms[t];
+      const double al = T.alpha;
+      // op(A)(i, k) = A[i ai + k ak] (dL x dLi), op(B)(k, j) = B[k bk + j bj] (dRi x dR)
+      const int64_t ai = T.ta ? T.lda : 1, ak = T.ta ? 1 : T.lda;
+      const int64_t bk = T.tb ? T.ldb : 1, bj = T.tb ? 1 : T.ldb;
+      const double* __restrict__ A = T.A ? T.A : x;  // absent operands are never read
+      const double* __restrict__ B = T.B ? T.B : x;
+      const int mode = T.mode;
+      if (mode == 3) {  // W = al op(A) X (16 nL x 16 nRi), then Y += W op(B)
+        const int ldw = 16 * nL + 1;
+        for (int q = w; q < nL * nRi; q += 4) {
+          const int ti = q % nL, tj = q / nL;
+          const int i = 16 * ti + lr;
+          chm_d4 d = chm_d4{0.0, 0.0, 0.0, 0.0};
+          for (int k0 = 0; k0 < 16 * nLi; k0 += 16) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int k = k0 + 4 * u + lg;
+              av[u] = gld(A, i < dL && k < dLi, i * ai + k * ak);
+              bv[u] = Xs[k + ldx * (16 * tj + lr)];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) d = FMFMA(av[u], bv[u], d);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; r++) Ws[(16 * ti + 4 * r + lg) + ldw * (16 * tj + lr)] = al * d[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NT; s++) {
+          const int q = w + 4 * s;
+          if (q >= nt) break;
+          const int ti = q % nL, tj = q / nL;
+          const int j = 16 * tj + lr;
+          for (int k0 = 0; k0 < 16 * nRi; k0 += 16) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int k = k0 + 4 * u + lg;
+              av[u] = Ws[(16 * ti + lr) + ldw * k];
+              bv[u] = gld(B, k < dRi && j < dR, k * bk + j * bj);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[s] = FMFMA(av[u], bv[u], acc[s]);
+          }
+        }
+        __syncthreads();  // Ws is rewritten by the next term
+      } else if (mode == 4) {  // W = X op(B) (16 nLi x 16 nR), then Y += al op(A) W
+        const int ldw = 16 * nLi + 1;
+        for (int q = w; q < nLi * nR; q += 4) {
+          const int ti = q % nLi, tj = q / nLi;
+          const int j = 16 * tj + lr;
+          chm_d4 d = chm_d4{0.0, 0.0, 0.0, 0.0};
+          for (int k0 = 0; k0 < 16 * nRi; k0 += 16) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int k = k0 + 4 * u + lg;
+              av[u] = Xs[(16 * ti + lr) + ldx * k];
+              bv[u] = gld(B, k < dRi && j < dR, k * bk + j * bj);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) d = FMFMA(av[u], bv[u], d);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; r++) Ws[(16 * ti + 4 * r + lg) + ldw * (16 * tj + lr)] = d[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NT; s++) {
+          const int q = w + 4 * s;
+          if (q >= nt) break;
+          const int ti = q % nL, tj = q / nL;
+          const int i = 16 * ti + lr;
+          for (int k0 = 0; k0 < 16 * nLi; k0 += 16) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int k = k0 + 4 * u + lg;
+              av[u] = al * gld(A, i < dL && k < dLi, i * ai + k * ak);
+              bv[u] = Ws[k + ldw * (16 * tj + lr)];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[s] = FMFMA(av[u], bv[u], acc[s]);
+          }
+        }
+        __syncthreads();
+      } else if (mode == 1) {  // Y += al op(A) X
+#pragma unroll
+        for (int s = 0; s < NT; s++) {
+          const int q = w + 4 * s;
+          if (q >= nt) break;
+          const int ti = q % nL, tj = q / nL;
+          const int i = 16 * ti + lr;
+          for (int k0 = 0; k0 < 16 * nLi; k0 += 16) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int k = k0 + 4 * u + lg;
+              av[u] = al * gld(A, i < dL && k < dLi, i * ai + k * ak);
+              bv[u] = Xs[k + ldx * (16 * tj + lr)];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[s] = FMFMA(av[u], bv[u], acc[s]);
+          }
+        }
+      } else if (mode == 2) {  // Y += al X op(B)
+#pragma unroll
+        for (int s = 0; s < NT; s++) {
+          const int q = w + 4 * s;
+          if (q >= nt) break;
+          const int ti = q % nL, tj = q / nL;
+          const int j = 16 * tj + lr;
+          for (int k0 = 0; k0 < 16 * nRi; k0 += 16) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int k = k0 + 4 * u + lg;
+              av[u] = al * Xs[(16 * ti + lr) + ldx * k];
+              bv[u] = gld(B, k < dRi && j < dR, k * bk + j * bj);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[s] = FMFMA(av[u], bv[u], acc[s]);
+          }
+        }
+      } else {  // Y += al X (same dimensions)
+#pragma unroll
+        for (int s = 0; s < NT; s++) {
+          const int q = w + 4 * s;
+          if (q >= nt) break;
+          const int ti = q % nL, tj = q / nL;
+#pragma unroll
+          for (int r = 0; r < 4; r++) acc[s][r] += al * Xs[(16 * ti + 4 * r + lg) + ldx * (16 * tj + lr)];
+        }
+      }
+    }
+  }
+  double* Y = b.direct ? y + b.out : part + b.out;
+#pragma unroll
+  for (int s = 0; s < NT; s++) {
+    const int q = w + 4 * s;
+    if (q >= nt) break;
+    const int ti = q % nL, tj = q / nL;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int l = 16 * ti + 4 * r + lg, c = 16 * tj + lr;
+      if (l < dL && c < dR) {
+        if (b.direct) Y[l + (int64_t)dL * c] += acc[s][r];
+        else Y[l + (int64_t)dL * c] = acc[s][r];
+      }
+    }
+  }
+}
+#undef FMFMA
+
 constexpr int CHAIN_PER_S = (18 * 18 + 255) / 256;                                  // 2 x 2 tiles of a 36 x 36 block
 constexpr int CHAIN_PER_L = ((CHAIN_MAXD / 2) * (CHAIN_MAXD / 2) + 255) / 256;     // of a 64 x 64 block
 
@@ -918,7 +1125,10 @@ __global__ __launch_bounds__(256) void k_lincomb_vecs(const double* const* __res
 __global__ __launch_bounds__(256) void k_precond(const double* __restrict__ diag, const double* __restrict__ u,
                                                  double shift, double cutoff, double* __restrict__ work,
                                                  double* __restrict__ t, int64_t n, int stage, double alpha,
-                                                 const float* __restrict__ m) {
+                                                 const float* __restrict__ m, const double* __restrict__ dshift,
+                                                 const double* __restrict__ dalpha) {
+  if (dshift) shift = *dshift;
+  if (dalpha) alpha = *dalpha;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
     const double d = diag[i] - shift;
     const double k = (fabs(d) > cutoff) ? d : cutoff;
@@ -973,7 +1183,10 @@ __global__ __launch_bounds__(256) void k_copies(const DevCopy* __restrict__ item
 // has |a_p . a_q| > sqrt(m) eps |a_p| |a_q|, the rounding floor of an m-term dot product
 // (the stopping rule of one-sided Jacobi in Drmac & Veselic, SIAM J. Matrix Anal. Appl.
 // 29 (2008) 1322, as LAPACK's dgesvj uses it). A job that reaches SVD_MAX_SWEEPS without
-// meeting it sets *info = 1.
+// meeting it sets *info = 1. Columns whose norm is at most eps |A|_F are numerically zero:
+// they are not rotated (their directions are rounding noise and would keep the sweeps
+// going), their singular values are set to zero, and their vectors on the m-long side are
+// completed to an orthonormal set like those of exactly zero singular values.
 constexpr int SVD_TPP = 16;
 constexpr int SVD_GROUPS = 256 / SVD_TPP;
 constexpr double SVD_EPS = 1.1102230246251565e-16;  // 2^-53
@@ -993,8 +1206,17 @@ __global__ __launch_bounds__(256) void k_svd_jacobi(const DevSvd* __restrict__ i
   __shared__ int rotated;
   __shared__ double sig[SVD_JACOBI_MAXN];
   __shared__ int src[SVD_JACOBI_MAXN];
+  __shared__ double fro[256];
+  double f2 = 0.0;
+  for (int e = tid; e < m * n; e += 256) f2 += A[e] * A[e];
+  fro[tid] = f2;
   for (int e = tid; e < n * n; e += 256) Q[e] = (e % n == e / n) ? 1.0 : 0.0;
   __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) fro[tid] += fro[tid + h];
+    __syncthreads();
+  }
+  const double floor2 = SVD_EPS * SVD_EPS * fro[0];
   const int n2 = n + (n & 1);
   const double tol = sqrt((double)m) * SVD_EPS;
   bool converged = n <= 1;
@@ -1018,7 +1240,7 @@ __global__ __launch_bounds__(256) void k_svd_jacobi(const DevSvd* __restrict__ i
             al += x * x; be += y * y; ga += x * y;
           }
         al = group_sum(al); be = group_sum(be); ga = group_sum(ga);
-        if (p >= 0 && ga != 0.0 && al > 0.0 && be > 0.0 && fabs(ga) > tol * sqrt(al * be)) {
+        if (p >= 0 && ga != 0.0 && al > floor2 && be > floor2 && al > 0.0 && be > 0.0 && fabs(ga) > tol * sqrt(al * be)) {
           const double zeta = (be - al) / (2.0 * ga);
           const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
           const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
@@ -1051,7 +1273,7 @@ __global__ __launch_bounds__(256) void k_svd_jacobi(const DevSvd* __restrict__ i
     double s2 = 0.0;
     for (int i = l; i < m; i += SVD_TPP) s2 += A[i + (size_t)m * j] * A[i + (size_t)m * j];
     s2 = group_sum(s2);
-    if (l == 0) sig[j] = sqrt(s2);
+    if (l == 0) sig[j] = s2 > floor2 ? sqrt(s2) : 0.0;
   }
   __syncthreads();
   for (int j = tid; j < n; j += 256) {
@@ -1108,7 +1330,258 @@ __global__ __launch_bounds__(256) void k_svd_jacobi(const DevSvd* __restrict__ i
   }
 }
 
-// x += level * (u - 1/2) with u uniform in [0, 1) from a splitmix64 hash of the index.
+// ---------------------------------------------------------- Davidson on the device ---
+// The scalars and small matrices of one Davidson solve (Davidson.cpp:105-191, 214-350) in
+// a DevDav record on the device; the kernels read the basis size from it, so the same
+// launch sequence (and one captured graph) serves every iteration.
+__global__ __launch_bounds__(256) void k_dav_store(const DevDav* __restrict__ st, double* const* __restrict__ vec,
+                                                   double* const* __restrict__ hv, const double* __restrict__ x,
+                                                   const double* __restrict__ y, int64_t n) {
+  const int c = st->nv;
+  double* __restrict__ v = vec[c];
+  double* __restrict__ h = hv[c];
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    v[i] = x[i];
+    h[i] = y[i];
+  }
+}
+
+// partial[c * gridDim.x + b] = chunk b of a[c] . b for c < nv + add
+__global__ __launch_bounds__(256) void k_dav_dots_partial(const DevDav* __restrict__ st, int add,
+                                                          const double* const* __restrict__ a,
+                                                          const double* __restrict__ b, int64_t n,
+                                                          double* __restrict__ partial) {
+  __shared__ double red[256];
+  const int c = blockIdx.y;
+  if (c >= st->nv + add) return;  // uniform over the block: no barrier is skipped by part of it
+  const double* __restrict__ p = a[c];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) s = fma(p[i], b[i], s);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(int64_t)c * gridDim.x + blockIdx.x] = red[0];
+}
+
+// mode 0: M[c, nv] = M[nv, c] = sum of the partials (the new column of V^T H V);
+// mode 1: proj[c] = sum of the partials (projections of the new vector)
+__global__ __launch_bounds__(256) void k_dav_dots_reduce(DevDav* __restrict__ st, int add,
+                                                         const double* __restrict__ partial, int parts, int mode) {
+  __shared__ double red[256];
+  const int c = blockIdx.x;
+  const int nv = st->nv;
+  if (c >= nv + add) return;
+  double s = 0.0;
+  for (int q = threadIdx.x; q < parts; q += 256) s += partial[(int64_t)c * parts + q];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (mode == 0) {
+      st->M[c + DAV_MAXV * nv] = red[0];
+      st->M[nv + DAV_MAXV * c] = red[0];
+    } else {
+      st->proj[c] = red[0];
+    }
+  }
+}
+
+// t -= sum_{c < nv} proj[c] vec[c]
+__global__ __launch_bounds__(256) void k_dav_project_out(const DevDav* __restrict__ st,
+                                                         const double* const* __restrict__ vec, double* __restrict__ t,
+                                                         int64_t n) {
+  __shared__ double pr[DAV_MAXV];
+  __shared__ const double* vp[DAV_MAXV];
+  const int nv = st->nv;
+  if (threadIdx.x < DAV_MAXV) {
+    pr[threadIdx.x] = threadIdx.x < nv ? st->proj[threadIdx.x] : 0.0;
+    vp[threadIdx.x] = threadIdx.x < nv ? vec[threadIdx.x] : nullptr;
+  }
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double v = t[i];
+    for (int c = 0; c < nv; c++) v -= pr[c] * vp[c][i];
+    t[i] = v;
+  }
+}
+
+// out = t / |t| with |t|^2 = st->nrm2; a zero or invalid norm marks the solve as failed
+__global__ __launch_bounds__(256) void k_dav_normalize(DevDav* __restrict__ st, const double* __restrict__ t,
+                                                       double* __restrict__ out, int64_t n) {
+  const double q = st->nrm2;
+  const bool ok = q > 0.0 && q < 1e300;
+  if (!ok && blockIdx.x == 0 && threadIdx.x == 0) st->bad = 1;
+  const double f = ok ? 1.0 / sqrt(q) : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) out[i] = f * t[i];
+}
+
+// Eigen-decomposition of the (nv + 1) x (nv + 1) leading block of M by cyclic Jacobi in
+// one workgroup, parallel ordering (disjoint pairs of each round-robin step rotated at
+// once: rows first, then columns), until no off-diagonal element is above 1e-17 of the
+// geometric mean of its diagonal pair (or 60 sweeps). Eigenvalues ascending with their
+// vectors (columns of evec, leading dimension DAV_MAXV); nv becomes nv + 1.
+__global__ __launch_bounds__(256) void k_dav_eig(DevDav* __restrict__ st) {
+  __shared__ double A[DAV_MAXV][DAV_MAXV + 1];
+  __shared__ double Q[DAV_MAXV][DAV_MAXV + 1];
+  __shared__ double cs[DAV_MAXV / 2], sn[DAV_MAXV / 2];
+  __shared__ int pp[DAV_MAXV / 2], pq[DAV_MAXV / 2];
+  __shared__ int rot;
+  __shared__ int order[DAV_MAXV];
+  const int tid = threadIdx.x;
+  const int n = st->nv + 1;
+  const int m = n + (n & 1);  // even number of slots for the round-robin (a dummy index past n)
+  for (int e = tid; e < DAV_MAXV * DAV_MAXV; e += 256) {
+    const int r = e % DAV_MAXV, c = e / DAV_MAXV;
+    A[r][c] = (r < n && c < n) ? st->M[r + DAV_MAXV * c] : 0.0;
+    Q[r][c] = (r == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < 60; sweep++) {
+    if (tid == 0) rot = 0;
+    __syncthreads();
+    for (int step = 0; step < m - 1; step++) {
+      // round-robin pairing: slot 0 fixed, the others rotate
+      if (tid < m / 2) {
+        auto slot = [&](int k) { return k == 0 ? 0 : 1 + (k - 1 + step) % (m - 1); };
+        int p = slot(tid), q = slot(m - 1 - tid);
+        if (p > q) { const int w = p; p = q; q = w; }
+        double c = 1.0, s = 0.0;
+        if (q < n) {
+          const double apq = A[p][q];
+          const double app = A[p][p], aqq = A[q][q];
+          if (fabs(apq) > 1e-17 * sqrt(fabs(app * aqq)) && apq != 0.0) {
+            const double th = (aqq - app) / (2.0 * apq);
+            const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+            c = 1.0 / sqrt(t * t + 1.0);
+            s = t * c;
+            rot = 1;
+          }
+        }
+        pp[tid] = p; pq[tid] = q; cs[tid] = c; sn[tid] = s;
+      }
+      __syncthreads();
+      // rows p, q of A  <-  J^T A
+      for (int e = tid; e < (m / 2) * n; e += 256) {
+        const int k = e % n, w = e / n;
+        const int p = pp[w], q = pq[w];
+        if (q >= n) continue;
+        const double c = cs[w], s = sn[w];
+        const double ap = A[p][k], aq = A[q][k];
+        A[p][k] = c * ap - s * aq;
+        A[q][k] = s * ap + c * aq;
+      }
+      __syncthreads();
+      // columns p, q of A and of Q  <-  A J, Q J
+      for (int e = tid; e < (m / 2) * n; e += 256) {
+        const int k = e % n, w = e / n;
+        const int p = pp[w], q = pq[w];
+        if (q >= n) continue;
+        const double c = cs[w], s = sn[w];
+        const double ap = A[k][p], aq = A[k][q];
+        A[k][p] = c * ap - s * aq;
+        A[k][q] = s * ap + c * aq;
+        const double qp = Q[k][p], qq = Q[k][q];
+        Q[k][p] = c * qp - s * qq;
+        Q[k][q] = s * qp + c * qq;
+      }
+      __syncthreads();
+    }
+    if (rot == 0) break;
+    __syncthreads();
+  }
+  if (tid == 0) {  // ascending order
+    for (int i = 0; i < n; i++) order[i] = i;
+    for (int i = 1; i < n; i++) {
+      const int v = order[i];
+      int j = i - 1;
+      while (j >= 0 && A[order[j]][order[j]] > A[v][v]) { order[j + 1] = order[j]; j--; }
+      order[j + 1] = v;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < n * n; e += 256) {
+    const int r = e % n, c = e / n;
+    st->evec[r + DAV_MAXV * c] = Q[r][order[c]];
+  }
+  if (tid < n) st->eig[tid] = A[order[tid]][order[tid]];
+  __syncthreads();
+  if (tid == 0) {
+    st->theta = A[order[0]][order[0]];
+    st->nv = n;
+  }
+}
+
+// u = sum_c evec[c, 0] vec[c], r = sum_c evec[c, 0] hv[c] - theta u, partial sums of r^2
+__global__ __launch_bounds__(256) void k_dav_ritz(const DevDav* __restrict__ st, const double* const* __restrict__ vec,
+                                                  const double* const* __restrict__ hv, double* __restrict__ u,
+                                                  double* __restrict__ r, int64_t n, double* __restrict__ partial) {
+  __shared__ double cf[DAV_MAXV];
+  __shared__ const double* vp[DAV_MAXV];
+  __shared__ const double* hp[DAV_MAXV];
+  __shared__ double red[256];
+  const int nv = st->nv;
+  const double th = st->theta;
+  if (threadIdx.x < DAV_MAXV) {
+    const bool in = threadIdx.x < nv;
+    cf[threadIdx.x] = in ? st->evec[threadIdx.x] : 0.0;
+    vp[threadIdx.x] = in ? vec[threadIdx.x] : nullptr;
+    hp[threadIdx.x] = in ? hv[threadIdx.x] : nullptr;
+  }
+  __syncthreads();
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double a = 0.0, b = 0.0;
+    for (int c = 0; c < nv; c++) {
+      a = fma(cf[c], vp[c][i], a);
+      b = fma(cf[c], hp[c][i], b);
+    }
+    const double ri = b - th * a;
+    u[i] = a;
+    r[i] = ri;
+    s = fma(ri, ri, s);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// st->rn2 = sum of the partials; st->alpha from the two preconditioner dots
+__global__ __launch_bounds__(256) void k_dav_scalar(DevDav* __restrict__ st, const double* __restrict__ partial, int parts,
+                                                    int what) {
+  __shared__ double red[256];
+  if (what == 1) {  // Olsen coefficient alpha = -(w . r) / (w . u) (Davidson.cpp:331-340)
+    if (threadIdx.x == 0) st->alpha = -st->wr[0] / st->wr[1];
+    return;
+  }
+  double s = 0.0;
+  for (int q = threadIdx.x; q < parts; q += 256) s += partial[q];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) st->rn2 = red[0];
+}
+
+// device wall clock around the Heff product: which 0 starts, 1 adds the elapsed ticks
+__global__ void k_dav_clock(DevDav* __restrict__ st, int which) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const unsigned long long t = wall_clock64();
+  if (which == 0) st->t0 = t;
+  else st->heff_ticks += t - st->t0;
+}
+
 // Dot products of array pairs, one workgroup per pair: four strided partial sums per
 // thread, then a fixed-order tree over the workgroup.
 __global__ __launch_bounds__(256) void k_dots(const DevDot* __restrict__ items, double* __restrict__ out) {
@@ -1133,6 +1606,7 @@ __global__ __launch_bounds__(256) void k_dots(const DevDot* __restrict__ items, 
   if (t == 0) out[blockIdx.x] = red[0];
 }
 
+// x += level * (u - 1/2) with u uniform in [0, 1) from a splitmix64 hash of the index.
 __global__ __launch_bounds__(256) void k_add_noise(double* __restrict__ x, int64_t n, uint64_t seed, double level) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
     uint64_t z = seed * 0x9E3779B97F4A7C15ULL + (uint64_t)i;
@@ -1231,6 +1705,27 @@ bool chain_kernel_setup() {
                              4 * CHAIN_MAXD * CHAIN_MAXD * (int)sizeof(double)) == hipSuccess;
 }
 
+bool fchain_kernel_setup() {
+  const bool a = hipFuncSetAttribute((const void*)k_fchain<FCHAIN_NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     2 * FCHAIN_XS * (int)sizeof(double)) == hipSuccess;
+  const bool b = hipFuncSetAttribute((const void*)k_fchain<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     2 * 65 * 64 * (int)sizeof(double)) == hipSuccess;
+  return a && b;
+}
+
+void launch_fchain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups, const DevCTerm* terms, const double* x,
+                   double* y, double* part, int dmax, hipStream_t s) {
+  if (nblocks <= 0) return;
+  const int d16 = (std::min(dmax, FCHAIN_MAXD) + 15) / 16 * 16;
+  const int stride = (d16 + 1) * d16;
+  if (d16 <= 64)  // at most 16 output tiles: 4 per wavefront
+    hipLaunchKernelGGL(k_fchain<4>, dim3(nblocks), dim3(256), (size_t)2 * stride * sizeof(double), s, blocks, groups, terms,
+                       x, y, part, stride);
+  else
+    hipLaunchKernelGGL(k_fchain<FCHAIN_NT>, dim3(nblocks), dim3(256), (size_t)2 * stride * sizeof(double), s, blocks, groups,
+                       terms, x, y, part, stride);
+}
+
 
 int dot_parts(int64_t n) {
   int64_t g = (n + 255) / 256;
@@ -1280,8 +1775,51 @@ void launch_lincomb_vecs(const double* const* d_vecs, const double* d_coefs, int
 
 void launch_precond(const double* diag, const double* u, double shift, double cutoff, double* work, double* t,
                     int64_t n, int stage, double alpha, hipStream_t s, const float* m) {
-  hipLaunchKernelGGL(k_precond, dim3(grid_for(n)), dim3(256), 0, s, diag, u, shift, cutoff, work, t, n, stage, alpha, m);
+  hipLaunchKernelGGL(k_precond, dim3(grid_for(n)), dim3(256), 0, s, diag, u, shift, cutoff, work, t, n, stage, alpha, m,
+                     (const double*)nullptr, (const double*)nullptr);
 }
+
+void launch_precond_dev(const double* diag, const double* u, const double* dshift, double cutoff, double* work, double* t,
+                        int64_t n, int stage, const double* dalpha, hipStream_t s, const float* m) {
+  hipLaunchKernelGGL(k_precond, dim3(grid_for(n)), dim3(256), 0, s, diag, u, 0.0, cutoff, work, t, n, stage, 0.0, m, dshift,
+                     dalpha);
+}
+
+void launch_dav_store(const DevDav* st, double* const* vec, double* const* hv, const double* x, const double* y, int64_t n,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_dav_store, dim3(grid_for(n)), dim3(256), 0, s, st, vec, hv, x, y, n);
+}
+
+void launch_dav_dots(DevDav* st, int add, const double* const* a, const double* b, int64_t n, double* partial, int parts,
+                     int mode, hipStream_t s) {
+  hipLaunchKernelGGL(k_dav_dots_partial, dim3(parts, DAV_MAXV), dim3(256), 0, s, st, add, a, b, n, partial);
+  hipLaunchKernelGGL(k_dav_dots_reduce, dim3(DAV_MAXV), dim3(256), 0, s, st, add, partial, parts, mode);
+}
+
+void launch_dav_project_out(const DevDav* st, const double* const* vec, double* t, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_dav_project_out, dim3(grid_for(n)), dim3(256), 0, s, st, vec, t, n);
+}
+
+void launch_dav_normalize(DevDav* st, const double* t, double* out, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_dav_normalize, dim3(grid_for(n)), dim3(256), 0, s, st, t, out, n);
+}
+
+void launch_dav_eig(DevDav* st, hipStream_t s) { hipLaunchKernelGGL(k_dav_eig, dim3(1), dim3(256), 0, s, st); }
+
+void launch_dav_ritz(DevDav* st, const double* const* vec, const double* const* hv, double* u, double* r, int64_t n,
+                     double* partial, int parts, hipStream_t s) {
+  hipLaunchKernelGGL(k_dav_ritz, dim3(parts), dim3(256), 0, s, st, vec, hv, u, r, n, partial);
+  hipLaunchKernelGGL(k_dav_scalar, dim3(1), dim3(256), 0, s, st, partial, parts, 0);
+}
+
+void launch_dav_alpha(DevDav* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_dav_scalar, dim3(1), dim3(64), 0, s, st, nullptr, 0, 1);
+}
+
+void launch_dav_clock(DevDav* st, int which, hipStream_t s) {
+  hipLaunchKernelGGL(k_dav_clock, dim3(1), dim3(64), 0, s, st, which);
+}
+
 
 void launch_fill_hash(double* p, int64_t n, uint64_t seed, double scale, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_hash, dim3(grid_for(n)), dim3(256), 0, s, p, n, seed, scale);

@@ -269,17 +269,24 @@ int chm_heff_diag(chm_ctx* ctx, double* d, int on_device);
  * before Heff::SolveDAVIDSON; Heff.cpp:330-386 with Davidson(V, 32, 3, rtol, 1e-12)).
  * On return x holds the eigenvector in the program convention. */
 int chm_davidson(chm_ctx* ctx, double* x, int on_device, double rtol, double* eigenvalue, int* n_matvec);
+/* Counters of the Davidson solves of this context: out[0] deflations (restarts at 32
+ * vectors, Davidson.cpp:352-412), out[1] iteration graphs captured, out[2] graph launches
+ * (one per iteration after the first of a solve). Measurement; no reference counterpart. */
+int chm_davidson_stats(const chm_ctx* ctx, int64_t* out);
 
 /* Routing and work-item sizes of the Heff kernels (no reference counterpart): the
  * largest |kappa||kappa'| of the explicit sector matrix, terms per chained work item and
- * per input group, and the MFMA chained kernel for blocks up to 48 (1) or the LDS one
- * (0); a negative value (or 0 for the two term counts) keeps the current setting.
+ * per input group, the MFMA chained kernel for blocks up to 48 (1) or the LDS one (0),
+ * and fchain_min: chained blocks whose largest dimension is above it (and at most 96) run
+ * in the fused MFMA kernel k_fchain, the rest of those up to 64 in the LDS kernel; a
+ * negative value (or 0 for the two term counts) keeps the current setting.
  * chm_set_heff_knobs sets the defaults. chm_set_tuning installs the autotuner's table:
- * n rows of 8 values (largest block dimension lo, hi; V lo, hi; then the four knobs as
+ * n rows of 9 values (largest block dimension lo, hi; V lo, hi; then the five knobs as
  * above); chm_heff_prepare uses the first row whose ranges hold the site's largest
- * block dimension and V, or the defaults. chm_tuning_used: out[6] = row used (-1: none),
- * largest block dimension, then the four knobs of the last chm_heff_prepare. */
-int chm_set_heff_knobs(chm_ctx* ctx, int64_t explicit_max, int item_terms, int group_terms, int chain_mfma);
+ * block dimension and V, or the defaults. chm_tuning_used: out[7] = row used (-1: none),
+ * largest block dimension, then the five knobs of the last chm_heff_prepare. */
+int chm_set_heff_knobs(chm_ctx* ctx, int64_t explicit_max, int item_terms, int group_terms, int chain_mfma,
+                       int fchain_min);
 int chm_set_tuning(chm_ctx* ctx, int n, const int64_t* rows);
 int chm_tuning_used(const chm_ctx* ctx, int64_t* out);
 /* F_H: 2mnk summed over every dgemm_ the reference's makeHeff issues at this site;

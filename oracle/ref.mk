@@ -42,3 +42,12 @@ clean:
 	rm -rf $(OUT)
 
 .PHONY: all clean
+
+# The level-2 wiring of INTEGRATION.md compiled against the reference's headers and library
+# and against libchemps2_amd.so (tests/integration/level2_check.cpp); run on a GPU.
+AMD := $(HERE)../chemps2_amd
+$(OUT)/level2_check: $(HERE)../tests/integration/level2_check.cpp $(OUT)/libchemps2_ref.so $(AMD)/libchemps2_amd.so
+	$(CXX) $(CXXFLAGS) -I$(HERE)../include -o $@ $< -L$(OUT) -lchemps2_ref -L$(AMD) -lchemps2_amd \
+	  -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../../chemps2_amd' -Wl,-rpath,/opt/rocm/lib $(LDLIBS)
+
+level2: $(OUT)/level2_check
