@@ -409,8 +409,10 @@ constexpr int FCHAIN_NT = ((FCHAIN_MAXD / 16) * (FCHAIN_MAXD / 16) + 3) / 4;
 // grants k_fchain the LDS of its largest blocks (false: the device refuses; the blocks
 // then run as GEMM tiles)
 bool fchain_kernel_setup();
+// fp32: the FP32 variant (v_mfma_f32_16x16x4f32, FP32 operands and sums, added to y in
+// FP64), for the low-precision phase of the mixed form
 void launch_fchain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups, const DevCTerm* terms, const double* x,
-                   double* y, double* part, int dmax, hipStream_t s);
+                   double* y, double* part, int dmax, bool fp32, hipStream_t s);
 void launch_chain_mfma(const DevCBlock* blocks, const int* offs, const DevCGroup* groups, const DevCTerm* terms,
                        const double* x, double* y, double* part, hipStream_t s);
 void launch_ops(const DevOItem* items, int nitems, const DevGemmOp* ops, int P, hipStream_t s);

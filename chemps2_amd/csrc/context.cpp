@@ -945,7 +945,8 @@ void Ctx::apply_with(const double* xp, double* yp, double* bp, double* cp, doubl
   launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, cp, chain_P, false, stream);
   launch_chain(cblocks_l.p, ncblocks_l, cgroups.p, cterms.p, xp, yp, cp, chain_P_l, true, stream);
   for (int bk = 0; bk < 2; bk++)
-    launch_fchain(cblocks_f[bk].p, ncblocks_f[bk], cgroups.p, cterms.p, xp, yp, cp, fchain_dmax[bk], stream);
+    launch_fchain(cblocks_f[bk].p, ncblocks_f[bk], cgroups.p, cterms.p, xp, yp, cp, fchain_dmax[bk],
+                  mv_low && fchain_fp32, stream);
   launch_chain_reduce(creds.p, ncreds, cp, yp, stream);
   if (n_lower > 0) {  // lower states, in state order (HeffDiagrams1.cpp:65-86)
     launch_multi_dot(lower_ptrs.p, n_lower, xp, V, lp, ld, stream);
@@ -1603,7 +1604,7 @@ double Ctx::davidson_host(double* x, double rtol, int* n_matvec) {
   // makes lowers the residual norm; the first step that does not switches this solve
   // back to the plain diagonal correction of Davidson.cpp:322-350
   bool learned_ok = precond_mode == 1 && mlp.ready, last_learned = false;
-  double prev_rnorm = HUGE_VAL;
+  double prev_rnorm = HUGE_VAL, prev_low = -1.0;
   for (int iter = 0; iter < 100000; iter++) {
     // Davidson.cpp:242-320
     {
@@ -1632,7 +1633,11 @@ double Ctx::davidson_host(double* x, double rtol, int* n_matvec) {
       n_learned_fallbacks++;
     }
     prev_rnorm = rnorm;
-    if (mv_low && rnorm <= std::max(rtol, mixed_switch)) {
+    // the low-precision products have a residual floor near 1e-6 |theta| (FP32 chained
+    // products): the refinement also starts once the residual stops falling
+    const bool low_stalled = mv_low && prev_low > 0.0 && rnorm > 0.7 * prev_low;
+    prev_low = mv_low ? rnorm : -1.0;
+    if (mv_low && (rnorm <= std::max(rtol, mixed_switch) || low_stalled)) {
       mv_low = false;
       prev_rnorm = HUGE_VAL;
       hip_check(hipMemcpyAsync(t, u, n * sizeof(double), hipMemcpyDeviceToDevice, stream), "refinement start");
@@ -1934,7 +1939,7 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
   // makes lowers the residual norm; the first step that does not switches this solve
   // back to the plain diagonal correction of Davidson.cpp:322-350
   bool learned_ok = precond_mode == 1 && mlp.ready, last_learned = false;
-  double prev_rnorm = HUGE_VAL;
+  double prev_rnorm = HUGE_VAL, prev_low = -1.0;
   for (int iter = 0; iter < 100000; iter++) {
     const double rnorm = std::sqrt(dav_host->rn2);
     const double theta = dav_host->theta;
@@ -1944,7 +1949,11 @@ double Ctx::davidson(double* x, double rtol, int* n_matvec) {
       n_learned_fallbacks++;
     }
     prev_rnorm = rnorm;
-    if (mv_low && rnorm <= std::max(rtol, mixed_switch)) {  // refined products from the Ritz vector on
+    // the low-precision products have a residual floor near 1e-6 |theta| (FP32 chained
+    // products): the refinement also starts once the residual stops falling
+    const bool low_stalled = mv_low && prev_low > 0.0 && rnorm > 0.7 * prev_low;
+    prev_low = mv_low ? rnorm : -1.0;
+    if (mv_low && (rnorm <= std::max(rtol, mixed_switch) || low_stalled)) {  // refined products from the Ritz vector on
       mv_low = false;
       prev_rnorm = HUGE_VAL;
       last_learned = false;
@@ -2096,6 +2105,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   ctx->c.def_chain_mfma = ctx->c.use_chain_mfma;
   if (const char* e = std::getenv("CHM_FCHAIN")) ctx->c.use_fchain = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_FCHAIN_MIN")) ctx->c.fchain_min = std::atoi(e);
+  if (const char* e = std::getenv("CHM_FCHAIN_FP32")) ctx->c.fchain_fp32 = std::atoi(e) != 0;
   ctx->c.def_fchain_min = ctx->c.fchain_min;
   if (const char* e = std::getenv("CHM_DAVIDSON_DEVICE")) ctx->c.dav_device = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_DAVIDSON_GRAPH")) ctx->c.dav_graph = std::atoi(e) != 0;

@@ -159,9 +159,11 @@ struct Ctx : HostState {
   int64_t explicit_threshold = 0;
   DBuf<double> Mvals;
   DBuf<int> Mcol;
-  // Heff forms: 0 FP64; 1 mixed: the explicit matrix stored as hi/lo float pairs, the
-  // Davidson iterating with the hi floats until the residual falls below mixed_switch
-  // and refining with hi + lo (2^-48) products after that.
+  // Heff forms: 0 FP64; 1 mixed: the explicit matrix stored as hi/lo float pairs and the
+  // fused chained items in FP32 (k_fchain32); the Davidson iterates with these
+  // low-precision products (hi floats only) until the residual falls below mixed_switch
+  // or stops falling, then restarts from the Ritz vector with FP64 products (hi + lo,
+  // 2^-48) to the tolerance.
   int heff_form = 0;
   bool M_split = false;              // storage of Mvals at the prepared site
   int64_t M_nnz = 0;
@@ -245,6 +247,9 @@ struct Ctx : HostState {
   // off; blocks whose largest dimension is at most fchain_min (CHM_FCHAIN_MIN) stay on
   // k_chain.
   bool use_fchain = true, fchain_ok = false;
+  // the mixed form's low-precision products run k_fchain's items in FP32 on the matrix
+  // cores (k_fchain32); CHM_FCHAIN_FP32=0 keeps them FP64
+  bool fchain_fp32 = true;
   int fchain_min = 32, def_fchain_min = 32;
   DBuf<DevCBlock> cblocks_f[2];
   int ncblocks_f[2] = {0, 0};

@@ -949,6 +949,203 @@ __global__ __launch_bounds__(256) void k_fchain(const DevCBlock* __restrict__ bl
 }
 #undef FMFMA
 
+// The same chained products with FP32 operands and FP32 accumulation on
+// v_mfma_f32_16x16x4f32, for the low-precision phase of the mixed form: x and the
+// intermediate are staged in LDS as floats, operator blocks are rounded to float as they
+// are read, and the item's sum is added to y in FP64. The f32 instruction returns
+// D(4 lg + r, lr) in register r of lane (lr, lg) (rows in blocks of four, unlike the f64
+// one's interleaved rows). Relative error of one product about 2^-24 sqrt(K) per term.
+typedef float chm_f4 __attribute__((ext_vector_type(4)));
+#define FMFMA32(a, b, c) __builtin_amdgcn_mfma_f32_16x16x4f32((a), (b), (c), 0, 0, 0)
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_fchain32(const DevCBlock* __restrict__ blocks, const DevCGroup* __restrict__ groups,
+                                               const DevCTerm* __restrict__ terms, const double* __restrict__ x,
+                                               double* __restrict__ y, double* __restrict__ part, int lds_stride) {
+  extern __shared__ float ldsf[];
+  float* Xs = ldsf;
+  float* Ws = ldsf + lds_stride;
+  const DevCBlock b = blocks[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, lr = lane & 15, lg = lane >> 4;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int dL = b.dL, dR = b.dR;
+  const int nL = (dL + 15) >> 4, nR = (dR + 15) >> 4, nt = nL * nR;
+  chm_f4 acc[NT];
+#pragma unroll
+  for (int s = 0; s < NT; s++) acc[s] = chm_f4{0.f, 0.f, 0.f, 0.f};
+  for (int g = b.g0; g < b.g0 + b.ng; g++) {
+    const DevCGroup G = groups[g];
+    const int dLi = G.dLi, dRi = G.dRi;
+    const int nLi = (dLi + 15) >> 4, nRi = (dRi + 15) >> 4;
+    const int ldx = 16 * nLi + 1;
+    __syncthreads();  // the previous group's readers of Xs and Ws are done
+    {
+      const double* __restrict__ X = x + G.xoff;
+      const int rows = 16 * nLi, cnt = rows * 16 * nRi;
+      for (int e = tid; e < cnt; e += 256) {
+        const int i = e % rows, j = e / rows;
+        Xs[i + ldx * j] = (i < dLi && j < dRi) ? (float)X[i + (int64_t)dLi * j] : 0.f;
+      }
+    }
+    __syncthreads();
+    for (int t = G.t0; t < G.t0 + G.nt; t++) {
+      const DevCTerm T = terms[t];
+      const float al = (float)T.alpha;
+      // op(A)(i, k) = A[i ai + k ak] (dL x dLi), op(B)(k, j) = B[k bk + j bj] (dRi x dR)
+      const int64_t ai = T.ta ? T.lda : 1, ak = T.ta ? 1 : T.lda;
+      const int64_t bk = T.tb ? T.ldb : 1, bj = T.tb ? 1 : T.ldb;
+      const double* __restrict__ A = T.A ? T.A : x;  // absent operands are never read
+      const double* __restrict__ B = T.B ? T.B : x;
+      const int mode = T.mode;
+      if (mode == 3) {  // W = al op(A) X (16 nL x 16 nRi), then Y += W op(B)
+        const int ldw = 16 * nL + 1;
+        for (int q = w; q < nL * nRi; q += 4) {
+          const int ti = q % nL, tj = q / nL;
+          const int i = 16 * ti + lr;
+          chm_f4 d = chm_f4{0.f, 0.f, 0.f, 0.f};
+          for (int k0 = 0; k0 < 16 * nLi; k0 += 16) {
+            float av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int k = k0 + 4 * u + lg;
+              av[u] = (float)gld(A, i < dL && k < dLi, i * ai + k * ak);
+              bv[u] = Xs[k + ldx * (16 * tj + lr)];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) d = FMFMA32(av[u], bv[u], d);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; r++) Ws[(16 * ti + 4 * lg + r) + ldw * (16 * tj + lr)] = al * d[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NT; s++) {
+          const int q = w + 4 * s;
+          if (q >= nt) break;
+          const int ti = q % nL, tj = q / nL;
+          const int j = 16 * tj + lr;
+          for (int k0 = 0; k0 < 16 * nRi; k0 += 16) {
+            float av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int k = k0 + 4 * u + lg;
+              av[u] = Ws[(16 * ti + lr) + ldw * k];
+              bv[u] = (float)gld(B, k < dRi && j < dR, k * bk + j * bj);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[s] = FMFMA32(av[u], bv[u], acc[s]);
+          }
+        }
+        __syncthreads();  // Ws is rewritten by the next term
+      } else if (mode == 4) {  // W = X op(B) (16 nLi x 16 nR), then Y += al op(A) W
+        const int ldw = 16 * nLi + 1;
+        for (int q = w; q < nLi * nR; q += 4) {
+          const int ti = q % nLi, tj = q / nLi;
+          const int j = 16 * tj + lr;
+          chm_f4 d = chm_f4{0.f, 0.f, 0.f, 0.f};
+          for (int k0 = 0; k0 < 16 * nRi; k0 += 16) {
+            float av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int k = k0 + 4 * u + lg;
+              av[u] = Xs[(16 * ti + lr) + ldx * k];
+              bv[u] = (float)gld(B, k < dRi && j < dR, k * bk + j * bj);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) d = FMFMA32(av[u], bv[u], d);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; r++) Ws[(16 * ti + 4 * lg + r) + ldw * (16 * tj + lr)] = d[r];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NT; s++) {
+          const int q = w + 4 * s;
+          if (q >= nt) break;
+          const int ti = q % nL, tj = q / nL;
+          const int i = 16 * ti + lr;
+          for (int k0 = 0; k0 < 16 * nLi; k0 += 16) {
+            float av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int k = k0 + 4 * u + lg;
+              av[u] = al * (float)gld(A, i < dL && k < dLi, i * ai + k * ak);
+              bv[u] = Ws[k + ldw * (16 * tj + lr)];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[s] = FMFMA32(av[u], bv[u], acc[s]);
+          }
+        }
+        __syncthreads();
+      } else if (mode == 1) {  // Y += al op(A) X
+#pragma unroll
+        for (int s = 0; s < NT; s++) {
+          const int q = w + 4 * s;
+          if (q >= nt) break;
+          const int ti = q % nL, tj = q / nL;
+          const int i = 16 * ti + lr;
+          for (int k0 = 0; k0 < 16 * nLi; k0 += 16) {
+            float av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int k = k0 + 4 * u + lg;
+              av[u] = al * (float)gld(A, i < dL && k < dLi, i * ai + k * ak);
+              bv[u] = Xs[k + ldx * (16 * tj + lr)];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[s] = FMFMA32(av[u], bv[u], acc[s]);
+          }
+        }
+      } else if (mode == 2) {  // Y += al X op(B)
+#pragma unroll
+        for (int s = 0; s < NT; s++) {
+          const int q = w + 4 * s;
+          if (q >= nt) break;
+          const int ti = q % nL, tj = q / nL;
+          const int j = 16 * tj + lr;
+          for (int k0 = 0; k0 < 16 * nRi; k0 += 16) {
+            float av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int k = k0 + 4 * u + lg;
+              av[u] = al * Xs[(16 * ti + lr) + ldx * k];
+              bv[u] = (float)gld(B, k < dRi && j < dR, k * bk + j * bj);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[s] = FMFMA32(av[u], bv[u], acc[s]);
+          }
+        }
+      } else {  // Y += al X (same dimensions)
+#pragma unroll
+        for (int s = 0; s < NT; s++) {
+          const int q = w + 4 * s;
+          if (q >= nt) break;
+          const int ti = q % nL, tj = q / nL;
+#pragma unroll
+          for (int r = 0; r < 4; r++) acc[s][r] += al * Xs[(16 * ti + 4 * lg + r) + ldx * (16 * tj + lr)];
+        }
+      }
+    }
+  }
+  double* Y = b.direct ? y + b.out : part + b.out;
+#pragma unroll
+  for (int s = 0; s < NT; s++) {
+    const int q = w + 4 * s;
+    if (q >= nt) break;
+    const int ti = q % nL, tj = q / nL;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int l = 16 * ti + 4 * lg + r, c = 16 * tj + lr;
+      if (l < dL && c < dR) {
+        if (b.direct) Y[l + (int64_t)dL * c] += (double)acc[s][r];
+        else Y[l + (int64_t)dL * c] = (double)acc[s][r];
+      }
+    }
+  }
+}
+#undef FMFMA32
+
+
 constexpr int CHAIN_PER_S = (18 * 18 + 255) / 256;                                  // 2 x 2 tiles of a 36 x 36 block
 constexpr int CHAIN_PER_L = ((CHAIN_MAXD / 2) * (CHAIN_MAXD / 2) + 255) / 256;     // of a 64 x 64 block
 
@@ -1710,14 +1907,25 @@ bool fchain_kernel_setup() {
                                      2 * FCHAIN_XS * (int)sizeof(double)) == hipSuccess;
   const bool b = hipFuncSetAttribute((const void*)k_fchain<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      2 * 65 * 64 * (int)sizeof(double)) == hipSuccess;
-  return a && b;
+  const bool c = hipFuncSetAttribute((const void*)k_fchain32<FCHAIN_NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     2 * FCHAIN_XS * (int)sizeof(float)) == hipSuccess;
+  return a && b && c;
 }
 
 void launch_fchain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups, const DevCTerm* terms, const double* x,
-                   double* y, double* part, int dmax, hipStream_t s) {
+                   double* y, double* part, int dmax, bool fp32, hipStream_t s) {
   if (nblocks <= 0) return;
   const int d16 = (std::min(dmax, FCHAIN_MAXD) + 15) / 16 * 16;
   const int stride = (d16 + 1) * d16;
+  if (fp32) {
+    if (d16 <= 64)
+      hipLaunchKernelGGL(k_fchain32<4>, dim3(nblocks), dim3(256), (size_t)2 * stride * sizeof(float), s, blocks, groups,
+                         terms, x, y, part, stride);
+    else
+      hipLaunchKernelGGL(k_fchain32<FCHAIN_NT>, dim3(nblocks), dim3(256), (size_t)2 * stride * sizeof(float), s, blocks,
+                         groups, terms, x, y, part, stride);
+    return;
+  }
   if (d16 <= 64)  // at most 16 output tiles: 4 per wavefront
     hipLaunchKernelGGL(k_fchain<4>, dim3(nblocks), dim3(256), (size_t)2 * stride * sizeof(double), s, blocks, groups, terms,
                        x, y, part, stride);

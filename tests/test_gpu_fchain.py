@@ -77,3 +77,30 @@ def test_fchain_cylinder_D1000():
         assert e <= 1e-12
     finally:
         c.close()
+
+
+@pytest.mark.parametrize("site", [5, 9])
+def test_fchain_fp32_mixed_form(site):
+    """The mixed form's low-precision product (k_fchain32 on v_mfma_f32_16x16x4f32 for the
+    fused items, hi floats for the explicit part) is within the FP32 error bound of the
+    oracle, and the refined product (FP64 kernels, hi + lo) equals it to 1e-12."""
+    irr, T, V = random_abinitio(12, 9, group=7)
+    o = OracleDMRG(12, 7, irr, 12, 0, 0, T, V)
+    o.init(150, 2)
+    o.sweep_left_to(site, 150, rtol=1e-4)
+    n = o.site(site)
+    x = np.random.default_rng(site).normal(size=n)
+    y0, _ = o.apply(x)
+    print(f"case FP32 fused chain kernel, mixed form, L=12 D=150 site {site}, V={n}", flush=True)
+    for env in ({"CHM_FCHAIN_MIN": "0", "CHM_EXPLICIT_MAX": "0"}, {"CHM_FCHAIN_MIN": "0"}):
+        c = _ctx(env)
+        try:
+            c.set_heff_form(chemps2_amd.FORM_MIXED)
+            assert load_site(c, o, site) == n
+            e_low = rel(c.apply(x, f32=True), y0)
+            e_ref = rel(c.apply(x), y0)
+            print(f"  {env}: low precision {e_low:.2e}, refined {e_ref:.2e}", flush=True)
+            assert 1e-12 < e_low <= 1e-5  # FP32 products were used, within their bound
+            assert e_ref <= 1e-12
+        finally:
+            c.close()
