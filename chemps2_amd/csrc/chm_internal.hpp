@@ -264,7 +264,7 @@ struct DevCBlock {  // one work item: input groups [g0, g0 + ng) of one output b
   int dL, dR;
   int g0, ng;
   int direct;
-  int pad;
+  int tr;             // k_fchain: the item computes the transposed block (terms already exchanged)
 };
 struct DevCGroup {
   int64_t xoff;
@@ -401,18 +401,21 @@ void launch_chain_reduce(const DevBRed* reds, int nreds, const double* part, dou
 // offs is a host array. Larger blocks run in k_chain.
 constexpr int CHAIN_MFMA_NT = 3;
 // k_fchain: chained products of blocks up to FCHAIN_MAXD on the matrix cores, one
-// workgroup per work item (x and one intermediate in LDS); FCHAIN_NT output tiles of
-// 16 x 16 per wavefront
-constexpr int FCHAIN_MAXD = 96;
-constexpr int FCHAIN_NT = ((FCHAIN_MAXD / 16) * (FCHAIN_MAXD / 16) + 3) / 4;
-// dmax: the largest block dimension of the launch (sets the LDS of a workgroup)
-// grants k_fchain the LDS of its largest blocks (false: the device refuses; the blocks
-// then run as GEMM tiles)
+// workgroup of four wavefronts per work item, each wavefront owning 16-row strips of the
+// output block (as computed: transposed for items with tr set). Items are bucketed by
+// nt = max(FCHAIN_NT0, ceil(largest dimension / 16)) (a bucket's LDS holds op(A), op(B)
+// and x padded to 16 nt) and nj = ceil(columns of the block as computed / 16) <= nt.
+constexpr int FCHAIN_NT0 = 2;
+constexpr int FCHAIN_MAXD = 80;
+constexpr int fchain_bucket(int nt, int nj) { return (nt - 2) * (nt + 1) / 2 + nj - 1; }
+constexpr int FCHAIN_NB = 14;  // (nt, nj) = (2, 1) .. (5, 5)
+// grants k_fchain the LDS of its buckets (false: the device refuses; the blocks then run
+// in the other chained kernels and as GEMM tiles)
 bool fchain_kernel_setup();
-// fp32: the FP32 variant (v_mfma_f32_16x16x4f32, FP32 operands and sums, added to y in
-// FP64), for the low-precision phase of the mixed form
+// bucket: fchain_bucket(nt, nj); fp32: the FP32 variant (v_mfma_f32_16x16x4f32, FP32
+// operands and sums, added to y in FP64), for the low-precision phase of the mixed form
 void launch_fchain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups, const DevCTerm* terms, const double* x,
-                   double* y, double* part, int dmax, bool fp32, hipStream_t s);
+                   double* y, double* part, int bucket, bool fp32, hipStream_t s);
 void launch_chain_mfma(const DevCBlock* blocks, const int* offs, const DevCGroup* groups, const DevCTerm* terms,
                        const double* x, double* y, double* part, hipStream_t s);
 void launch_ops(const DevOItem* items, int nitems, const DevGemmOp* ops, int P, hipStream_t s);

@@ -224,11 +224,11 @@ void Ctx::apply_tuning() {
   for (size_t r = 0; r < tune.size(); r++) {
     const TuneRow& t = tune[r];
     if (site_dmax < t.dmax_lo || site_dmax > t.dmax_hi || V < t.v_lo || V > t.v_hi) continue;
-    if (t.explicit_max >= 0) explicit_max = t.explicit_max;
-    if (t.item_terms > 0) chain_item_terms = (int)t.item_terms;
-    if (t.group_terms > 0) chain_group_terms = (int)t.group_terms;
-    if (t.chain_mfma >= 0) use_chain_mfma = t.chain_mfma != 0;
-    if (t.fchain_min >= 0) fchain_min = (int)t.fchain_min;
+    if (t.explicit_max >= 0 && !env_pinned[0]) explicit_max = t.explicit_max;
+    if (t.item_terms > 0 && !env_pinned[1]) chain_item_terms = (int)t.item_terms;
+    if (t.group_terms > 0 && !env_pinned[2]) chain_group_terms = (int)t.group_terms;
+    if (t.chain_mfma >= 0 && !env_pinned[3]) use_chain_mfma = t.chain_mfma != 0;
+    if (t.fchain_min >= 0 && !env_pinned[4]) fchain_min = (int)t.fchain_min;
     tune_row = (int)r;
     break;
   }
@@ -493,13 +493,13 @@ void Ctx::prepare(int st) {
     // are then concatenated in order (group, term and partial offsets shifted), so the
     // lists do not depend on the thread count
     struct Part {
-      std::vector<DevCBlock> cb, cbl, cbm[CHAIN_MFMA_NT], cbf[2];  // cbf: k_fchain, blocks <= 64 / <= FCHAIN_MAXD
+      std::vector<DevCBlock> cb, cbl, cbm[CHAIN_MFMA_NT], cbf[FCHAIN_NB];  // cbf: k_fchain, by tile shape
       std::vector<DevCGroup> cg;
       std::vector<DevCTerm> ct;
       std::vector<DevBRed> cr;
       std::vector<const Term*> rest;
       int64_t cpart_n = 0;
-      int Pmax = 1, PmaxL = 1, Fmax[2] = {1, 1};
+      int Pmax = 1, PmaxL = 1;
     };
     std::vector<size_t> bstart;  // first chain index of every output block
     for (size_t q = 0; q < chain.size(); q++)
@@ -509,9 +509,10 @@ void Ctx::prepare(int st) {
     const int64_t nparts = std::max<int64_t>(1, std::min<int64_t>(256, (nblk + 15) / 16));
     std::vector<Part> parts(nparts);
     const int maxd = chain_large_ok ? CHAIN_MAXD : 36;
-    // blocks up to FCHAIN_MAXD can take the fused MFMA kernel: those above maxd always,
+    // blocks up to fchain_maxd (at most FCHAIN_MAXD) can take the fused MFMA kernel: those above maxd always,
     // the others when their largest dimension is above fchain_min
-    const int lim = use_fchain ? std::max(maxd, FCHAIN_MAXD) : maxd;
+    const int fmaxd = std::min(fchain_maxd, FCHAIN_MAXD);
+    const int lim = use_fchain ? std::max(maxd, fmaxd) : maxd;
     parallel_for(nparts, nthr, [&](int64_t pi, int) {
       Part& P = parts[pi];
       std::vector<DevCBlock>&cb = P.cb, &cbl = P.cbl;
@@ -523,14 +524,16 @@ void Ctx::prepare(int st) {
       std::vector<const Term*>& rest = P.rest;
       int64_t& cpart_n = P.cpart_n;
       int &Pmax = P.Pmax, &PmaxL = P.PmaxL;
-      int* Fmax = P.Fmax;
       const size_t qb = bstart[nblk * pi / nparts], qe = bstart[nblk * (pi + 1) / nparts];
       for (size_t q0 = qb; q0 < qe;) {
         const int k = chain[q0]->out;
         size_t q1 = q0;
         bool small = use_chain_kernel && dLof(k) <= lim && dRof(k) <= lim;
+        int dmax = std::max(dLof(k), dRof(k));
         while (q1 < qe && chain[q1]->out == k) {
-          if (dLof(chain[q1]->in) > lim || dRof(chain[q1]->in) > lim) small = false;
+          const int in = chain[q1]->in;
+          if (dLof(in) > lim || dRof(in) > lim) small = false;
+          dmax = std::max({dmax, dLof(in), dRof(in)});
           q1++;
         }
         if (!small) {
@@ -540,67 +543,79 @@ void Ctx::prepare(int st) {
         }
         DevCBlock B{};
         B.dL = dLof(k); B.dR = dRof(k);
-        const int g_first = (int)cg.size();
+        const bool fused = use_fchain && dmax <= fmaxd && (dmax > maxd || dmax > fchain_min);
         int Pk = B.dL * B.dR;
-        int dmax = std::max(B.dL, B.dR);
-        for (size_t a = q0; a < q1;) {
-          const int in = chain[a]->in;
-          size_t e = a;
-          while (e < q1 && chain[e]->in == in) e++;
-          DevCGroup G{};
-          G.xoff = S.k2i[in]; G.dLi = dLof(in); G.dRi = dRof(in); G.t0 = (int)ct.size();
-          Pk = std::max({Pk, G.dLi * G.dRi, B.dL * G.dRi, G.dLi * B.dR, B.dL * G.dLi, G.dRi * B.dR});
-          dmax = std::max({dmax, G.dLi, G.dRi});
-          for (size_t q = a; q < e; q++) {
-            const Term& t = *chain[q];
-            const int dL = B.dL, dR = B.dR, dLi = G.dLi, dRi = G.dRi;
-            DevCTerm T{};
-            T.alpha = t.alpha;
-            if (t.tl != 'I') { T.A = resolve(plan->ref(t, t.Lb)); T.lda = (t.tl == 'N') ? dL : dLi; T.ta = (t.tl == 'T'); }
-            if (t.tr != 'I') { T.B = resolve(plan->ref(t, t.Rb)); T.ldb = (t.tr == 'N') ? dRi : dR; T.tb = (t.tr == 'T'); }
-            if (t.tl == 'I' && t.tr == 'I') T.mode = 0;
-            else if (t.tr == 'I') T.mode = 1;
-            else if (t.tl == 'I') T.mode = 2;
-            else {
-              const double c3 = (double)dL * dRi * dLi + (double)dL * dR * dRi, c4 = (double)dLi * dR * dRi + (double)dL * dR * dLi;
-              T.mode = c3 <= c4 ? 3 : 4;
-            }
-            ct.push_back(T);
-          }
-          // groups of at most CHAIN_GROUP_TERMS terms (x is staged once per group)
-          const int t_end = (int)ct.size();
-          for (int t0 = G.t0; t0 < t_end; t0 += chain_group_terms) {
-            DevCGroup Gs = G;
-            Gs.t0 = t0;
-            Gs.nt = std::min(chain_group_terms, t_end - t0);
-            cg.push_back(Gs);
-          }
-          a = e;
-        }
-        // work items of about CHAIN_ITEM_TERMS terms; a block with several items sums
-        // their partials in k_chain_reduce
         std::vector<DevCBlock> items;
-        for (int g = g_first; g < (int)cg.size();) {
-          DevCBlock it = B;
-          it.g0 = g;
-          int nt = 0;
-          while (g < (int)cg.size() && (nt == 0 || nt + cg[g].nt <= chain_item_terms)) nt += cg[g++].nt;
-          it.ng = g - it.g0;
-          items.push_back(it);
+        // k_fchain computes the terms of the association op(A) (x op(B)) as the transposed
+        // block: those form items of their own (class 1), with operators exchanged
+        for (int cls = 0; cls < (fused ? 2 : 1); cls++) {
+          const int g_first = (int)cg.size();
+          for (size_t a = q0; a < q1;) {
+            const int in = chain[a]->in;
+            size_t e = a;
+            while (e < q1 && chain[e]->in == in) e++;
+            DevCGroup G{};
+            G.xoff = S.k2i[in]; G.dLi = dLof(in); G.dRi = dRof(in); G.t0 = (int)ct.size();
+            Pk = std::max({Pk, G.dLi * G.dRi, B.dL * G.dRi, G.dLi * B.dR, B.dL * G.dLi, G.dRi * B.dR});
+            for (size_t q = a; q < e; q++) {
+              const Term& t = *chain[q];
+              const int dL = B.dL, dR = B.dR, dLi = G.dLi, dRi = G.dRi;
+              DevCTerm T{};
+              T.alpha = t.alpha;
+              if (t.tl != 'I') { T.A = resolve(plan->ref(t, t.Lb)); T.lda = (t.tl == 'N') ? dL : dLi; T.ta = (t.tl == 'T'); }
+              if (t.tr != 'I') { T.B = resolve(plan->ref(t, t.Rb)); T.ldb = (t.tr == 'N') ? dRi : dR; T.tb = (t.tr == 'T'); }
+              if (t.tl == 'I' && t.tr == 'I') T.mode = 0;
+              else if (t.tr == 'I') T.mode = 1;
+              else if (t.tl == 'I') T.mode = 2;
+              else {
+                const double c3 = (double)dL * dRi * dLi + (double)dL * dR * dRi, c4 = (double)dLi * dR * dRi + (double)dL * dR * dLi;
+                T.mode = c3 <= c4 ? 3 : 4;
+              }
+              if (fused && (T.mode == 4) != (cls == 1)) continue;
+              if (cls == 1) {  // y^T += a op(B)^T x^T op(A)^T
+                std::swap(T.A, T.B);
+                std::swap(T.lda, T.ldb);
+                const int ta = T.ta;
+                T.ta = !T.tb;
+                T.tb = !ta;
+                T.mode = 3;
+              }
+              ct.push_back(T);
+            }
+            // groups of at most CHAIN_GROUP_TERMS terms (x is staged once per group)
+            const int t_end = (int)ct.size();
+            for (int t0 = G.t0; t0 < t_end; t0 += chain_group_terms) {
+              DevCGroup Gs = G;
+              Gs.t0 = t0;
+              Gs.nt = std::min(chain_group_terms, t_end - t0);
+              cg.push_back(Gs);
+            }
+            a = e;
+          }
+          // work items of about CHAIN_ITEM_TERMS terms; a block with several items sums
+          // their partials in k_chain_reduce
+          for (int g = g_first; g < (int)cg.size();) {
+            DevCBlock it = B;
+            it.g0 = g;
+            it.tr = cls;
+            int nt = 0;
+            while (g < (int)cg.size() && (nt == 0 || nt + cg[g].nt <= chain_item_terms)) nt += cg[g++].nt;
+            it.ng = g - it.g0;
+            items.push_back(it);
+          }
         }
         const int64_t nel = (int64_t)B.dL * B.dR;
         if (items.size() == 1) {
           items[0].direct = 1;
           items[0].out = S.k2i[k];
-        } else {
+        } else if (items.size() > 1) {
           cr.push_back({S.k2i[k], cpart_n, (int)nel, (int)items.size()});
           for (auto& it : items) { it.direct = 0; it.out = cpart_n; cpart_n += nel; }
         }
         const int nt = (dmax + 15) / 16;
-        if (use_fchain && (dmax > maxd || dmax > fchain_min)) {
-          const int bk = dmax <= 64 ? 0 : 1;
-          for (auto& it : items) cbf[bk].push_back(it);
-          Fmax[bk] = std::max(Fmax[bk], dmax);
+        if (fused) {
+          const int ntf = std::max(nt, FCHAIN_NT0);
+          for (auto& it : items) cbf[fchain_bucket(ntf, ((it.tr ? B.dL : B.dR) + 15) / 16)].push_back(it);
         } else if (use_chain_mfma && nt <= CHAIN_MFMA_NT) {
           for (auto& it : items) cbm[nt - 1].push_back(it);
         } else {
@@ -617,17 +632,15 @@ void Ctx::prepare(int st) {
     // offsets of every part in the concatenated lists (cbm: all parts of tile count 1,
     // then of 2, ...), then a parallel copy into page-locked staging with the indices
     // shifted, and asynchronous uploads (complete at the synchronisation ending prepare)
-    enum { L_CG, L_CT, L_CR, L_CB, L_CBL, L_REST, L_F0, L_F1, L_M0 };
+    enum { L_CG, L_CT, L_CR, L_CB, L_CBL, L_REST, L_F0, L_M0 = L_F0 + FCHAIN_NB };
     const int NL = L_M0 + CHAIN_MFMA_NT;
     std::vector<int64_t> off((size_t)NL * (nparts + 1), 0), poff(nparts + 1, 0);
     auto O = [&](int l, int64_t pi) -> int64_t& { return off[(size_t)l * (nparts + 1) + pi]; };
-    int Pmax = 1, PmaxL = 1, Fmx[2] = {1, 1};
+    int Pmax = 1, PmaxL = 1;
     for (int64_t pi = 0; pi < nparts; pi++) {
       const Part& P = parts[pi];
-      const size_t sz[NL - CHAIN_MFMA_NT] = {P.cg.size(),  P.ct.size(),     P.cr.size(),    P.cb.size(), P.cbl.size(),
-                                             P.rest.size(), P.cbf[0].size(), P.cbf[1].size()};
-      Fmx[0] = std::max(Fmx[0], P.Fmax[0]);
-      Fmx[1] = std::max(Fmx[1], P.Fmax[1]);
+      size_t sz[L_M0] = {P.cg.size(), P.ct.size(), P.cr.size(), P.cb.size(), P.cbl.size(), P.rest.size()};
+      for (int bk = 0; bk < FCHAIN_NB; bk++) sz[L_F0 + bk] = P.cbf[bk].size();
       for (int l = 0; l < L_M0; l++) O(l, pi + 1) = O(l, pi) + (int64_t)sz[l];
       for (int n = 0; n < CHAIN_MFMA_NT; n++) O(L_M0 + n, pi + 1) = O(L_M0 + n, pi) + (int64_t)P.cbm[n].size();
       poff[pi + 1] = poff[pi] + P.cpart_n;
@@ -636,20 +649,22 @@ void Ctx::prepare(int st) {
     }
     const int64_t n_cg = O(L_CG, nparts), n_ct = O(L_CT, nparts), n_cr = O(L_CR, nparts), n_cb = O(L_CB, nparts),
                   n_cbl = O(L_CBL, nparts), n_rest = O(L_REST, nparts), cpart_n = poff[nparts];
-    const int64_t n_f[2] = {O(L_F0, nparts), O(L_F1, nparts)};
+    int64_t n_f[FCHAIN_NB], n_fall = 0;
+    for (int bk = 0; bk < FCHAIN_NB; bk++) { n_f[bk] = O(L_F0 + bk, nparts); n_fall += n_f[bk]; }
     int64_t n_m[CHAIN_MFMA_NT], n_mall = 0;
     for (int n = 0; n < CHAIN_MFMA_NT; n++) { n_m[n] = O(L_M0 + n, nparts); n_mall += n_m[n]; }
     if (n_ct > INT32_MAX || n_cg > INT32_MAX) throw Error{ERR_SIZE, "too many chained terms"};
     cstage.reset();
     cstage.reserve(8 * 256 + n_cg * sizeof(DevCGroup) + n_ct * sizeof(DevCTerm) + n_cr * sizeof(DevBRed) +
-                   (n_cb + n_cbl + n_mall + n_f[0] + n_f[1]) * sizeof(DevCBlock));
+                   (n_cb + n_cbl + n_mall + n_fall) * sizeof(DevCBlock));
     DevCGroup* hcg = cstage.take<DevCGroup>(n_cg);
     DevCTerm* hct = cstage.take<DevCTerm>(n_ct);
     DevBRed* hcr = cstage.take<DevBRed>(n_cr);
     DevCBlock* hcb = cstage.take<DevCBlock>(n_cb);
     DevCBlock* hcbl = cstage.take<DevCBlock>(n_cbl);
     DevCBlock* hcbm = cstage.take<DevCBlock>(n_mall);
-    DevCBlock* hcbf[2] = {cstage.take<DevCBlock>(n_f[0]), cstage.take<DevCBlock>(n_f[1])};
+    DevCBlock* hcbf[FCHAIN_NB];
+    for (int bk = 0; bk < FCHAIN_NB; bk++) hcbf[bk] = cstage.take<DevCBlock>(n_f[bk]);
     std::vector<const Term*> rest(n_rest);
     parallel_for(nparts, nthr, [&](int64_t pi, int) {
       const Part& P = parts[pi];
@@ -669,8 +684,7 @@ void Ctx::prepare(int st) {
       };
       shift(hcb + O(L_CB, pi), P.cb);
       shift(hcbl + O(L_CBL, pi), P.cbl);
-      shift(hcbf[0] + O(L_F0, pi), P.cbf[0]);
-      shift(hcbf[1] + O(L_F1, pi), P.cbf[1]);
+      for (int bk = 0; bk < FCHAIN_NB; bk++) shift(hcbf[bk] + O(L_F0 + bk, pi), P.cbf[bk]);
       int64_t mbase = 0;
       for (int n = 0; n < CHAIN_MFMA_NT; n++) {
         shift(hcbm + mbase + O(L_M0 + n, pi), P.cbm[n]);
@@ -692,10 +706,9 @@ void Ctx::prepare(int st) {
     chain_P = std::min((Pmax + 31) / 32 * 32, CHAIN_SMALLP);
     up(cblocks_l, hcbl, n_cbl);
     ncblocks_l = (int)n_cbl;
-    for (int bk = 0; bk < 2; bk++) {
+    for (int bk = 0; bk < FCHAIN_NB; bk++) {
       up(cblocks_f[bk], hcbf[bk], n_f[bk]);
       ncblocks_f[bk] = (int)n_f[bk];
-      fchain_dmax[bk] = Fmx[bk];
     }
     up(creds, hcr, n_cr);
     ncreds = (int)n_cr;
@@ -944,9 +957,8 @@ void Ctx::apply_with(const double* xp, double* yp, double* bp, double* cp, doubl
   launch_chain_mfma(cblocks_m.p, cblocks_m_offs, cgroups.p, cterms.p, xp, yp, cp, stream);
   launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, cp, chain_P, false, stream);
   launch_chain(cblocks_l.p, ncblocks_l, cgroups.p, cterms.p, xp, yp, cp, chain_P_l, true, stream);
-  for (int bk = 0; bk < 2; bk++)
-    launch_fchain(cblocks_f[bk].p, ncblocks_f[bk], cgroups.p, cterms.p, xp, yp, cp, fchain_dmax[bk],
-                  mv_low && fchain_fp32, stream);
+  for (int bk = 0; bk < FCHAIN_NB; bk++)
+    launch_fchain(cblocks_f[bk].p, ncblocks_f[bk], cgroups.p, cterms.p, xp, yp, cp, bk, mv_low && fchain_fp32, stream);
   launch_chain_reduce(creds.p, ncreds, cp, yp, stream);
   if (n_lower > 0) {  // lower states, in state order (HeffDiagrams1.cpp:65-86)
     launch_multi_dot(lower_ptrs.p, n_lower, xp, V, lp, ld, stream);
@@ -2099,6 +2111,11 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_SVD_GESVDJ")) ctx->c.svd_large_gesvdj = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_TILE_W_CAP")) ctx->c.tile_w_cap = std::max(1LL, std::atoll(e));
   if (const char* e = std::getenv("CHM_OPS_KERNEL")) ctx->c.use_ops_kernel = std::atoi(e) != 0;
+  ctx->c.env_pinned[0] = std::getenv("CHM_EXPLICIT_MAX") != nullptr;
+  ctx->c.env_pinned[1] = std::getenv("CHM_CHAIN_ITEM_TERMS") != nullptr;
+  ctx->c.env_pinned[2] = std::getenv("CHM_CHAIN_GROUP_TERMS") != nullptr;
+  ctx->c.env_pinned[3] = std::getenv("CHM_CHAIN_MFMA") != nullptr;
+  ctx->c.env_pinned[4] = std::getenv("CHM_FCHAIN_MIN") != nullptr;
   ctx->c.def_explicit_max = ctx->c.explicit_max;
   ctx->c.def_item_terms = ctx->c.chain_item_terms;
   ctx->c.def_group_terms = ctx->c.chain_group_terms;
@@ -2106,6 +2123,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_FCHAIN")) ctx->c.use_fchain = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_FCHAIN_MIN")) ctx->c.fchain_min = std::atoi(e);
   if (const char* e = std::getenv("CHM_FCHAIN_FP32")) ctx->c.fchain_fp32 = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CHM_FCHAIN_MAXD")) ctx->c.fchain_maxd = std::atoi(e);
   ctx->c.def_fchain_min = ctx->c.fchain_min;
   if (const char* e = std::getenv("CHM_DAVIDSON_DEVICE")) ctx->c.dav_device = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_DAVIDSON_GRAPH")) ctx->c.dav_graph = std::atoi(e) != 0;

@@ -160,7 +160,7 @@ struct Ctx : HostState {
   DBuf<double> Mvals;
   DBuf<int> Mcol;
   // Heff forms: 0 FP64; 1 mixed: the explicit matrix stored as hi/lo float pairs and the
-  // fused chained items in FP32 (k_fchain32); the Davidson iterates with these
+  // fused chained items in FP32 (k_fchain on FP32 MFMA); the Davidson iterates with these
   // low-precision products (hi floats only) until the residual falls below mixed_switch
   // or stops falling, then restarts from the Ritz vector with FP64 products (hi + lo,
   // 2^-48) to the tolerance.
@@ -218,6 +218,9 @@ struct Ctx : HostState {
   struct TuneRow {
     int64_t dmax_lo, dmax_hi, v_lo, v_hi, explicit_max, item_terms, group_terms, chain_mfma, fchain_min;
   };
+  // knobs the environment set at creation (explicit_max, item_terms, group_terms,
+  // chain_mfma, fchain_min): the table does not override them
+  bool env_pinned[5] = {false, false, false, false, false};
   std::vector<TuneRow> tune;
   int64_t def_explicit_max = 4096;
   int def_item_terms = CHAIN_ITEM_TERMS, def_group_terms = CHAIN_GROUP_TERMS;
@@ -242,18 +245,21 @@ struct Ctx : HostState {
   int ncblocks = 0;
   int chain_P = 1;  // doubles per LDS array of the chained kernel
   // k_fchain (chained products of blocks up to FCHAIN_MAXD on the matrix cores): work
-  // items of output blocks whose largest dimension is at most 64 / above, and that
-  // dimension's maximum over each list (sets the launch's LDS). CHM_FCHAIN=0 turns it
-  // off; blocks whose largest dimension is at most fchain_min (CHM_FCHAIN_MIN) stay on
-  // k_chain.
+  // items in buckets of tile shape (fchain_bucket). CHM_FCHAIN=0
+  // turns it off; blocks whose largest dimension is at most fchain_min (CHM_FCHAIN_MIN)
+  // stay on k_chain.
   bool use_fchain = true, fchain_ok = false;
   // the mixed form's low-precision products run k_fchain's items in FP32 on the matrix
-  // cores (k_fchain32); CHM_FCHAIN_FP32=0 keeps them FP64
+  // cores; CHM_FCHAIN_FP32=0 keeps them FP64
   bool fchain_fp32 = true;
   int fchain_min = 32, def_fchain_min = 32;
-  DBuf<DevCBlock> cblocks_f[2];
-  int ncblocks_f[2] = {0, 0};
-  int fchain_dmax[2] = {1, 1};
+  // CHM_FCHAIN_MAXD: blocks above it run as GEMM tiles. 64 by default: blocks of 65-80
+  // fit the fused kernel (one workgroup per compute unit, 158 KB of LDS) but its operator
+  // loads are then exposed; at L = 32, D = 2000 the tiles take 41.5 ms at the middle site
+  // against the fused kernel's 46.0
+  int fchain_maxd = 64;
+  DBuf<DevCBlock> cblocks_f[FCHAIN_NB];
+  int ncblocks_f[FCHAIN_NB] = {};
   DBuf<DevCBlock> cblocks_l;  // blocks of the large variant (same groups and terms)
   int ncblocks_l = 0;
   int chain_P_l = 1;

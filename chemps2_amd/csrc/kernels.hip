@@ -742,408 +742,286 @@ __global__ __launch_bounds__(256) void k_chain_mfma(const DevCBlock* __restrict_
 }
 #undef CHM_MFMA
 
-// Chained products of blocks up to FCHAIN_MAXD on the matrix cores, one workgroup per
-// work item (the records of k_chain: an output block and a range of its input groups),
-// output-stationary. The output block is split into 16 x 16 tiles dealt round-robin to
-// the four wavefronts, each holding its tiles as v_mfma_f64_16x16x4f64 accumulators
-// (lane l = lr + 16 lg supplies A(lr, k0 + lg) and B(k0 + lg, lr) and holds D(4 r + lg, lr)
-// in register r). Per input group x is staged in LDS, zero-padded to whole tiles, with an
-// odd leading dimension (conflict-free reads along both operand directions); a
-// two-operand term first forms its intermediate W = alpha op(A) x or x op(B) in LDS, all
-// waves sharing its tiles, then every wave adds W op(B) or alpha op(A) W to its tiles.
-// Operator blocks are read from global memory (cache-resident across the terms of an
-// item); their reads past a dimension are masked to zero.
-constexpr int FCHAIN_XS = (FCHAIN_MAXD + 1) * FCHAIN_MAXD;  // doubles of one LDS array
+// Chained products of blocks up to FCHAIN_MAXD on the matrix cores, one workgroup of four
+// wavefronts per work item (an output block and a range of its input groups, the
+// records of k_chain), output-stationary. Every term of an item has one of the forms
+//   mode 0: Y += a X,  1: Y += op(A) X,  2: Y += X op(B),  3: Y += (op(A) X) op(B);
+// the other association order, op(A) (X op(B)), is mode 3 of the transposed block: an
+// item with tr set computes Y^T += op(B)^T X^T op(A)^T, its terms' operators exchanged
+// and their transposition flags flipped on the host. Wavefront w owns the 16-row strip w
+// of Y and holds it as 16 x 16 accumulator tiles (a fifth strip is shared by all four,
+// each with part of every term's work, and its partial sums are added at the end of
+// the item). A two-operand term forms
+// its strip of the intermediate transposed, W^T = X^T op(A)^T, whose result tiles are
+// directly the A operands of W op(B) (the register chaining of k_chain_mfma), so the
+// intermediate never leaves registers. op(A), op(B) (the factor a folded into one of
+// them) and, per input group, X are staged in LDS zero-padded to whole tiles, with fixed
+// leading dimensions (compile-time offsets; op(A) is read along rows, 16 mod 32, the
+// others along columns, 4 mod 8, against bank conflicts); the next term's operators
+// are loaded into registers while the current term computes, and stored after the
+// barrier that ends it.
+//   f64: v_mfma_f64_16x16x4f64, result register r of lane (lr, lg) holds row 4 r + lg;
+//   f32 (the mixed form's low-precision products): v_mfma_f32_16x16x4f32, row 4 lg + r.
+// In a chained product lane group lg supplies the k index of its register, so the k
+// order of a step follows the result layout (any bijection of k onto the lanes serves).
+typedef float chm_f4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ double gld(const double* __restrict__ p, bool ok, int64_t idx) {
-  const double v = p[ok ? idx : 0];
-  return ok ? v : 0.0;
-}
-
-#define FMFMA(a, b, c) __builtin_amdgcn_mfma_f64_16x16x4f64((a), (b), (c), 0, 0, 0)
+template <bool F32>
+struct FcMath;
+template <>
+struct FcMath<false> {
+  typedef double R;
+  typedef chm_d4 V4;
+  static __device__ __forceinline__ V4 mma(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  static __device__ __forceinline__ int row(int r, int lg) { return 4 * r + lg; }
+};
+template <>
+struct FcMath<true> {
+  typedef float R;
+  typedef chm_f4 V4;
+  static __device__ __forceinline__ V4 mma(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static __device__ __forceinline__ int row(int r, int lg) { return 4 * lg + r; }
+};
 
 template <int NT>
+struct FcShape {
+  static constexpr int D = 16 * NT;
+  static constexpr int LDA = (NT & 1) ? D : D + 16;
+  static constexpr int LDB = D + 4;
+  static constexpr int MS = (NT + 3) / 4;  // strips per wavefront
+  static constexpr int NP = NT * NT;       // staged elements per thread
+  static constexpr int ELEMS = LDA * D + 2 * LDB * D;
+};
+
+// Loads the nf x ns matrix p (element (f, s) at p[f + ld s]) padded to whole 16 x 16
+// tiles: thread (f0, s0) = (tid % 16, tid / 16) takes (f0 + 16 fi, s0 + 16 si). The loads
+// are global (not flat: those would also hold the LDS counter the compute waits on), and a
+// padding element reads the block's first element and is replaced by zero.
+typedef __attribute__((address_space(1))) const double gdouble;
+template <int NT, typename P>
+__device__ __forceinline__ void fc_load(P (&v)[NT * NT], const double* __restrict__ p, int64_t ld, int nf, int ns, int tid) {
+  const int f0 = tid & 15, s0 = tid >> 4;
+  const int nf16 = (nf + 15) >> 4, ns16 = (ns + 15) >> 4;
+  gdouble* g = (gdouble*)p;
+#pragma unroll
+  for (int si = 0; si < NT; si++) {
+    if (si >= ns16) break;
+#pragma unroll
+    for (int fi = 0; fi < NT; fi++) {
+      if (fi >= nf16) break;
+      const int f = f0 + 16 * fi, s = s0 + 16 * si;
+      const bool ok = f < nf && s < ns;
+      const double e = g[ok ? f + ld * s : 0];
+      v[si * NT + fi] = ok ? (P)e : (P)0;
+    }
+  }
+}
+
+// Stores what fc_load loaded into an LDS matrix with leading dimension LD: element
+// (f, s) at (f, s), or at (s, f) when sw, scaled by sc
+template <int NT, int LD, typename R, typename P>
+__device__ __forceinline__ void fc_store(R* __restrict__ dst, const P (&v)[NT * NT], bool sw, int nf, int ns, int tid,
+                                         double sc) {
+  const int f0 = tid & 15, s0 = tid >> 4;
+  const int nf16 = (nf + 15) >> 4, ns16 = (ns + 15) >> 4;
+#pragma unroll
+  for (int si = 0; si < NT; si++) {
+    if (si >= ns16) break;
+#pragma unroll
+    for (int fi = 0; fi < NT; fi++) {
+      if (fi >= nf16) break;
+      const int f = f0 + 16 * fi, s = s0 + 16 * si;
+      dst[sw ? s + LD * f : f + LD * s] = (R)(sc * v[si * NT + fi]);
+    }
+  }
+}
+
+template <int NT, int NJ, bool F32>
 __global__ __launch_bounds__(256) void k_fchain(const DevCBlock* __restrict__ blocks, const DevCGroup* __restrict__ groups,
                                                const DevCTerm* __restrict__ terms, const double* __restrict__ x,
-                                               double* __restrict__ y, double* __restrict__ part, int lds_stride) {
-  extern __shared__ double lds[];
-  double* Xs = lds;
-  double* Ws = lds + lds_stride;
+                                               double* __restrict__ y, double* __restrict__ part) {
+  typedef FcMath<F32> M;
+  typedef typename M::R R;
+  typedef typename M::V4 V4;
+  typedef FcShape<NT> SH;
+  constexpr int LDA = SH::LDA, LDB = SH::LDB, MS = SH::MS, NP = SH::NP;
+  extern __shared__ __align__(16) unsigned char fc_lds[];
+  R* __restrict__ As = (R*)fc_lds;
+  R* __restrict__ Bs = As + LDA * SH::D;
+  R* __restrict__ Xs = Bs + LDB * SH::D;
   const DevCBlock b = blocks[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, lr = lane & 15, lg = lane >> 4;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int dL = b.dL, dR = b.dR;
-  const int nL = (dL + 15) >> 4, nR = (dR + 15) >> 4, nt = nL * nR;
-  chm_d4 acc[NT];
+  const bool tr = b.tr != 0;
+  const int dLp = tr ? b.dR : b.dL, dRp = tr ? b.dL : b.dR;  // the block as computed
+  const int nL = (dLp + 15) >> 4;  // strips; the block has NJ column tiles
+  V4 acc[MS][NJ];
 #pragma unroll
-  for (int s = 0; s < NT; s++) acc[s] = chm_d4{0.0, 0.0, 0.0, 0.0};
-  for (int g = b.g0; g < b.g0 + b.ng; g++) {
-    const DevCGroup G = groups[g];
-    const int dLi = G.dLi, dRi = G.dRi;
-    const int nLi = (dLi + 15) >> 4, nRi = (dRi + 15) >> 4;
-    const int ldx = 16 * nLi + 1;
-    __syncthreads();  // the previous group's readers of Xs and Ws are done
-    {
-      const double* __restrict__ X = x + G.xoff;
-      const int rows = 16 * nLi, cnt = rows * 16 * nRi;
-      for (int e = tid; e < cnt; e += 256) {
-        const int i = e % rows, j = e / rows;
-        Xs[i + ldx * j] = (i < dLi && j < dRi) ? X[i + (int64_t)dLi * j] : 0.0;
+  for (int m = 0; m < MS; m++)
+#pragma unroll
+    for (int j = 0; j < NJ; j++) acc[m][j] = V4{0, 0, 0, 0};
+
+  R pa[NP], pb[NP];  // the next term's operators (FP32 variant: rounded as loaded)
+  // operators of term T (input group G) into registers: op(A) is dLp x dLi', stored with
+  // element (i, k) at A[i + lda k], or at A[i lda + k] when ta; op(B) is dRi' x dRp
+  auto load_ops = [&](const DevCTerm& T, const DevCGroup& G) {
+    const int dLip = tr ? G.dRi : G.dLi, dRip = tr ? G.dLi : G.dRi;
+    if (T.A) fc_load<NT, R>(pa, T.A, T.lda, T.ta ? dLip : dLp, T.ta ? dLp : dLip, tid);
+    if (T.B) fc_load<NT, R>(pb, T.B, T.ldb, T.tb ? dRp : dRip, T.tb ? dRip : dRp, tid);
+  };
+  auto store_ops = [&](const DevCTerm& T, const DevCGroup& G) {
+    const int dLip = tr ? G.dRi : G.dLi, dRip = tr ? G.dLi : G.dRi;
+    if (T.A) fc_store<NT, LDA>(As, pa, T.ta != 0, T.ta ? dLip : dLp, T.ta ? dLp : dLip, tid, T.alpha);
+    if (T.B) fc_store<NT, LDB>(Bs, pb, T.tb != 0, T.tb ? dRp : dRip, T.tb ? dRip : dRp, tid, T.A ? 1.0 : T.alpha);
+  };
+  // x of group G (dLi x dRi, column-major) as X' = X, or X^T when tr
+  auto stage_x = [&](const DevCGroup& G) {
+    R px[NP];
+    fc_load<NT, R>(px, x + G.xoff, G.dLi, G.dLi, G.dRi, tid);
+    fc_store<NT, LDB>(Xs, px, tr, G.dLi, G.dRi, tid, 1.0);
+  };
+
+  const int gend = b.g0 + b.ng;
+  int g = b.g0;
+  DevCGroup G = groups[g];
+  int t = G.t0;
+  DevCTerm T = terms[t];
+  stage_x(G);
+  load_ops(T, G);
+  store_ops(T, G);
+  __syncthreads();
+  for (;;) {
+    int tn = t + 1, gn = g;
+    DevCGroup Gn = G;
+    if (tn == G.t0 + G.nt) {
+      gn = g + 1;
+      if (gn < gend) {
+        Gn = groups[gn];
+        tn = Gn.t0;
       }
     }
+    const bool more = gn < gend;
+    DevCTerm Tn = T;
+    if (more) {
+      Tn = terms[tn];
+      load_ops(Tn, Gn);  // in flight while this term computes
+    }
+    const int nLi = tr ? (G.dRi + 15) >> 4 : (G.dLi + 15) >> 4;
+    const int nRi = tr ? (G.dLi + 15) >> 4 : (G.dRi + 15) >> 4;
+    const int mode = T.mode;
+    // Every wavefront runs the same instruction stream: one without a strip of its own in
+    // a pass repeats the last strip (its sums are not stored). The tile loops have fixed trip
+    // counts, so that the accumulators stay in place (tiles past the block's own see only
+    // padding or other data and are discarded); the k loops run over the block's size.
+#pragma unroll
+    for (int m = 0; m < MS; m++) {
+      if (4 * m >= nL) break;  // no strip of this pass in the block (uniform)
+      // pass 0: wavefront w owns strip w; pass 1 (blocks of 5 strips): the four
+      // wavefronts share strip 4, each taking every fourth k-block or intermediate tile
+      // into a partial sum that the end of the item adds up
+      const bool shared = m > 0;
+      const int s = shared ? 4 : min(w, nL - 1);
+      const int q0 = shared ? w : 0, dq = shared ? 4 : 1;
+      const int ra = 16 * s + lr;  // this lane's row of op(A), and of X in mode 2
+      if (mode == 3) {
+#pragma unroll 1
+        for (int jj = q0; jj < nRi; jj += dq) {
+          V4 wt = V4{0, 0, 0, 0};  // tile jj of W^T = X^T op(A)^T
+#pragma unroll 1
+          for (int kb = 0; kb < nLi; kb++) {
+            const R* __restrict__ ap = As + ra + LDA * (16 * kb + lg);
+            const R* __restrict__ xp = Xs + (16 * kb + lg) + LDB * (16 * jj + lr);
+#pragma unroll
+            for (int u = 0; u < 4; u++) wt = M::mma(xp[4 * u], ap[LDA * 4 * u], wt);
+          }
+          // Y += W op(B) over the 16 k of tile jj, W's tile as the A operands
+          const R* __restrict__ bp = Bs + (16 * jj + M::row(0, lg)) + LDB * lr;
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const R a = wt[r];
+#pragma unroll
+            for (int j = 0; j < NJ; j++) acc[m][j] = M::mma(a, bp[M::row(r, 0) + LDB * 16 * j], acc[m][j]);
+          }
+        }
+      } else if (mode == 1) {
+#pragma unroll 1
+        for (int kb = q0; kb < nLi; kb += dq) {
+          const R* __restrict__ ap = As + ra + LDA * (16 * kb + lg);
+          const R* __restrict__ xp = Xs + (16 * kb + lg) + LDB * lr;
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const R a = ap[LDA * 4 * u];
+#pragma unroll
+            for (int j = 0; j < NJ; j++) acc[m][j] = M::mma(a, xp[4 * u + LDB * 16 * j], acc[m][j]);
+          }
+        }
+      } else if (mode == 2) {
+#pragma unroll 1
+        for (int kb = q0; kb < nRi; kb += dq) {
+          const R* __restrict__ xp = Xs + ra + LDB * (16 * kb + lg);
+          const R* __restrict__ bp = Bs + (16 * kb + lg) + LDB * lr;
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const R a = xp[LDB * 4 * u];
+#pragma unroll
+            for (int j = 0; j < NJ; j++) acc[m][j] = M::mma(a, bp[4 * u + LDB * 16 * j], acc[m][j]);
+          }
+        }
+      } else if (!shared || w == 0) {
+        const R al = (R)T.alpha;
+#pragma unroll
+        for (int j = 0; j < NJ; j++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) acc[m][j][r] += al * Xs[(16 * s + M::row(r, lg)) + LDB * (16 * j + lr)];
+      }
+    }
+    __syncthreads();  // every wavefront is done with this term's LDS
+    if (!more) break;
+    if (gn != g) stage_x(Gn);
+    store_ops(Tn, Gn);
     __syncthreads();
-    for (int t = G.t0; t < G.t0 + G.nt; t++) {
-      const DevCTerm T = terms[t];
-      const double al = T.alpha;
-      // op(A)(i, k) = A[i ai + k ak] (dL x dLi), op(B)(k, j) = B[k bk + j bj] (dRi x dR)
-      const int64_t ai = T.ta ? T.lda : 1, ak = T.ta ? 1 : T.lda;
-      const int64_t bk = T.tb ? T.ldb : 1, bj = T.tb ? 1 : T.ldb;
-      const double* __restrict__ A = T.A ? T.A : x;  // absent operands are never read
-      const double* __restrict__ B = T.B ? T.B : x;
-      const int mode = T.mode;
-      if (mode == 3) {  // W = al op(A) X (16 nL x 16 nRi), then Y += W op(B)
-        const int ldw = 16 * nL + 1;
-        for (int q = w; q < nL * nRi; q += 4) {
-          const int ti = q % nL, tj = q / nL;
-          const int i = 16 * ti + lr;
-          chm_d4 d = chm_d4{0.0, 0.0, 0.0, 0.0};
-          for (int k0 = 0; k0 < 16 * nLi; k0 += 16) {
-            double av[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int k = k0 + 4 * u + lg;
-              av[u] = gld(A, i < dL && k < dLi, i * ai + k * ak);
-              bv[u] = Xs[k + ldx * (16 * tj + lr)];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) d = FMFMA(av[u], bv[u], d);
-          }
-#pragma unroll
-          for (int r = 0; r < 4; r++) Ws[(16 * ti + 4 * r + lg) + ldw * (16 * tj + lr)] = al * d[r];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < NT; s++) {
-          const int q = w + 4 * s;
-          if (q >= nt) break;
-          const int ti = q % nL, tj = q / nL;
-          const int j = 16 * tj + lr;
-          for (int k0 = 0; k0 < 16 * nRi; k0 += 16) {
-            double av[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int k = k0 + 4 * u + lg;
-              av[u] = Ws[(16 * ti + lr) + ldw * k];
-              bv[u] = gld(B, k < dRi && j < dR, k * bk + j * bj);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) acc[s] = FMFMA(av[u], bv[u], acc[s]);
-          }
-        }
-        __syncthreads();  // Ws is rewritten by the next term
-      } else if (mode == 4) {  // W = X op(B) (16 nLi x 16 nR), then Y += al op(A) W
-        const int ldw = 16 * nLi + 1;
-        for (int q = w; q < nLi * nR; q += 4) {
-          const int ti = q % nLi, tj = q / nLi;
-          const int j = 16 * tj + lr;
-          chm_d4 d = chm_d4{0.0, 0.0, 0.0, 0.0};
-          for (int k0 = 0; k0 < 16 * nRi; k0 += 16) {
-            double av[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int k = k0 + 4 * u + lg;
-              av[u] = Xs[(16 * ti + lr) + ldx * k];
-              bv[u] = gld(B, k < dRi && j < dR, k * bk + j * bj);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) d = FMFMA(av[u], bv[u], d);
-          }
-#pragma unroll
-          for (int r = 0; r < 4; r++) Ws[(16 * ti + 4 * r + lg) + ldw * (16 * tj + lr)] = d[r];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < NT; s++) {
-          const int q = w + 4 * s;
-          if (q >= nt) break;
-          const int ti = q % nL, tj = q / nL;
-          const int i = 16 * ti + lr;
-          for (int k0 = 0; k0 < 16 * nLi; k0 += 16) {
-            double av[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int k = k0 + 4 * u + lg;
-              av[u] = al * gld(A, i < dL && k < dLi, i * ai + k * ak);
-              bv[u] = Ws[k + ldw * (16 * tj + lr)];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) acc[s] = FMFMA(av[u], bv[u], acc[s]);
-          }
-        }
-        __syncthreads();
-      } else if (mode == 1) {  // Y += al op(A) X
-#pragma unroll
-        for (int s = 0; s < NT; s++) {
-          const int q = w + 4 * s;
-          if (q >= nt) break;
-          const int ti = q % nL, tj = q / nL;
-          const int i = 16 * ti + lr;
-          for (int k0 = 0; k0 < 16 * nLi; k0 += 16) {
-            double av[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int k = k0 + 4 * u + lg;
-              av[u] = al * gld(A, i < dL && k < dLi, i * ai + k * ak);
-              bv[u] = Xs[k + ldx * (16 * tj + lr)];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) acc[s] = FMFMA(av[u], bv[u], acc[s]);
-          }
-        }
-      } else if (mode == 2) {  // Y += al X op(B)
-#pragma unroll
-        for (int s = 0; s < NT; s++) {
-          const int q = w + 4 * s;
-          if (q >= nt) break;
-          const int ti = q % nL, tj = q / nL;
-          const int j = 16 * tj + lr;
-          for (int k0 = 0; k0 < 16 * nRi; k0 += 16) {
-            double av[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int k = k0 + 4 * u + lg;
-              av[u] = al * Xs[(16 * ti + lr) + ldx * k];
-              bv[u] = gld(B, k < dRi && j < dR, k * bk + j * bj);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) acc[s] = FMFMA(av[u], bv[u], acc[s]);
-          }
-        }
-      } else {  // Y += al X (same dimensions)
-#pragma unroll
-        for (int s = 0; s < NT; s++) {
-          const int q = w + 4 * s;
-          if (q >= nt) break;
-          const int ti = q % nL, tj = q / nL;
-#pragma unroll
-          for (int r = 0; r < 4; r++) acc[s][r] += al * Xs[(16 * ti + 4 * r + lg) + ldx * (16 * tj + lr)];
-        }
-      }
-    }
+    t = tn;
+    g = gn;
+    G = Gn;
+    T = Tn;
   }
-  double* Y = b.direct ? y + b.out : part + b.out;
+  if (MS > 1 && nL > 4) {  // the shared strip: partial sums through LDS (the term loop ended on a barrier)
+    R* __restrict__ red = As;
 #pragma unroll
-  for (int s = 0; s < NT; s++) {
-    const int q = w + 4 * s;
-    if (q >= nt) break;
-    const int ti = q % nL, tj = q / nL;
+    for (int j = 0; j < NJ; j++)
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int l = 16 * ti + 4 * r + lg, c = 16 * tj + lr;
-      if (l < dL && c < dR) {
-        if (b.direct) Y[l + (int64_t)dL * c] += acc[s][r];
-        else Y[l + (int64_t)dL * c] = acc[s][r];
+      for (int r = 0; r < 4; r++) red[((w * NJ + j) * 4 + r) * 64 + lane] = acc[MS - 1][j][r];
+    __syncthreads();
+    if (w == 0)
+#pragma unroll
+      for (int j = 0; j < NJ; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          R v = red[(j * 4 + r) * 64 + lane];
+#pragma unroll
+          for (int q = 1; q < 4; q++) v += red[((q * NJ + j) * 4 + r) * 64 + lane];
+          acc[MS - 1][j][r] = v;
+        }
+  }
+  double* __restrict__ Y = b.direct ? y + b.out : part + b.out;
+#pragma unroll
+  for (int m = 0; m < MS; m++) {
+    const int s = m > 0 ? 4 : w;  // the shared strip is stored by wavefront 0
+    if (s >= nL || (m > 0 && w != 0)) break;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int l = 16 * s + M::row(r, lg), c = 16 * j + lr;  // element (l, c) of the block as computed
+        if (l < dLp && c < dRp) {
+          const int64_t o = tr ? c + (int64_t)b.dL * l : l + (int64_t)b.dL * c;
+          if (b.direct) Y[o] += (double)acc[m][j][r];
+          else Y[o] = (double)acc[m][j][r];
+        }
       }
     }
   }
 }
-#undef FMFMA
-
-// The same chained products with FP32 operands and FP32 accumulation on
-// v_mfma_f32_16x16x4f32, for the low-precision phase of the mixed form: x and the
-// intermediate are staged in LDS as floats, operator blocks are rounded to float as they
-// are read, and the item's sum is added to y in FP64. The f32 instruction returns
-// D(4 lg + r, lr) in register r of lane (lr, lg) (rows in blocks of four, unlike the f64
-// one's interleaved rows). Relative error of one product about 2^-24 sqrt(K) per term.
-typedef float chm_f4 __attribute__((ext_vector_type(4)));
-#define FMFMA32(a, b, c) __builtin_amdgcn_mfma_f32_16x16x4f32((a), (b), (c), 0, 0, 0)
-
-template <int NT>
-__global__ __launch_bounds__(256) void k_fchain32(const DevCBlock* __restrict__ blocks, const DevCGroup* __restrict__ groups,
-                                               const DevCTerm* __restrict__ terms, const double* __restrict__ x,
-                                               double* __restrict__ y, double* __restrict__ part, int lds_stride) {
-  extern __shared__ float ldsf[];
-  float* Xs = ldsf;
-  float* Ws = ldsf + lds_stride;
-  const DevCBlock b = blocks[blockIdx.x];
-  const int tid = threadIdx.x, lane = tid & 63, lr = lane & 15, lg = lane >> 4;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int dL = b.dL, dR = b.dR;
-  const int nL = (dL + 15) >> 4, nR = (dR + 15) >> 4, nt = nL * nR;
-  chm_f4 acc[NT];
-#pragma unroll
-  for (int s = 0; s < NT; s++) acc[s] = chm_f4{0.f, 0.f, 0.f, 0.f};
-  for (int g = b.g0; g < b.g0 + b.ng; g++) {
-    const DevCGroup G = groups[g];
-    const int dLi = G.dLi, dRi = G.dRi;
-    const int nLi = (dLi + 15) >> 4, nRi = (dRi + 15) >> 4;
-    const int ldx = 16 * nLi + 1;
-    __syncthreads();  // the previous group's readers of Xs and Ws are done
-    {
-      const double* __restrict__ X = x + G.xoff;
-      const int rows = 16 * nLi, cnt = rows * 16 * nRi;
-      for (int e = tid; e < cnt; e += 256) {
-        const int i = e % rows, j = e / rows;
-        Xs[i + ldx * j] = (i < dLi && j < dRi) ? (float)X[i + (int64_t)dLi * j] : 0.f;
-      }
-    }
-    __syncthreads();
-    for (int t = G.t0; t < G.t0 + G.nt; t++) {
-      const DevCTerm T = terms[t];
-      const float al = (float)T.alpha;
-      // op(A)(i, k) = A[i ai + k ak] (dL x dLi), op(B)(k, j) = B[k bk + j bj] (dRi x dR)
-      const int64_t ai = T.ta ? T.lda : 1, ak = T.ta ? 1 : T.lda;
-      const int64_t bk = T.tb ? T.ldb : 1, bj = T.tb ? 1 : T.ldb;
-      const double* __restrict__ A = T.A ? T.A : x;  // absent operands are never read
-      const double* __restrict__ B = T.B ? T.B : x;
-      const int mode = T.mode;
-      if (mode == 3) {  // W = al op(A) X (16 nL x 16 nRi), then Y += W op(B)
-        const int ldw = 16 * nL + 1;
-        for (int q = w; q < nL * nRi; q += 4) {
-          const int ti = q % nL, tj = q / nL;
-          const int i = 16 * ti + lr;
-          chm_f4 d = chm_f4{0.f, 0.f, 0.f, 0.f};
-          for (int k0 = 0; k0 < 16 * nLi; k0 += 16) {
-            float av[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int k = k0 + 4 * u + lg;
-              av[u] = (float)gld(A, i < dL && k < dLi, i * ai + k * ak);
-              bv[u] = Xs[k + ldx * (16 * tj + lr)];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) d = FMFMA32(av[u], bv[u], d);
-          }
-#pragma unroll
-          for (int r = 0; r < 4; r++) Ws[(16 * ti + 4 * lg + r) + ldw * (16 * tj + lr)] = al * d[r];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < NT; s++) {
-          const int q = w + 4 * s;
-          if (q >= nt) break;
-          const int ti = q % nL, tj = q / nL;
-          const int j = 16 * tj + lr;
-          for (int k0 = 0; k0 < 16 * nRi; k0 += 16) {
-            float av[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int k = k0 + 4 * u + lg;
-              av[u] = Ws[(16 * ti + lr) + ldw * k];
-              bv[u] = (float)gld(B, k < dRi && j < dR, k * bk + j * bj);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) acc[s] = FMFMA32(av[u], bv[u], acc[s]);
-          }
-        }
-        __syncthreads();  // Ws is rewritten by the next term
-      } else if (mode == 4) {  // W = X op(B) (16 nLi x 16 nR), then Y += al op(A) W
-        const int ldw = 16 * nLi + 1;
-        for (int q = w; q < nLi * nR; q += 4) {
-          const int ti = q % nLi, tj = q / nLi;
-          const int j = 16 * tj + lr;
-          chm_f4 d = chm_f4{0.f, 0.f, 0.f, 0.f};
-          for (int k0 = 0; k0 < 16 * nRi; k0 += 16) {
-            float av[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int k = k0 + 4 * u + lg;
-              av[u] = Xs[(16 * ti + lr) + ldx * k];
-              bv[u] = (float)gld(B, k < dRi && j < dR, k * bk + j * bj);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) d = FMFMA32(av[u], bv[u], d);
-          }
-#pragma unroll
-          for (int r = 0; r < 4; r++) Ws[(16 * ti + 4 * lg + r) + ldw * (16 * tj + lr)] = d[r];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < NT; s++) {
-          const int q = w + 4 * s;
-          if (q >= nt) break;
-          const int ti = q % nL, tj = q / nL;
-          const int i = 16 * ti + lr;
-          for (int k0 = 0; k0 < 16 * nLi; k0 += 16) {
-            float av[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int k = k0 + 4 * u + lg;
-              av[u] = al * (float)gld(A, i < dL && k < dLi, i * ai + k * ak);
-              bv[u] = Ws[k + ldw * (16 * tj + lr)];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) acc[s] = FMFMA32(av[u], bv[u], acc[s]);
-          }
-        }
-        __syncthreads();
-      } else if (mode == 1) {  // Y += al op(A) X
-#pragma unroll
-        for (int s = 0; s < NT; s++) {
-          const int q = w + 4 * s;
-          if (q >= nt) break;
-          const int ti = q % nL, tj = q / nL;
-          const int i = 16 * ti + lr;
-          for (int k0 = 0; k0 < 16 * nLi; k0 += 16) {
-            float av[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int k = k0 + 4 * u + lg;
-              av[u] = al * (float)gld(A, i < dL && k < dLi, i * ai + k * ak);
-              bv[u] = Xs[k + ldx * (16 * tj + lr)];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) acc[s] = FMFMA32(av[u], bv[u], acc[s]);
-          }
-        }
-      } else if (mode == 2) {  // Y += al X op(B)
-#pragma unroll
-        for (int s = 0; s < NT; s++) {
-          const int q = w + 4 * s;
-          if (q >= nt) break;
-          const int ti = q % nL, tj = q / nL;
-          const int j = 16 * tj + lr;
-          for (int k0 = 0; k0 < 16 * nRi; k0 += 16) {
-            float av[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int k = k0 + 4 * u + lg;
-              av[u] = al * Xs[(16 * ti + lr) + ldx * k];
-              bv[u] = (float)gld(B, k < dRi && j < dR, k * bk + j * bj);
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) acc[s] = FMFMA32(av[u], bv[u], acc[s]);
-          }
-        }
-      } else {  // Y += al X (same dimensions)
-#pragma unroll
-        for (int s = 0; s < NT; s++) {
-          const int q = w + 4 * s;
-          if (q >= nt) break;
-          const int ti = q % nL, tj = q / nL;
-#pragma unroll
-          for (int r = 0; r < 4; r++) acc[s][r] += al * Xs[(16 * ti + 4 * lg + r) + ldx * (16 * tj + lr)];
-        }
-      }
-    }
-  }
-  double* Y = b.direct ? y + b.out : part + b.out;
-#pragma unroll
-  for (int s = 0; s < NT; s++) {
-    const int q = w + 4 * s;
-    if (q >= nt) break;
-    const int ti = q % nL, tj = q / nL;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int l = 16 * ti + 4 * lg + r, c = 16 * tj + lr;
-      if (l < dL && c < dR) {
-        if (b.direct) Y[l + (int64_t)dL * c] += (double)acc[s][r];
-        else Y[l + (int64_t)dL * c] = (double)acc[s][r];
-      }
-    }
-  }
-}
-#undef FMFMA32
 
 
 constexpr int CHAIN_PER_S = (18 * 18 + 255) / 256;                                  // 2 x 2 tiles of a 36 x 36 block
@@ -1375,8 +1253,9 @@ __global__ __launch_bounds__(256) void k_copies(const DevCopy* __restrict__ item
 }
 
 // One-sided (Hestenes) Jacobi SVD of the small centre-sector matrices of Split, one
-// workgroup per matrix. Each round rotates a set of disjoint column pairs (round-robin
-// tournament order), every pair on a group of SVD_TPP lanes; sweeps end when no pair
+// workgroup of SVD_THREADS per matrix. Each round rotates a set of disjoint column pairs
+// (round-robin tournament order), every pair on a group of SVD_TPP lanes, all n / 2 pairs
+// of a round at once for n up to 2 SVD_GROUPS (one barrier per round); sweeps end when no pair
 // has |a_p . a_q| > sqrt(m) eps |a_p| |a_q|, the rounding floor of an m-term dot product
 // (the stopping rule of one-sided Jacobi in Drmac & Veselic, SIAM J. Matrix Anal. Appl.
 // 29 (2008) 1322, as LAPACK's dgesvj uses it). A job that reaches SVD_MAX_SWEEPS without
@@ -1384,8 +1263,9 @@ __global__ __launch_bounds__(256) void k_copies(const DevCopy* __restrict__ item
 // they are not rotated (their directions are rounding noise and would keep the sweeps
 // going), their singular values are set to zero, and their vectors on the m-long side are
 // completed to an orthonormal set like those of exactly zero singular values.
-constexpr int SVD_TPP = 16;
-constexpr int SVD_GROUPS = 256 / SVD_TPP;
+constexpr int SVD_THREADS = 1024;
+constexpr int SVD_TPP = 8;
+constexpr int SVD_GROUPS = SVD_THREADS / SVD_TPP;
 constexpr double SVD_EPS = 1.1102230246251565e-16;  // 2^-53
 constexpr int SVD_MAX_SWEEPS = 80;
 
@@ -1394,7 +1274,7 @@ __device__ inline double group_sum(double v) {
   return v;
 }
 
-__global__ __launch_bounds__(256) void k_svd_jacobi(const DevSvd* __restrict__ items) {
+__global__ __launch_bounds__(SVD_THREADS) void k_svd_jacobi(const DevSvd* __restrict__ items) {
   const DevSvd it = items[blockIdx.x];
   const int m = it.m, n = it.n;
   const int tid = threadIdx.x, g = tid / SVD_TPP, l = tid % SVD_TPP;
@@ -1403,13 +1283,13 @@ __global__ __launch_bounds__(256) void k_svd_jacobi(const DevSvd* __restrict__ i
   __shared__ int rotated;
   __shared__ double sig[SVD_JACOBI_MAXN];
   __shared__ int src[SVD_JACOBI_MAXN];
-  __shared__ double fro[256];
+  __shared__ double fro[SVD_THREADS];
   double f2 = 0.0;
-  for (int e = tid; e < m * n; e += 256) f2 += A[e] * A[e];
+  for (int e = tid; e < m * n; e += SVD_THREADS) f2 += A[e] * A[e];
   fro[tid] = f2;
-  for (int e = tid; e < n * n; e += 256) Q[e] = (e % n == e / n) ? 1.0 : 0.0;
+  for (int e = tid; e < n * n; e += SVD_THREADS) Q[e] = (e % n == e / n) ? 1.0 : 0.0;
   __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
+  for (int h = SVD_THREADS / 2; h > 0; h >>= 1) {
     if (tid < h) fro[tid] += fro[tid + h];
     __syncthreads();
   }
@@ -1473,7 +1353,7 @@ __global__ __launch_bounds__(256) void k_svd_jacobi(const DevSvd* __restrict__ i
     if (l == 0) sig[j] = s2 > floor2 ? sqrt(s2) : 0.0;
   }
   __syncthreads();
-  for (int j = tid; j < n; j += 256) {
+  for (int j = tid; j < n; j += SVD_THREADS) {
     int rank = 0;
     for (int k = 0; k < n; k++) rank += (sig[k] > sig[j] || (sig[k] == sig[j] && k < j)) ? 1 : 0;
     src[rank] = j;
@@ -1902,38 +1782,47 @@ bool chain_kernel_setup() {
                              4 * CHAIN_MAXD * CHAIN_MAXD * (int)sizeof(double)) == hipSuccess;
 }
 
+typedef void (*FcKernel)(const DevCBlock*, const DevCGroup*, const DevCTerm*, const double*, double*, double*);
+
+template <bool F32>
+struct FcTable {
+  static constexpr FcKernel k[FCHAIN_NB] = {
+      k_fchain<2, 1, F32>, k_fchain<2, 2, F32>, k_fchain<3, 1, F32>, k_fchain<3, 2, F32>, k_fchain<3, 3, F32>,
+      k_fchain<4, 1, F32>, k_fchain<4, 2, F32>, k_fchain<4, 3, F32>, k_fchain<4, 4, F32>, k_fchain<5, 1, F32>,
+      k_fchain<5, 2, F32>, k_fchain<5, 3, F32>, k_fchain<5, 4, F32>, k_fchain<5, 5, F32>};
+};
+template <bool F32>
+constexpr FcKernel FcTable<F32>::k[FCHAIN_NB];
+
+static int fc_nt(int bucket) {
+  int nt = FCHAIN_NT0;
+  while (fchain_bucket(nt + 1, 1) <= bucket) nt++;
+  return nt;
+}
+
+static size_t fc_lds_bytes(int nt, bool f32) {
+  const int e = nt == 2 ? FcShape<2>::ELEMS : nt == 3 ? FcShape<3>::ELEMS : nt == 4 ? FcShape<4>::ELEMS : FcShape<5>::ELEMS;
+  return (size_t)e * (f32 ? sizeof(float) : sizeof(double));
+}
+
 bool fchain_kernel_setup() {
-  const bool a = hipFuncSetAttribute((const void*)k_fchain<FCHAIN_NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     2 * FCHAIN_XS * (int)sizeof(double)) == hipSuccess;
-  const bool b = hipFuncSetAttribute((const void*)k_fchain<4>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     2 * 65 * 64 * (int)sizeof(double)) == hipSuccess;
-  const bool c = hipFuncSetAttribute((const void*)k_fchain32<FCHAIN_NT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     2 * FCHAIN_XS * (int)sizeof(float)) == hipSuccess;
-  return a && b && c;
+  static_assert(16 * (FCHAIN_NT0 + 3) == FCHAIN_MAXD && fchain_bucket(5, 5) == FCHAIN_NB - 1, "fused chain buckets");
+  bool ok = true;
+  for (int bk = 0; bk < FCHAIN_NB; bk++)
+    for (int f = 0; f < 2; f++) {
+      const FcKernel k = f ? FcTable<true>::k[bk] : FcTable<false>::k[bk];
+      ok = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fc_lds_bytes(fc_nt(bk), f)) ==
+               hipSuccess && ok;
+    }
+  return ok;
 }
 
 void launch_fchain(const DevCBlock* blocks, int nblocks, const DevCGroup* groups, const DevCTerm* terms, const double* x,
-                   double* y, double* part, int dmax, bool fp32, hipStream_t s) {
-  if (nblocks <= 0) return;
-  const int d16 = (std::min(dmax, FCHAIN_MAXD) + 15) / 16 * 16;
-  const int stride = (d16 + 1) * d16;
-  if (fp32) {
-    if (d16 <= 64)
-      hipLaunchKernelGGL(k_fchain32<4>, dim3(nblocks), dim3(256), (size_t)2 * stride * sizeof(float), s, blocks, groups,
-                         terms, x, y, part, stride);
-    else
-      hipLaunchKernelGGL(k_fchain32<FCHAIN_NT>, dim3(nblocks), dim3(256), (size_t)2 * stride * sizeof(float), s, blocks,
-                         groups, terms, x, y, part, stride);
-    return;
-  }
-  if (d16 <= 64)  // at most 16 output tiles: 4 per wavefront
-    hipLaunchKernelGGL(k_fchain<4>, dim3(nblocks), dim3(256), (size_t)2 * stride * sizeof(double), s, blocks, groups, terms,
-                       x, y, part, stride);
-  else
-    hipLaunchKernelGGL(k_fchain<FCHAIN_NT>, dim3(nblocks), dim3(256), (size_t)2 * stride * sizeof(double), s, blocks, groups,
-                       terms, x, y, part, stride);
+                   double* y, double* part, int bucket, bool fp32, hipStream_t s) {
+  if (nblocks <= 0 || bucket < 0 || bucket >= FCHAIN_NB) return;
+  const FcKernel k = fp32 ? FcTable<true>::k[bucket] : FcTable<false>::k[bucket];
+  hipLaunchKernelGGL(k, dim3(nblocks), dim3(256), fc_lds_bytes(fc_nt(bucket), fp32), s, blocks, groups, terms, x, y, part);
 }
-
 
 int dot_parts(int64_t n) {
   int64_t g = (n + 255) / 256;
@@ -2038,7 +1927,7 @@ void launch_copies(const DevCopy* items, int n, hipStream_t s) {
 }
 
 void launch_svd_jacobi(const DevSvd* items, int n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_svd_jacobi, dim3(n), dim3(256), 0, s, items);
+  if (n > 0) hipLaunchKernelGGL(k_svd_jacobi, dim3(n), dim3(SVD_THREADS), 0, s, items);
 }
 
 void launch_add_noise(double* x, int64_t n, uint64_t seed, double level, hipStream_t s) {

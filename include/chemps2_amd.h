@@ -277,13 +277,16 @@ int chm_davidson_stats(const chm_ctx* ctx, int64_t* out);
 /* Routing and work-item sizes of the Heff kernels (no reference counterpart): the
  * largest |kappa||kappa'| of the explicit sector matrix, terms per chained work item and
  * per input group, the MFMA chained kernel for blocks up to 48 (1) or the LDS one (0),
- * and fchain_min: chained blocks whose largest dimension is above it (and at most 96) run
- * in the fused MFMA kernel k_fchain, the rest of those up to 64 in the LDS kernel; a
- * negative value (or 0 for the two term counts) keeps the current setting.
+ * and fchain_min: chained blocks whose largest dimension is above it (and at most 64, or
+ * CHM_FCHAIN_MAXD up to 80) run in the fused MFMA kernel k_fchain, the rest of those up
+ * to 64 in the LDS kernel; a negative value (or 0 for the two term counts) keeps the
+ * current setting.
  * chm_set_heff_knobs sets the defaults. chm_set_tuning installs the autotuner's table:
  * n rows of 9 values (largest block dimension lo, hi; V lo, hi; then the five knobs as
  * above); chm_heff_prepare uses the first row whose ranges hold the site's largest
- * block dimension and V, or the defaults. chm_tuning_used: out[7] = row used (-1: none),
+ * block dimension and V, or the defaults; a knob given in the environment when the
+ * context was created (CHM_EXPLICIT_MAX, CHM_CHAIN_ITEM_TERMS, CHM_CHAIN_GROUP_TERMS,
+ * CHM_CHAIN_MFMA, CHM_FCHAIN_MIN) keeps that value. chm_tuning_used: out[7] = row used (-1: none),
  * largest block dimension, then the five knobs of the last chm_heff_prepare. */
 int chm_set_heff_knobs(chm_ctx* ctx, int64_t explicit_max, int item_terms, int group_terms, int chain_mfma,
                        int fchain_min);

@@ -1,9 +1,11 @@
-"""The fused MFMA chained kernel (k_fchain: chained products of blocks up to 96 on the
-matrix cores, one workgroup per work item) against the oracle's Heff (HeffDiagrams2-5,
-Heff.cpp:43-248): every chained block it may take is sent to it (CHM_FCHAIN_MIN=0), at
-an oracle state of a D2h Hamiltonian (interior and edge sites, blocks of the <= 64
-variant) and at the 4 x 8 Hubbard cylinder at D = 1000 (blocks of 65-96 in the large
-variant, larger ones on the GEMM tiles). Products to 1e-12 relative."""
+"""The fused MFMA chained kernel (k_fchain: chained products of blocks up to 80 on the
+matrix cores, one workgroup per work item, row strips per wavefront, the other
+association order as the transposed block) against the oracle's Heff (HeffDiagrams2-5,
+Heff.cpp:43-248): every chained block it may take is sent to it (CHM_FCHAIN_MIN=0,
+CHM_FCHAIN_MAXD=80), at an oracle state of a D2h Hamiltonian (interior and edge sites)
+and at the 4 x 8 Hubbard cylinder at D = 1000 (blocks of 65-80 with their shared fifth
+strip, larger ones on the GEMM tiles). Products to 1e-12 relative; the FP32 variant of
+the mixed form within its error bound, refined to 1e-12."""
 import os
 
 import numpy as np
@@ -44,7 +46,7 @@ def test_fchain_oracle_state(site):
     x = np.random.default_rng(site).normal(size=n)
     y0, f0 = o.apply(x)
     print(f"case fused chain kernel, L=12 D=150 site {site}, V={n}", flush=True)
-    for env in ({"CHM_FCHAIN_MIN": "0", "CHM_EXPLICIT_MAX": "0"}, {"CHM_FCHAIN_MIN": "0"}):
+    for env in ({"CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80", "CHM_EXPLICIT_MAX": "0"}, {"CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80"}):
         c = _ctx(env)
         try:
             assert load_site(c, o, site) == n
@@ -65,7 +67,7 @@ def test_fchain_cylinder_D1000():
     x = np.random.default_rng(3).normal(size=n)
     y0, _ = o.apply(x)
     print(f"case fused chain kernel, cylinder D=1000 site 15, V={n}", flush=True)
-    c = _ctx({"CHM_FCHAIN_MIN": "0"})
+    c = _ctx({"CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80"})
     try:
         c.set_problem(32, 0, irr, 28, 0, 0, mx_table(T, V, 28))
         c.set_sectors_auto(1000)
@@ -92,7 +94,7 @@ def test_fchain_fp32_mixed_form(site):
     x = np.random.default_rng(site).normal(size=n)
     y0, _ = o.apply(x)
     print(f"case FP32 fused chain kernel, mixed form, L=12 D=150 site {site}, V={n}", flush=True)
-    for env in ({"CHM_FCHAIN_MIN": "0", "CHM_EXPLICIT_MAX": "0"}, {"CHM_FCHAIN_MIN": "0"}):
+    for env in ({"CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80", "CHM_EXPLICIT_MAX": "0"}, {"CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80"}):
         c = _ctx(env)
         try:
             c.set_heff_form(chemps2_amd.FORM_MIXED)
