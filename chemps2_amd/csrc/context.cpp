@@ -756,65 +756,105 @@ void Ctx::prepare(int st) {
       q0 = q1;
     }
   }
+  // Index pass (serial, integers only): every two-operand term's phase-1 op, its first
+  // tile and its W offset; every output block's first phase-2 tile. The records are then
+  // filled on the host threads.
+  const size_t nq = chain.size();
+  std::vector<int> opi1(nq, -1);
+  std::vector<int64_t> tile1(nq, 0);
   tb1.assign(1, 0);
   tb2.assign(1, 0);
-  size_t nextb = 1;
-  // phase 1: one group per two-operand term
-  for (size_t q = 0; q < chain.size(); q++) {
-    if (nextb < bstart.size() && q == bstart[nextb]) {
-      tb1.push_back((int)t1.size());
-      Wsize = 0;
-      nextb++;
-    }
-    const Term& t = *chain[q];
-    if (t.tl == 'I' || t.tr == 'I') continue;
-    const int dL = dLof(t.out), dR = dRof(t.out), dLi = dLof(t.in), dRi = dRof(t.in);
-    DevGemmOp op{};
-    WRec w;
-    if (!t.xr_first) {
-      w = {Wsize, dL, dRi};
-      op.A = resolve(plan->ref(t, t.Lb)); op.lda = (t.tl == 'N') ? dL : dLi; op.ta = (t.tl == 'T');
-      op.B = rel_x(t.in); op.ldb = dLi; op.tb = 0; op.k = dLi; op.flags = 2;
-    } else {
-      w = {Wsize, dLi, dR};
-      op.A = rel_x(t.in); op.lda = dLi; op.ta = 0;
-      op.B = resolve(plan->ref(t, t.Rb)); op.ldb = (t.tr == 'N') ? dRi : dR; op.tb = (t.tr == 'T'); op.k = dRi; op.flags = 1;
-    }
-    op.alpha = t.alpha;
-    op.kind = 0;
-    wrec[q] = w;
-    Wsize += ((int64_t)w.rows * w.cols + 7) / 8 * 8;
-    const int opi = (int)o1.size();
-    o1.push_back(op);
-    for (int m0 = 0; m0 < w.rows; m0 += 64)
-      for (int n0 = 0; n0 < w.cols; n0 += 64) {
-        DevTile tl{};
-        tl.C = (double*)(intptr_t)w.off;  // patched below once W exists
-        tl.ldc = w.rows; tl.m0 = m0; tl.n0 = n0; tl.m = w.rows; tl.n = w.cols; tl.op0 = opi; tl.nop = 1; tl.flags = 0;
-        t1.push_back(tl);
+  int64_t nt1 = 0;
+  int no1 = 0;
+  {
+    size_t nextb = 1;
+    for (size_t q = 0; q < nq; q++) {
+      if (nextb < bstart.size() && q == bstart[nextb]) {
+        tb1.push_back((int)nt1);
+        Wsize = 0;
+        nextb++;
       }
-  }
-  tb1.push_back((int)t1.size());
-  W.alloc(std::max<int64_t>(Wmax, 1));
-  for (auto& tl : t1) tl.C = W.p + (intptr_t)tl.C;
-  // phase 2: one group per output block, added to the explicit product
-  nextb = 1;
-  for (size_t q0 = 0; q0 < chain.size();) {
-    if (nextb < bstart.size() && q0 == bstart[nextb]) {
-      tb2.push_back((int)t2.size());
-      nextb++;
+      tile1[q] = nt1;
+      const Term& t = *chain[q];
+      if (t.tl == 'I' || t.tr == 'I') continue;
+      const int dL = dLof(t.out), dR = dRof(t.out), dLi = dLof(t.in), dRi = dRof(t.in);
+      const WRec w = t.xr_first ? WRec{Wsize, dLi, dR} : WRec{Wsize, dL, dRi};
+      wrec[q] = w;
+      Wsize += ((int64_t)w.rows * w.cols + 7) / 8 * 8;
+      opi1[q] = no1++;
+      nt1 += (int64_t)((w.rows + 63) / 64) * ((w.cols + 63) / 64);
     }
+    tb1.push_back((int)nt1);
+  }
+  std::vector<size_t> bq;  // first term of every output block
+  std::vector<int64_t> tile2;
+  int64_t nt2 = 0;
+  {
+    size_t nextb = 1;
+    for (size_t q0 = 0; q0 < nq;) {
+      if (nextb < bstart.size() && q0 == bstart[nextb]) {
+        tb2.push_back((int)nt2);
+        nextb++;
+      }
+      const int k = chain[q0]->out;
+      size_t q1 = q0;
+      while (q1 < nq && chain[q1]->out == k) q1++;
+      bq.push_back(q0);
+      tile2.push_back(nt2);
+      nt2 += (int64_t)((dLof(k) + 63) / 64) * ((dRof(k) + 63) / 64);
+      q0 = q1;
+    }
+    bq.push_back(nq);
+    tb2.push_back((int)nt2);
+  }
+  if (nt1 > INT32_MAX || nt2 > INT32_MAX || nq > (size_t)INT32_MAX) throw Error{ERR_SIZE, "too many Heff tiles"};
+  W.alloc(std::max<int64_t>(Wmax, 1));
+  o1.resize(no1);
+  t1.resize(nt1);
+  o2.resize(nq);
+  t2.resize(nt2);
+  // phase 1: one group per two-operand term
+  parallel_for(((int64_t)nq + 4095) / 4096, nthr, [&](int64_t c, int) {
+    const size_t qe = std::min(nq, (size_t)(c + 1) * 4096);
+    for (size_t q = (size_t)c * 4096; q < qe; q++) {
+      if (opi1[q] < 0) continue;
+      const Term& t = *chain[q];
+      const int dL = dLof(t.out), dR = dRof(t.out), dLi = dLof(t.in), dRi = dRof(t.in);
+      DevGemmOp op{};
+      if (!t.xr_first) {
+        op.A = resolve(plan->ref(t, t.Lb)); op.lda = (t.tl == 'N') ? dL : dLi; op.ta = (t.tl == 'T');
+        op.B = rel_x(t.in); op.ldb = dLi; op.tb = 0; op.k = dLi; op.flags = 2;
+      } else {
+        op.A = rel_x(t.in); op.lda = dLi; op.ta = 0;
+        op.B = resolve(plan->ref(t, t.Rb)); op.ldb = (t.tr == 'N') ? dRi : dR; op.tb = (t.tr == 'T'); op.k = dRi; op.flags = 1;
+      }
+      op.alpha = t.alpha;
+      op.kind = 0;
+      o1[opi1[q]] = op;
+      const WRec& w = wrec[q];
+      int64_t ti = tile1[q];
+      for (int m0 = 0; m0 < w.rows; m0 += 64)
+        for (int n0 = 0; n0 < w.cols; n0 += 64) {
+          DevTile tl{};
+          tl.C = W.p + w.off;
+          tl.ldc = w.rows; tl.m0 = m0; tl.n0 = n0; tl.m = w.rows; tl.n = w.cols; tl.op0 = opi1[q]; tl.nop = 1; tl.flags = TILE_NO_AXPY;
+          t1[ti++] = tl;
+        }
+    }
+  });
+  // phase 2: one group per output block, added to the explicit product
+  parallel_for((int64_t)bq.size() - 1, nthr, [&](int64_t bi, int) {
+    const size_t q0 = bq[bi], q1 = bq[bi + 1];
     const int k = chain[q0]->out;
-    size_t q1 = q0;
-    while (q1 < chain.size() && chain[q1]->out == k) q1++;
     const int dL = dLof(k), dR = dRof(k);
-    const int op0 = (int)o2.size();
+    bool axpy = false;
     for (size_t q = q0; q < q1; q++) {
       const Term& t = *chain[q];
       const int dLi = dLof(t.in), dRi = dRof(t.in);
       DevGemmOp op{};
       op.alpha = t.alpha;
       if (t.tl == 'I' && t.tr == 'I') {
+        axpy = true;
         op.kind = 1; op.A = rel_x(t.in); op.lda = dL; op.flags = 1;
       } else if (t.tr == 'I') {
         op.A = resolve(plan->ref(t, t.Lb)); op.lda = (t.tl == 'N') ? dL : dLi; op.ta = (t.tl == 'T');
@@ -833,19 +873,19 @@ void Ctx::prepare(int st) {
         op.A = resolve(plan->ref(t, t.Lb)); op.lda = (t.tl == 'N') ? dL : dLi; op.ta = (t.tl == 'T');
         op.B = W.p + w.off; op.ldb = w.rows; op.tb = 0; op.k = dLi;
       }
-      o2.push_back(op);
+      o2[q] = op;
     }
-    const int nop = (int)o2.size() - op0;
+    const int op0 = (int)q0, nop = (int)(q1 - q0);
+    int64_t ti = tile2[bi];
     for (int m0 = 0; m0 < dL; m0 += 64)
       for (int n0 = 0; n0 < dR; n0 += 64) {
         DevTile tl{};
         tl.C = (double*)(intptr_t)S.k2i[k];
-        tl.ldc = dL; tl.m0 = m0; tl.n0 = n0; tl.m = dL; tl.n = dR; tl.op0 = op0; tl.nop = nop; tl.flags = 3;
-        t2.push_back(tl);
+        tl.ldc = dL; tl.m0 = m0; tl.n0 = n0; tl.m = dL; tl.n = dR; tl.op0 = op0; tl.nop = nop;
+        tl.flags = 3 | (axpy ? 0 : TILE_NO_AXPY);
+        t2[ti++] = tl;
       }
-    q0 = q1;
-  }
-  tb2.push_back((int)t2.size());
+  });
   ops1.upload(o1, stream);
   ops2.upload(o2, stream);
   tiles1.upload(t1, stream);
@@ -950,8 +990,8 @@ void Ctx::apply_with(const double* xp, double* yp, double* bp, double* cp, doubl
                yp, stream);
   if (timing) hip_check(hipEventRecord(ev[1], stream), "event");
   for (size_t b = 0; b + 1 < tb1.size(); b++) {  // GEMM tiles, batch by batch (W is shared)
-    launch_tiles(tiles1.p + tb1[b], ops1.p, tb1[b + 1] - tb1[b], xp, yp, stream, use_tiles_mfma);
-    launch_tiles(tiles2.p + tb2[b], ops2.p, tb2[b + 1] - tb2[b], xp, yp, stream, use_tiles_mfma);
+    launch_tiles(tiles1.p + tb1[b], ops1.p, tb1[b + 1] - tb1[b], xp, yp, stream, tiles_kernel);
+    launch_tiles(tiles2.p + tb2[b], ops2.p, tb2[b + 1] - tb2[b], xp, yp, stream, tiles_kernel);
   }
   if (timing) hip_check(hipEventRecord(ev[2], stream), "event");
   launch_chain_mfma(cblocks_m.p, cblocks_m_offs, cgroups.p, cterms.p, xp, yp, cp, stream);
@@ -1245,7 +1285,7 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
           for (int n0 = 0; n0 < u.k2; n0 += 64) {
             DevTile tl{};
             tl.C = Wp + iw;
-            tl.ldc = u.m; tl.m0 = m0; tl.n0 = n0; tl.m = u.m; tl.n = u.k2; tl.op0 = opi; tl.nop = 1; tl.flags = 0;
+            tl.ldc = u.m; tl.m0 = m0; tl.n0 = n0; tl.m = u.m; tl.n = u.k2; tl.op0 = opi; tl.nop = 1; tl.flags = TILE_NO_AXPY;
             t1[it++] = tl;
           }
       }
@@ -1301,11 +1341,14 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
           if (o2[pos].kind == 0) pl = std::max({pl, m * o2[pos].k, o2[pos].k * nn});
       } else {
         int64_t it = t2start[q];
+        bool axpy = false;
+        for (int64_t pos = qstart[q]; pos < qstart[q + 1]; pos++) axpy |= o2[pos].kind == 1;
         for (int m0 = 0; m0 < m; m0 += 64)
           for (int n0 = 0; n0 < nn; n0 += 64) {
             DevTile tl{};
             tl.C = C; tl.ldc = m; tl.m0 = m0; tl.n0 = n0; tl.m = m; tl.n = nn;
-            tl.op0 = (int)qstart[q]; tl.nop = (int)(qstart[q + 1] - qstart[q]); tl.flags = 0;
+            tl.op0 = (int)qstart[q]; tl.nop = (int)(qstart[q + 1] - qstart[q]);
+            tl.flags = axpy ? 0 : TILE_NO_AXPY;
             t2[it++] = tl;
           }
       }
@@ -1329,9 +1372,9 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
   if (nred) hip_check(hipMemcpyAsync(u_rd.p, rd, nred * sizeof(DevPRed), hipMemcpyHostToDevice, stream), "reductions");
   const double t_up = now_s();
   const int P = std::min(OPS_P, (Pneed.load() + 31) / 32 * 32);
-  launch_tiles(u_dt1.p, u_d1.p, (int)nt1, nullptr, nullptr, stream, use_tiles_mfma);
+  launch_tiles(u_dt1.p, u_d1.p, (int)nt1, nullptr, nullptr, stream, tiles_kernel);
   launch_ops(u_it1.p, (int)ni1, u_d1.p, P, stream);
-  launch_tiles(u_dt2.p, u_d2.p, (int)nt2, nullptr, nullptr, stream, use_tiles_mfma);
+  launch_tiles(u_dt2.p, u_d2.p, (int)nt2, nullptr, nullptr, stream, tiles_kernel);
   launch_ops(u_it2.p, (int)ni2, u_d2.p, P, stream);
   launch_part_reduce(u_rd.p, (int)nred, u_part.p, stream);
   hip_check(hipGetLastError(), "update kernels");
@@ -2105,7 +2148,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_SVD_JACOBI_MAXN")) ctx->c.svd_jacobi_maxn = std::atoi(e);
   if (const char* e = std::getenv("CHM_CHAIN_KERNEL")) ctx->c.use_chain_kernel = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_CHAIN_MFMA")) ctx->c.use_chain_mfma = std::atoi(e) != 0;
-  if (const char* e = std::getenv("CHM_TILES_MFMA")) ctx->c.use_tiles_mfma = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CHM_TILES_MFMA")) ctx->c.tiles_kernel = std::max(0, std::min(2, std::atoi(e)));
   if (const char* e = std::getenv("CHM_CHAIN_ITEM_TERMS")) ctx->c.chain_item_terms = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("CHM_CHAIN_GROUP_TERMS")) ctx->c.chain_group_terms = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("CHM_SVD_GESVDJ")) ctx->c.svd_large_gesvdj = std::atoi(e) != 0;

@@ -237,7 +237,7 @@ struct Ctx : HostState {
   bool svd_large_gesvdj = false;     // CHM_SVD_GESVDJ=1: large Split sectors by rocsolver_dgesvdj instead of dgesvd
   int chain_item_terms = CHAIN_ITEM_TERMS;    // CHM_CHAIN_ITEM_TERMS
   int chain_group_terms = CHAIN_GROUP_TERMS;  // CHM_CHAIN_GROUP_TERMS
-  bool use_tiles_mfma = true;        // CHM_TILES_MFMA=0: GEMM tiles in k_tiles (vector FMA) instead of k_tiles_mfma
+  int tiles_kernel = 2;  // CHM_TILES_MFMA: GEMM tiles in 2 k_tiles_wave, 1 k_tiles_mfma (LDS staging), 0 k_tiles (vector FMA)
   int cblocks_m_offs[CHAIN_MFMA_NT + 1] = {};
   DBuf<DevCBlock> cblocks;
   DBuf<DevCGroup> cgroups;

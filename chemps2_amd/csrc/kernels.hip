@@ -240,6 +240,154 @@ __global__ __launch_bounds__(NT) void k_tiles_mfma(const DevTile* __restrict__ t
       }
 }
 
+// The same tiles without the padding (k_tiles_fit): only the 16 x 16 sub-tiles that hold
+// elements of the block are computed, ms x ns of them (ms, ns <= 4), dealt out to the
+// four wavefronts in turn (wavefront w takes sub-tiles w, w + 4, ...; C = ceil(ms ns / 4)
+// per wavefront, a template argument so that every accumulator has a fixed register). The
+// staging is k_tiles_mfma's: two LDS buffers, the next chunk's operands loaded into
+// registers while the current one computes, one barrier per chunk. Tiles whose flags
+// carry TILE_NO_AXPY skip the search for axpy products; the others find them 64 records
+// at a time with a ballot.
+template <int C>
+__device__ __forceinline__ void tiles_fit_body(const DevTile& t, const DevGemmOp* __restrict__ ops,
+                                               const double* __restrict__ xbase, double* __restrict__ ybase,
+                                               double (*As)[TK][TM + 1], double (*Bs)[TK][TN + 1], int ms, int nsub) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, lr = lane & 15, lg = lane >> 4;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int mrem = min(TM, t.m - t.m0), nrem = min(TN, t.n - t.n0);
+  // this wavefront's sub-tiles (q < C). A slot past nsub (only the last can be) repeats
+  // a sub-tile of the block and is not stored, so that the MFMA sequence has no branch
+  // (a conditional MFMA makes the compiler move the accumulators around it).
+  int si[C], sj[C];
+  bool on[C];
+#pragma unroll
+  for (int q = 0; q < C; q++) {
+    const int sq = wv + 4 * q;
+    on[q] = sq < nsub;
+    const int se = on[q] ? sq : 0;
+    si[q] = se % ms;
+    sj[q] = se / ms;
+  }
+  int offa[C], offb[C];
+#pragma unroll
+  for (int q = 0; q < C; q++) {
+    offa[q] = 16 * si[q] + lr;
+    offb[q] = 16 * sj[q] + lr;
+  }
+  chm_t4 acc[C];
+#pragma unroll
+  for (int q = 0; q < C; q++) acc[q] = chm_t4{0.0, 0.0, 0.0, 0.0};
+  // axpy products: C += alpha * op(A) on this lane's accumulator elements
+  if (!(t.flags & TILE_NO_AXPY)) {
+    for (int base = 0; base < t.nop; base += 64) {
+      const int oi = base + lane;
+      const int kind = oi < t.nop ? ops[t.op0 + oi].kind : 0;
+      uint64_t msk = __ballot(kind == 1);
+      while (msk) {
+        const int qb = __ffsll((unsigned long long)msk) - 1;
+        msk &= msk - 1;
+        const DevGemmOp op = ops[t.op0 + base + qb];
+        const double* A = rel(op.A, op.flags & 1, xbase);
+#pragma unroll
+        for (int q = 0; q < C; q++) {
+          if (!on[q]) continue;
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const int rr = 16 * si[q] + 4 * r + lg, cc = 16 * sj[q] + lr;
+            if (rr < mrem && cc < nrem)
+              acc[q][r] += op.alpha * (op.ta ? A[(size_t)(t.n0 + cc) + (size_t)op.lda * (t.m0 + rr)]
+                                             : A[(size_t)(t.m0 + rr) + (size_t)op.lda * (t.n0 + cc)]);
+          }
+        }
+      }
+    }
+  }
+  int o = 0, k0 = 0;
+  auto skip = [&]() {  // advance o to the next GEMM product with work left
+    while (o < t.nop && (ops[t.op0 + o].kind == 1 || k0 >= ops[t.op0 + o].k)) { o++; k0 = 0; }
+  };
+  skip();
+  double ra[4], rb[4];
+  auto fetch = [&]() {  // this thread's share of chunk (o, k0), alpha folded into A
+    const DevGemmOp op = ops[t.op0 + o];
+    const double* A = rel(op.A, op.flags & 1, xbase);
+    const double* B = rel(op.B, (op.flags >> 1) & 1, xbase);
+    const int krem = min(TK, op.k - k0);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int r, kk;
+      if (op.ta == 0) { r = tid & 63; kk = (tid >> 6) + 4 * q; } else { kk = tid & 15; r = (tid >> 4) + 16 * q; }
+      double v = 0.0;
+      if (r < mrem && kk < krem)
+        v = op.ta ? A[(size_t)(k0 + kk) + (size_t)op.lda * (t.m0 + r)] : A[(size_t)(t.m0 + r) + (size_t)op.lda * (k0 + kk)];
+      ra[q] = op.alpha * v;
+      int c, kb;
+      if (op.tb == 0) { kb = tid & 15; c = (tid >> 4) + 16 * q; } else { c = tid & 63; kb = (tid >> 6) + 4 * q; }
+      double w = 0.0;
+      if (c < nrem && kb < krem)
+        w = op.tb ? B[(size_t)(t.n0 + c) + (size_t)op.ldb * (k0 + kb)] : B[(size_t)(k0 + kb) + (size_t)op.ldb * (t.n0 + c)];
+      rb[q] = w;
+    }
+  };
+  auto put = [&](int buf, int ta, int tb) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (ta == 0) As[buf][(tid >> 6) + 4 * q][tid & 63] = ra[q];
+      else As[buf][tid & 15][(tid >> 4) + 16 * q] = ra[q];
+      if (tb == 0) Bs[buf][tid & 15][(tid >> 4) + 16 * q] = rb[q];
+      else Bs[buf][(tid >> 6) + 4 * q][tid & 63] = rb[q];
+    }
+  };
+  int buf = 0;
+  if (o < t.nop) fetch();
+  while (o < t.nop) {
+    const DevGemmOp op = ops[t.op0 + o];
+    put(buf, op.ta, op.tb);
+    __syncthreads();
+    k0 += TK;
+    skip();
+    if (o < t.nop) fetch();  // in flight while this chunk computes
+#pragma unroll
+    for (int ks = 0; ks < TK / 4; ks++) {  // k past the product's end is zero in LDS
+      const int kk = 4 * ks + lg;
+#pragma unroll
+      for (int q = 0; q < C; q++)
+        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[buf][kk][offa[q]], Bs[buf][kk][offb[q]], acc[q], 0, 0, 0);
+    }
+    buf ^= 1;
+  }
+  double* Cp = (t.flags & 1) ? ybase + (intptr_t)t.C : t.C;
+  const bool add = (t.flags & 2) != 0;
+#pragma unroll
+  for (int q = 0; q < C; q++) {
+    if (!on[q]) continue;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int rr = 16 * si[q] + 4 * r + lg, cc = 16 * sj[q] + lr;
+      if (rr < mrem && cc < nrem) {
+        double* p = Cp + (size_t)(t.m0 + rr) + (size_t)t.ldc * (t.n0 + cc);
+        *p = add ? *p + acc[q][r] : acc[q][r];
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_tiles_fit(const DevTile* __restrict__ tiles, const DevGemmOp* __restrict__ ops,
+                                                  const double* __restrict__ xbase, double* __restrict__ ybase) {
+  __shared__ double As[2][TK][TM + 1];
+  __shared__ double Bs[2][TK][TN + 1];
+  const DevTile t = tiles[blockIdx.x];
+  const int mrem = min(TM, t.m - t.m0), nrem = min(TN, t.n - t.n0);
+  const int ms = (mrem + 15) >> 4, ns = (nrem + 15) >> 4, nsub = ms * ns;
+  switch ((nsub + 3) >> 2) {
+    case 1: tiles_fit_body<1>(t, ops, xbase, ybase, As, Bs, ms, nsub); break;
+    case 2: tiles_fit_body<2>(t, ops, xbase, ybase, As, Bs, ms, nsub); break;
+    case 3: tiles_fit_body<3>(t, ops, xbase, ybase, As, Bs, ms, nsub); break;
+    default: tiles_fit_body<4>(t, ops, xbase, ybase, As, Bs, ms, nsub); break;
+  }
+}
+
 // Assembles the explicit sub-block of every (kappa, kappa') pair:
 // M[(l + dL r), (l' + dLi r')] = sum_t alpha_t op(A_t)[l, l'] op(B_t)[r', r].
 // SPLIT stores each value v as two floats, hi = fl32(v) in the first nnz floats of the
@@ -1704,9 +1852,10 @@ inline unsigned grid_for(int64_t n) {
 }  // namespace
 
 void launch_tiles(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const double* xbase, double* ybase,
-                  hipStream_t s, bool mfma) {
+                  hipStream_t s, int mode) {
   if (ntiles <= 0) return;
-  if (mfma) hipLaunchKernelGGL(k_tiles_mfma, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase);
+  if (mode == 2) hipLaunchKernelGGL(k_tiles_fit, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase);
+  else if (mode == 1) hipLaunchKernelGGL(k_tiles_mfma, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase);
   else hipLaunchKernelGGL(k_tiles, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase);
 }
 
