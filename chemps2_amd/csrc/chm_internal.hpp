@@ -234,6 +234,7 @@ struct DevTile {  // one 64 x 64 output tile and the products that accumulate in
   int flags;  // bit 0: C relative to y; bit 1: add to C instead of overwriting it; TILE_NO_AXPY
 };
 constexpr int TILE_NO_AXPY = 4;  // no product of the tile is an axpy (kind 1)
+constexpr int TILE_C_PART = 8;   // C relative to the partial-sum buffer
 // Small-block products of the operator updates: one workgroup per work item computes
 // out (m x n, contiguous) = sum of its products (DevGemmOp, absolute pointers, kind 0:
 // alpha op(A) op(B); kind 1: alpha op(A)) in 2 x 2 register tiles with the operands
@@ -388,7 +389,7 @@ void launch_add_noise(double* x, int64_t n, uint64_t seed, double level, hipStre
 // mode 2: k_tiles_fit (matrix cores, only the 16 x 16 sub-tiles and k-steps that hold
 // elements), 1: k_tiles_mfma (matrix cores, whole 64 x 64 tiles), 0: k_tiles (vector FMA)
 void launch_tiles(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const double* xbase, double* ybase,
-                  hipStream_t s, int mode = 2);
+                  hipStream_t s, int mode = 2, double* pbase = nullptr);
 void launch_combos(const DevCombo* d_combos, const DevSrc* d_src, int ncombo, const int64_t* d_chunk_start,
                    int64_t nchunks, hipStream_t s);
 void launch_diag(const DevDiagBlock* blocks, const DevDiag* terms, int nblocks, hipStream_t s);

@@ -786,28 +786,70 @@ void Ctx::prepare(int st) {
     }
     tb1.push_back((int)nt1);
   }
+  // Phase 2: an output block whose products are long (in 16-deep k chunks per tile) is
+  // split into segments of consecutive products, so that the launch has many more
+  // workgroups than the device holds at once; each segment writes its partial block into
+  // the partial-sum buffer and k_chain_reduce adds them to y.
+  auto chunks_of = [&](const Term& t) -> int64_t {
+    if (t.tl == 'I' && t.tr == 'I') return 0;
+    const int k = (t.tr == 'I' || (t.tl != 'I' && t.xr_first)) ? dLof(t.in) : dRof(t.in);
+    return (k + 15) / 16;
+  };
   std::vector<size_t> bq;  // first term of every output block
-  std::vector<int64_t> tile2;
-  int64_t nt2 = 0;
+  std::vector<int64_t> bwork;  // 16-deep k chunks per tile of every output block
   {
     size_t nextb = 1;
     for (size_t q0 = 0; q0 < nq;) {
-      if (nextb < bstart.size() && q0 == bstart[nextb]) {
-        tb2.push_back((int)nt2);
-        nextb++;
-      }
+      if (nextb < bstart.size() && q0 == bstart[nextb]) nextb++;
       const int k = chain[q0]->out;
       size_t q1 = q0;
-      while (q1 < nq && chain[q1]->out == k) q1++;
+      int64_t w = 0;
+      while (q1 < nq && chain[q1]->out == k) w += chunks_of(*chain[q1++]);
       bq.push_back(q0);
-      tile2.push_back(nt2);
-      nt2 += (int64_t)((dLof(k) + 63) / 64) * ((dRof(k) + 63) / 64);
+      bwork.push_back(w);
       q0 = q1;
     }
     bq.push_back(nq);
+  }
+  const int64_t nblk = (int64_t)bwork.size();
+  std::vector<int> bseg(nblk, 1);
+  {
+    int64_t wtot = 0;
+    for (int64_t bi = 0; bi < nblk; bi++) {
+      const int k = chain[bq[bi]]->out;
+      wtot += bwork[bi] * ((dLof(k) + 63) / 64) * ((dRof(k) + 63) / 64);
+    }
+    const int64_t wseg = std::max<int64_t>(64, wtot / std::max(1, tile_segments));
+    for (int64_t bi = 0; bi < nblk; bi++) bseg[bi] = (int)std::max<int64_t>(1, std::min<int64_t>(bwork[bi] / wseg, 64));
+  }
+  std::vector<int64_t> tile2(nblk), bpart(nblk, -1);
+  int64_t nt2 = 0, tpart_n = 0;
+  std::vector<DevBRed> tred;
+  {
+    size_t nextb = 1;
+    for (int64_t bi = 0; bi < nblk; bi++) {
+      if (nextb < bstart.size() && bq[bi] == bstart[nextb]) {
+        tb2.push_back((int)nt2);
+        nextb++;
+      }
+      const int k = chain[bq[bi]]->out;
+      const int64_t nel = (int64_t)dLof(k) * dRof(k);
+      tile2[bi] = nt2;
+      nt2 += (int64_t)((dLof(k) + 63) / 64) * ((dRof(k) + 63) / 64) * bseg[bi];
+      if (bseg[bi] > 1) {
+        bpart[bi] = tpart_n;
+        tred.push_back({S.k2i[k], cpart_need + tpart_n, (int)nel, bseg[bi]});
+        tpart_n += nel * bseg[bi];
+      }
+    }
     tb2.push_back((int)nt2);
   }
   if (nt1 > INT32_MAX || nt2 > INT32_MAX || nq > (size_t)INT32_MAX) throw Error{ERR_SIZE, "too many Heff tiles"};
+  tpart_off = cpart_need;
+  cpart_need += tpart_n;
+  cpart.alloc(std::max<int64_t>(cpart_need, 1));
+  treds.upload(tred, stream);
+  ntreds = (int)tred.size();
   W.alloc(std::max<int64_t>(Wmax, 1));
   o1.resize(no1);
   t1.resize(nt1);
@@ -842,19 +884,18 @@ void Ctx::prepare(int st) {
         }
     }
   });
-  // phase 2: one group per output block, added to the explicit product
-  parallel_for((int64_t)bq.size() - 1, nthr, [&](int64_t bi, int) {
+  // phase 2: one group per output block (or per segment of it), added to the explicit
+  // product
+  parallel_for(nblk, nthr, [&](int64_t bi, int) {
     const size_t q0 = bq[bi], q1 = bq[bi + 1];
     const int k = chain[q0]->out;
     const int dL = dLof(k), dR = dRof(k);
-    bool axpy = false;
     for (size_t q = q0; q < q1; q++) {
       const Term& t = *chain[q];
       const int dLi = dLof(t.in), dRi = dRof(t.in);
       DevGemmOp op{};
       op.alpha = t.alpha;
       if (t.tl == 'I' && t.tr == 'I') {
-        axpy = true;
         op.kind = 1; op.A = rel_x(t.in); op.lda = dL; op.flags = 1;
       } else if (t.tr == 'I') {
         op.A = resolve(plan->ref(t, t.Lb)); op.lda = (t.tl == 'N') ? dL : dLi; op.ta = (t.tl == 'T');
@@ -875,16 +916,35 @@ void Ctx::prepare(int st) {
       }
       o2[q] = op;
     }
-    const int op0 = (int)q0, nop = (int)(q1 - q0);
+    // segments of about equal chunk counts
+    const int ns = bseg[bi];
     int64_t ti = tile2[bi];
-    for (int m0 = 0; m0 < dL; m0 += 64)
-      for (int n0 = 0; n0 < dR; n0 += 64) {
-        DevTile tl{};
-        tl.C = (double*)(intptr_t)S.k2i[k];
-        tl.ldc = dL; tl.m0 = m0; tl.n0 = n0; tl.m = dL; tl.n = dR; tl.op0 = op0; tl.nop = nop;
-        tl.flags = 3 | (axpy ? 0 : TILE_NO_AXPY);
-        t2[ti++] = tl;
-      }
+    size_t qa = q0;
+    int64_t done = 0;
+    for (int sg = 0; sg < ns; sg++) {
+      size_t qb = qa;
+      const int64_t goal = bwork[bi] * (sg + 1) / ns;
+      if (sg == ns - 1) qb = q1;
+      else
+        while (qb < q1 && done < goal) done += chunks_of(*chain[qb++]);
+      bool axpy = false;
+      for (size_t q = qa; q < qb; q++) axpy |= o2[q].kind == 1;
+      for (int m0 = 0; m0 < dL; m0 += 64)
+        for (int n0 = 0; n0 < dR; n0 += 64) {
+          DevTile tl{};
+          if (ns == 1) {
+            tl.C = (double*)(intptr_t)S.k2i[k];
+            tl.flags = 3;
+          } else {
+            tl.C = (double*)(intptr_t)(tpart_off + bpart[bi] + (int64_t)sg * dL * dR);
+            tl.flags = TILE_C_PART;
+          }
+          if (!axpy) tl.flags |= TILE_NO_AXPY;
+          tl.ldc = dL; tl.m0 = m0; tl.n0 = n0; tl.m = dL; tl.n = dR; tl.op0 = (int)qa; tl.nop = (int)(qb - qa);
+          t2[ti++] = tl;
+        }
+      qa = qb;
+    }
   });
   ops1.upload(o1, stream);
   ops2.upload(o2, stream);
@@ -991,8 +1051,9 @@ void Ctx::apply_with(const double* xp, double* yp, double* bp, double* cp, doubl
   if (timing) hip_check(hipEventRecord(ev[1], stream), "event");
   for (size_t b = 0; b + 1 < tb1.size(); b++) {  // GEMM tiles, batch by batch (W is shared)
     launch_tiles(tiles1.p + tb1[b], ops1.p, tb1[b + 1] - tb1[b], xp, yp, stream, tiles_kernel);
-    launch_tiles(tiles2.p + tb2[b], ops2.p, tb2[b + 1] - tb2[b], xp, yp, stream, tiles_kernel);
+    launch_tiles(tiles2.p + tb2[b], ops2.p, tb2[b + 1] - tb2[b], xp, yp, stream, tiles_kernel, cp);
   }
+  launch_chain_reduce(treds.p, ntreds, cp, yp, stream);  // segments of split output blocks
   if (timing) hip_check(hipEventRecord(ev[2], stream), "event");
   launch_chain_mfma(cblocks_m.p, cblocks_m_offs, cgroups.p, cterms.p, xp, yp, cp, stream);
   launch_chain(cblocks.p, ncblocks, cgroups.p, cterms.p, xp, yp, cp, chain_P, false, stream);
@@ -2149,6 +2210,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_CHAIN_KERNEL")) ctx->c.use_chain_kernel = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_CHAIN_MFMA")) ctx->c.use_chain_mfma = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_TILES_MFMA")) ctx->c.tiles_kernel = std::max(0, std::min(2, std::atoi(e)));
+  if (const char* e = std::getenv("CHM_TILE_SEGMENTS")) ctx->c.tile_segments = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("CHM_CHAIN_ITEM_TERMS")) ctx->c.chain_item_terms = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("CHM_CHAIN_GROUP_TERMS")) ctx->c.chain_group_terms = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("CHM_SVD_GESVDJ")) ctx->c.svd_large_gesvdj = std::atoi(e) != 0;

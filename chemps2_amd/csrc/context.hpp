@@ -144,6 +144,10 @@ struct Ctx : HostState {
   DBuf<double> combo_arena, W, dx, dy, partial, scal;
   DBuf<DevTile> tiles1, tiles2;
   std::vector<int> tb1, tb2;  // tile ranges of the batches that share the W scratch
+  DBuf<DevBRed> treds;        // phase-2 output blocks split into segments: their reductions
+  int ntreds = 0;
+  int64_t tpart_off = 0;      // first partial sum of those segments in the chained partials
+  int tile_segments = 8192;   // CHM_TILE_SEGMENTS: phase-2 workgroups aimed at per launch
   int64_t tile_w_cap = 0;  // doubles of W scratch per batch (CHM_TILE_W_CAP; 0: a quarter of free memory)
   DBuf<DevGemmOp> ops1, ops2;
   int ntiles1 = 0, ntiles2 = 0;

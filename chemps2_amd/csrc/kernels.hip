@@ -21,8 +21,15 @@ __device__ inline const double* rel(const double* p, int is_rel, const double* b
   return is_rel ? base + (intptr_t)p : p;
 }
 
+// the output of tile t: relative to y (flags bit 0), to the partial sums (TILE_C_PART) or
+// absolute
+__device__ inline double* tile_out(const DevTile& t, double* ybase, double* pbase) {
+  return (t.flags & 1) ? ybase + (intptr_t)t.C : ((t.flags & TILE_C_PART) ? pbase + (intptr_t)t.C : t.C);
+}
+
 __global__ __launch_bounds__(NT) void k_tiles(const DevTile* __restrict__ tiles, const DevGemmOp* __restrict__ ops,
-                                              const double* __restrict__ xbase, double* __restrict__ ybase) {
+                                              const double* __restrict__ xbase, double* __restrict__ ybase,
+                                              double* __restrict__ pbase) {
   __shared__ double As[TK][TM + 1];
   __shared__ double Bs[TK][TN + 1];
   const DevTile t = tiles[blockIdx.x];
@@ -109,7 +116,7 @@ __global__ __launch_bounds__(NT) void k_tiles(const DevTile* __restrict__ tiles,
       __syncthreads();
     }
   }
-  double* C = (t.flags & 1) ? ybase + (intptr_t)t.C : t.C;
+  double* C = tile_out(t, ybase, pbase);
   const bool add = (t.flags & 2) != 0;
 #pragma unroll
   for (int i = 0; i < 4; i++)
@@ -132,7 +139,8 @@ __global__ __launch_bounds__(NT) void k_tiles(const DevTile* __restrict__ tiles,
 typedef double chm_t4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(NT) void k_tiles_mfma(const DevTile* __restrict__ tiles, const DevGemmOp* __restrict__ ops,
-                                                   const double* __restrict__ xbase, double* __restrict__ ybase) {
+                                                   const double* __restrict__ xbase, double* __restrict__ ybase,
+                                                   double* __restrict__ pbase) {
   __shared__ double As[2][TK][TM + 1];
   __shared__ double Bs[2][TK][TN + 1];
   const DevTile t = tiles[blockIdx.x];
@@ -224,7 +232,7 @@ __global__ __launch_bounds__(NT) void k_tiles_mfma(const DevTile* __restrict__ t
     }
     buf ^= 1;
   }
-  double* C = (t.flags & 1) ? ybase + (intptr_t)t.C : t.C;
+  double* C = tile_out(t, ybase, pbase);
   const bool add = (t.flags & 2) != 0;
 #pragma unroll
   for (int i = 0; i < 2; i++)
@@ -251,6 +259,7 @@ __global__ __launch_bounds__(NT) void k_tiles_mfma(const DevTile* __restrict__ t
 template <int C>
 __device__ __forceinline__ void tiles_fit_body(const DevTile& t, const DevGemmOp* __restrict__ ops,
                                                const double* __restrict__ xbase, double* __restrict__ ybase,
+                                               double* __restrict__ pbase,
                                                double (*As)[TK][TM + 1], double (*Bs)[TK][TN + 1], int ms, int nsub) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, lr = lane & 15, lg = lane >> 4;
@@ -357,7 +366,7 @@ __device__ __forceinline__ void tiles_fit_body(const DevTile& t, const DevGemmOp
     }
     buf ^= 1;
   }
-  double* Cp = (t.flags & 1) ? ybase + (intptr_t)t.C : t.C;
+  double* Cp = tile_out(t, ybase, pbase);
   const bool add = (t.flags & 2) != 0;
 #pragma unroll
   for (int q = 0; q < C; q++) {
@@ -374,17 +383,18 @@ __device__ __forceinline__ void tiles_fit_body(const DevTile& t, const DevGemmOp
 }
 
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_tiles_fit(const DevTile* __restrict__ tiles, const DevGemmOp* __restrict__ ops,
-                                                  const double* __restrict__ xbase, double* __restrict__ ybase) {
+                                                  const double* __restrict__ xbase, double* __restrict__ ybase,
+                                                  double* __restrict__ pbase) {
   __shared__ double As[2][TK][TM + 1];
   __shared__ double Bs[2][TK][TN + 1];
   const DevTile t = tiles[blockIdx.x];
   const int mrem = min(TM, t.m - t.m0), nrem = min(TN, t.n - t.n0);
   const int ms = (mrem + 15) >> 4, ns = (nrem + 15) >> 4, nsub = ms * ns;
   switch ((nsub + 3) >> 2) {
-    case 1: tiles_fit_body<1>(t, ops, xbase, ybase, As, Bs, ms, nsub); break;
-    case 2: tiles_fit_body<2>(t, ops, xbase, ybase, As, Bs, ms, nsub); break;
-    case 3: tiles_fit_body<3>(t, ops, xbase, ybase, As, Bs, ms, nsub); break;
-    default: tiles_fit_body<4>(t, ops, xbase, ybase, As, Bs, ms, nsub); break;
+    case 1: tiles_fit_body<1>(t, ops, xbase, ybase, pbase, As, Bs, ms, nsub); break;
+    case 2: tiles_fit_body<2>(t, ops, xbase, ybase, pbase, As, Bs, ms, nsub); break;
+    case 3: tiles_fit_body<3>(t, ops, xbase, ybase, pbase, As, Bs, ms, nsub); break;
+    default: tiles_fit_body<4>(t, ops, xbase, ybase, pbase, As, Bs, ms, nsub); break;
   }
 }
 
@@ -1852,11 +1862,11 @@ inline unsigned grid_for(int64_t n) {
 }  // namespace
 
 void launch_tiles(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const double* xbase, double* ybase,
-                  hipStream_t s, int mode) {
+                  hipStream_t s, int mode, double* pbase) {
   if (ntiles <= 0) return;
-  if (mode == 2) hipLaunchKernelGGL(k_tiles_fit, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase);
-  else if (mode == 1) hipLaunchKernelGGL(k_tiles_mfma, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase);
-  else hipLaunchKernelGGL(k_tiles, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase);
+  if (mode == 2) hipLaunchKernelGGL(k_tiles_fit, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase, pbase);
+  else if (mode == 1) hipLaunchKernelGGL(k_tiles_mfma, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase, pbase);
+  else hipLaunchKernelGGL(k_tiles, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase, pbase);
 }
 
 void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs, double* M, int64_t nnz, bool split,
