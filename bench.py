@@ -61,6 +61,48 @@ def read_traffic(name):
         return None, None
 
 
+def traffic_from(fetch_csv, write_csv, steps, out_name):
+    """HBM bytes per Heff application from two rocprofv3 --pmc runs of --site-from (one
+    with FETCH_SIZE, one with WRITE_SIZE, in KiB per dispatch): the counted products are
+    the dispatches between the two 1 GiB copy marks, and each counter is scaled by what it
+    reads for the mark itself (1 GiB read, 1 GiB written)."""
+    import csv
+
+    def per_dispatch(path, counter):
+        rows = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r.get("Counter_Name") != counter:
+                    continue
+                d = int(r["Dispatch_Id"])
+                rows[d] = (r["Kernel_Name"], float(r["Counter_Value"]))
+        return [rows[d] for d in sorted(rows)]
+
+    def window(seq):
+        marks = [i for i, (n, _) in enumerate(seq) if "copyBuffer" in n or "elementwise" in n]
+        big = sorted(marks, key=lambda i: -seq[i][1])[:2]
+        i0, i1 = sorted(big)
+        return seq[i0][1], [v for _, v in seq[i0 + 1:i1]], seq[i1][1]
+
+    f0, fk, f1 = window(per_dispatch(fetch_csv, "FETCH_SIZE"))
+    w0, wk, w1 = window(per_dispatch(write_csv, "WRITE_SIZE"))
+    gib_kib = COPY_MARK_DOUBLES * 8 / 1024.0
+    fscale = gib_kib / (0.5 * (f0 + f1))
+    wscale = gib_kib / (0.5 * (w0 + w1))
+    rd = sum(fk) * fscale * 1024.0 / steps
+    wr = sum(wk) * wscale * 1024.0 / steps
+    out = {
+        "kernel": "every kernel of one Heff application (k_bspmv, k_tiles_fit, k_fchain, k_chain and their reductions)",
+        "workload": "Heff x at sites 15-16 of the warmed-up L=32 D=2000 random D2h state",
+        "bytes_per_launch": rd + wr, "read_bytes": rd, "write_bytes": wr,
+        "fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4),
+        "dispatches_per_application": len(fk) / steps, "applications": steps,
+    }
+    with open(os.path.join(ROOT, "profiles", out_name), "w") as f:
+        json.dump(out, f, indent=1)
+    return out
+
+
 def new_ctx(L, irr, mx, D, form, params, ckpt):
     import chemps2_amd
 
@@ -95,9 +137,26 @@ def warm_up(L, D, seed, irr, mx, rtol, ckpt):
     return {"seconds": round(time.time() - t0, 2), "schedule": out, "total_dim_per_boundary": dims}
 
 
-def site_bench(L, D, irr, mx, ckpt, site, steps, warmup):
+COPY_MARK_DOUBLES = 1 << 27  # 1 GiB: the plain copy that brackets the counted products
+
+
+def copy_mark():
+    """A plain device copy of 1 GiB (torch, hipMemcpy device to device): the counter
+    profiles use it to scale FETCH_SIZE and WRITE_SIZE and to find the counted products."""
+    import torch
+
+    a = torch.empty(COPY_MARK_DOUBLES, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def site_bench(L, D, irr, mx, ckpt, site, steps, warmup, marks=False):
     """Heff.x at sites (site, site+1) of the warmed-up state: the left half sweep down to
-    site+1 (fixed dimensions) builds the right operators, then steps timed applications."""
+    site+1 (fixed dimensions) builds the right operators, then steps timed applications.
+    marks: a 1 GiB copy before and after the timed applications (counter profiles)."""
     import numpy as np
     import torch
 
@@ -119,12 +178,16 @@ def site_bench(L, D, irr, mx, ckpt, site, steps, warmup):
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
+    if marks:
+        copy_mark()
     e0.record(stream)
     for _ in range(steps):
         c.apply(x, y)
     e1.record(stream)
     e1.synchronize()
     ms = e0.elapsed_time(e1) / steps
+    if marks:
+        copy_mark()
     c.timing(True)
     for _ in range(steps):
         c.apply(x, y)
@@ -249,7 +312,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--save-warm", metavar="H5", help="only the warm-up: save its state to H5 and exit")
+    ap.add_argument("--site-from", metavar="H5", help="only the site benchmark, from a state --save-warm wrote, "
+                    "its timed applications between two 1 GiB copy marks (for rocprofv3 --pmc)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/" + "heff_site15_L32_D2000_traffic.json from the counter_collection.csv "
+                    "of two --site-from runs under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (--steps as in those runs)")
     args = ap.parse_args()
+    if args.traffic_from:
+        print(json.dumps(traffic_from(args.traffic_from[0], args.traffic_from[1], args.steps,
+                                      "heff_site15_L32_D2000_traffic.json")))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1).",
@@ -270,11 +343,18 @@ def main():
     L, D, seed = args.L, args.D, args.seed
     irr, T, V = random_abinitio(L, seed, group=7)
     mx = mx_table(T, V, L)
-    tmp = tempfile.mkdtemp(prefix="chm_bench_")
-    ckpt = os.path.join(tmp, "warm.h5")
-    warm = warm_up(L, D, seed, irr, mx, args.rtol, ckpt)
-
     site = L // 2 - 1
+    if args.site_from:
+        sb, roof = site_bench(L, D, irr, mx, args.site_from, site, args.steps, args.warmup, marks=True)
+        print(json.dumps({"site_bench": sb, "roofline": roof}), flush=True)
+        return
+    tmp = tempfile.mkdtemp(prefix="chm_bench_")
+    ckpt = args.save_warm or os.path.join(tmp, "warm.h5")
+    warm = warm_up(L, D, seed, irr, mx, args.rtol, ckpt)
+    if args.save_warm:
+        print(json.dumps({"warm_up": warm, "checkpoint": ckpt}), flush=True)
+        return
+
     sb, roof = site_bench(L, D, irr, mx, ckpt, site, args.steps, args.warmup)
 
     wpath = os.path.join(ROOT, "chemps2_amd", "data", "precond_mlp.npy")

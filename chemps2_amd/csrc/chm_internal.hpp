@@ -233,6 +233,16 @@ struct DevTile {  // one 64 x 64 output tile and the products that accumulate in
   int op0, nop;
   int flags;  // bit 0: C relative to y; bit 1: add to C instead of overwriting it; TILE_NO_AXPY
 };
+struct DevTile1 {  // a tile of one GEMM product (k > 0, no axpy) with the product itself
+  double* C;
+  const double* A;
+  const double* B;
+  double alpha;
+  int ldc, m0, n0, mrem, nrem;  // mrem, nrem: rows and columns of the tile that hold elements
+  int lda, ldb, k, ta, tb;
+  int flags;  // bit 0: A relative to x, bit 1: B relative to x
+  int pad;
+};
 constexpr int TILE_NO_AXPY = 4;  // no product of the tile is an axpy (kind 1)
 constexpr int TILE_C_PART = 8;   // C relative to the partial-sum buffer
 // Small-block products of the operator updates: one workgroup per work item computes
@@ -390,6 +400,23 @@ void launch_add_noise(double* x, int64_t n, uint64_t seed, double level, hipStre
 // elements), 1: k_tiles_mfma (matrix cores, whole 64 x 64 tiles), 0: k_tiles (vector FMA)
 void launch_tiles(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const double* xbase, double* ybase,
                   hipStream_t s, int mode = 2, double* pbase = nullptr);
+// Tiles of one GEMM product each, overwriting their outputs, all of one bucket
+// (tile_bucket): c = 0, at most 32 x 32, one wavefront per tile (k_tiles_wave1); c = 1..4,
+// c sub-tiles per wavefront of a workgroup per tile (k_tiles_stream1). The workgroups
+// (TILE_STREAM_GROUPS, or TILE_WAVE_GROUPS of four wavefronts for c = 0: what the device
+// holds at once) walk the list.
+// Grouped tiles of at most 32 x 32 whose products are all GEMMs with k > 0: one
+// wavefront per tile (k_tiles_wave2).
+void launch_tiles_wave2(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const double* xbase, double* ybase,
+                        hipStream_t s, double* pbase = nullptr);
+constexpr int TILE_STREAM_GROUPS = 1024;
+constexpr int TILE_WAVE_GROUPS = 768;
+constexpr int TILE_BUCKETS = 5;
+inline int tile_bucket(int mrem, int nrem) {
+  if (mrem <= 32 && nrem <= 32) return 0;
+  return ((mrem + 15) / 16 * ((nrem + 15) / 16) + 3) / 4;
+}
+void launch_tiles_stream1(const DevTile1* tiles, int ntiles, int c, const double* xbase, hipStream_t s);
 void launch_combos(const DevCombo* d_combos, const DevSrc* d_src, int ncombo, const int64_t* d_chunk_start,
                    int64_t nchunks, hipStream_t s);
 void launch_diag(const DevDiagBlock* blocks, const DevDiag* terms, int nblocks, hipStream_t s);

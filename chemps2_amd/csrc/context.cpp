@@ -721,7 +721,8 @@ void Ctx::prepare(int st) {
   const double t_route = now_s();
   // ---- chained GEMM tiles for the remaining terms (in (out, in) order)
   std::vector<DevGemmOp> o1, o2;
-  std::vector<DevTile> t1, t2;
+  std::vector<DevTile1> t1;
+  std::vector<DevTile> t2;
   int64_t Wsize = 0;
   struct WRec { int64_t off; int rows, cols; };
   std::vector<WRec> wrec(chain.size(), {-1, 0, 0});
@@ -756,25 +757,28 @@ void Ctx::prepare(int st) {
       q0 = q1;
     }
   }
-  // Index pass (serial, integers only): every two-operand term's phase-1 op, its first
-  // tile and its W offset; every output block's first phase-2 tile. The records are then
-  // filled on the host threads.
+  // Index pass (serial, integers only): every two-operand term's phase-1 op and its W
+  // offset, and the phase-1 tiles per batch and per bucket (tile_bucket), whose ranges
+  // tb1 holds as [B b + c, B b + c + 1) with B = TILE_BUCKETS. The records are then filled
+  // on the host threads (ops) and in order (tiles, by bucket).
   const size_t nq = chain.size();
   std::vector<int> opi1(nq, -1);
-  std::vector<int64_t> tile1(nq, 0);
   tb1.assign(1, 0);
   tb2.assign(1, 0);
-  int64_t nt1 = 0;
   int no1 = 0;
+  std::vector<std::array<int64_t, TILE_BUCKETS>> bcount(1, std::array<int64_t, TILE_BUCKETS>{});
+  auto each_tile = [](const WRec& w, auto&& f) {
+    for (int m0 = 0; m0 < w.rows; m0 += 64)
+      for (int n0 = 0; n0 < w.cols; n0 += 64) f(m0, n0, tile_bucket(std::min(64, w.rows - m0), std::min(64, w.cols - n0)));
+  };
   {
     size_t nextb = 1;
     for (size_t q = 0; q < nq; q++) {
       if (nextb < bstart.size() && q == bstart[nextb]) {
-        tb1.push_back((int)nt1);
+        bcount.push_back(std::array<int64_t, TILE_BUCKETS>{});
         Wsize = 0;
         nextb++;
       }
-      tile1[q] = nt1;
       const Term& t = *chain[q];
       if (t.tl == 'I' || t.tr == 'I') continue;
       const int dL = dLof(t.out), dR = dRof(t.out), dLi = dLof(t.in), dRi = dRof(t.in);
@@ -782,10 +786,15 @@ void Ctx::prepare(int st) {
       wrec[q] = w;
       Wsize += ((int64_t)w.rows * w.cols + 7) / 8 * 8;
       opi1[q] = no1++;
-      nt1 += (int64_t)((w.rows + 63) / 64) * ((w.cols + 63) / 64);
+      each_tile(w, [&](int, int, int c) { bcount.back()[c]++; });
     }
-    tb1.push_back((int)nt1);
   }
+  int64_t nt1 = 0;
+  for (auto& bc : bcount)
+    for (int c = 0; c < TILE_BUCKETS; c++) {
+      nt1 += bc[c];
+      tb1.push_back((int)std::min<int64_t>(nt1, INT32_MAX));
+    }
   // Phase 2: an output block whose products are long (in 16-deep k chunks per tile) is
   // split into segments of consecutive products, so that the launch has many more
   // workgroups than the device holds at once; each segment writes its partial block into
@@ -822,27 +831,76 @@ void Ctx::prepare(int st) {
     const int64_t wseg = std::max<int64_t>(64, wtot / std::max(1, tile_segments));
     for (int64_t bi = 0; bi < nblk; bi++) bseg[bi] = (int)std::max<int64_t>(1, std::min<int64_t>(bwork[bi] / wseg, 64));
   }
-  std::vector<int64_t> tile2(nblk), bpart(nblk, -1);
+  // Segment boundaries (about equal chunk counts) and whether a segment holds an axpy
+  // product; then the tiles per batch in two ranges, tb2[2 b] (tiles of at most 32 x 32
+  // without axpy: one wavefront each, k_tiles_wave2) and tb2[2 b + 1] (the rest, k_tiles_fit).
+  std::vector<int64_t> segq0(nblk + 1, 0);  // first segment of every block in segq / segax
+  std::vector<size_t> segq;                 // first term of every segment
+  std::vector<char> segax;
+  for (int64_t bi = 0; bi < nblk; bi++) {
+    const size_t q0 = bq[bi], q1 = bq[bi + 1];
+    const int ns = bseg[bi];
+    size_t qa = q0;
+    int64_t done = 0;
+    for (int sg = 0; sg < ns; sg++) {
+      size_t qb = qa;
+      const int64_t goal = bwork[bi] * (sg + 1) / ns;
+      if (sg == ns - 1) qb = q1;
+      else
+        while (qb < q1 && done < goal) done += chunks_of(*chain[qb++]);
+      char ax = 0;
+      for (size_t q = qa; q < qb; q++) ax |= (chain[q]->tl == 'I' && chain[q]->tr == 'I') ? 1 : 0;
+      segq.push_back(qa);
+      segax.push_back(ax);
+      qa = qb;
+    }
+    segq0[bi + 1] = (int64_t)segq.size();
+  }
+  segq.push_back(nq);
+  auto wave_tile = [](int dL, int dR, int m0, int n0, bool ax) {
+    return !ax && std::min(64, dL - m0) <= 32 && std::min(64, dR - n0) <= 32;
+  };
+  std::vector<int64_t> tile2w(nblk), tile2f(nblk), bpart(nblk, -1);
   int64_t nt2 = 0, tpart_n = 0;
   std::vector<DevBRed> tred;
   {
+    // per batch: wavefront tiles, then the rest
     size_t nextb = 1;
+    int64_t bb = 0;  // first block of the batch
+    auto close_batch = [&](int64_t be) {
+      int64_t nw = 0, nf = 0;
+      for (int64_t bi = bb; bi < be; bi++) {
+        const int k = chain[bq[bi]]->out;
+        const int dL = dLof(k), dR = dRof(k);
+        int64_t cw = 0, cf = 0;
+        for (int64_t sgi = segq0[bi]; sgi < segq0[bi + 1]; sgi++)
+          for (int m0 = 0; m0 < dL; m0 += 64)
+            for (int n0 = 0; n0 < dR; n0 += 64) (wave_tile(dL, dR, m0, n0, segax[sgi]) ? cw : cf)++;
+        tile2w[bi] = nt2 + nw;
+        tile2f[bi] = nf;  // shifted below, once the batch's wavefront tiles are counted
+        nw += cw;
+        nf += cf;
+      }
+      for (int64_t bi = bb; bi < be; bi++) tile2f[bi] += nt2 + nw;
+      tb2.push_back((int)(nt2 + nw));
+      nt2 += nw + nf;
+      tb2.push_back((int)nt2);
+      bb = be;
+    };
     for (int64_t bi = 0; bi < nblk; bi++) {
       if (nextb < bstart.size() && bq[bi] == bstart[nextb]) {
-        tb2.push_back((int)nt2);
+        close_batch(bi);
         nextb++;
       }
       const int k = chain[bq[bi]]->out;
       const int64_t nel = (int64_t)dLof(k) * dRof(k);
-      tile2[bi] = nt2;
-      nt2 += (int64_t)((dLof(k) + 63) / 64) * ((dRof(k) + 63) / 64) * bseg[bi];
       if (bseg[bi] > 1) {
         bpart[bi] = tpart_n;
         tred.push_back({S.k2i[k], cpart_need + tpart_n, (int)nel, bseg[bi]});
         tpart_n += nel * bseg[bi];
       }
     }
-    tb2.push_back((int)nt2);
+    close_batch(nblk);
   }
   if (nt1 > INT32_MAX || nt2 > INT32_MAX || nq > (size_t)INT32_MAX) throw Error{ERR_SIZE, "too many Heff tiles"};
   tpart_off = cpart_need;
@@ -872,18 +930,34 @@ void Ctx::prepare(int st) {
       }
       op.alpha = t.alpha;
       op.kind = 0;
+      if (op.k <= 0) throw Error{ERR_STATE, "Heff plan: a phase-1 product with no inner dimension"};
       o1[opi1[q]] = op;
-      const WRec& w = wrec[q];
-      int64_t ti = tile1[q];
-      for (int m0 = 0; m0 < w.rows; m0 += 64)
-        for (int n0 = 0; n0 < w.cols; n0 += 64) {
-          DevTile tl{};
-          tl.C = W.p + w.off;
-          tl.ldc = w.rows; tl.m0 = m0; tl.n0 = n0; tl.m = w.rows; tl.n = w.cols; tl.op0 = opi1[q]; tl.nop = 1; tl.flags = TILE_NO_AXPY;
-          t1[ti++] = tl;
-        }
     }
   });
+  {
+    size_t nextb = 1;
+    int b = 0;
+    int64_t pos[TILE_BUCKETS];
+    for (int c = 0; c < TILE_BUCKETS; c++) pos[c] = tb1[c];
+    for (size_t q = 0; q < nq; q++) {
+      if (nextb < bstart.size() && q == bstart[nextb]) {
+        b++;
+        nextb++;
+        for (int c = 0; c < TILE_BUCKETS; c++) pos[c] = tb1[TILE_BUCKETS * b + c];
+      }
+      if (opi1[q] < 0) continue;
+      const WRec& w = wrec[q];
+      const DevGemmOp& op = o1[opi1[q]];
+      each_tile(w, [&](int m0, int n0, int c) {
+        DevTile1 tl{};
+        tl.C = W.p + w.off;
+        tl.A = op.A; tl.B = op.B; tl.alpha = op.alpha;
+        tl.ldc = w.rows; tl.m0 = m0; tl.n0 = n0; tl.mrem = std::min(64, w.rows - m0); tl.nrem = std::min(64, w.cols - n0);
+        tl.lda = op.lda; tl.ldb = op.ldb; tl.k = op.k; tl.ta = op.ta; tl.tb = op.tb; tl.flags = op.flags;
+        t1[pos[c]++] = tl;
+      });
+    }
+  }
   // phase 2: one group per output block (or per segment of it), added to the explicit
   // product
   parallel_for(nblk, nthr, [&](int64_t bi, int) {
@@ -916,19 +990,13 @@ void Ctx::prepare(int st) {
       }
       o2[q] = op;
     }
-    // segments of about equal chunk counts
+    // segments as the index pass cut them
     const int ns = bseg[bi];
-    int64_t ti = tile2[bi];
-    size_t qa = q0;
-    int64_t done = 0;
+    int64_t tw = tile2w[bi], tf = tile2f[bi];
     for (int sg = 0; sg < ns; sg++) {
-      size_t qb = qa;
-      const int64_t goal = bwork[bi] * (sg + 1) / ns;
-      if (sg == ns - 1) qb = q1;
-      else
-        while (qb < q1 && done < goal) done += chunks_of(*chain[qb++]);
-      bool axpy = false;
-      for (size_t q = qa; q < qb; q++) axpy |= o2[q].kind == 1;
+      const int64_t sgi = segq0[bi] + sg;
+      const size_t qa = segq[sgi], qb = (sg == ns - 1) ? q1 : segq[sgi + 1];
+      const bool axpy = segax[sgi] != 0;
       for (int m0 = 0; m0 < dL; m0 += 64)
         for (int n0 = 0; n0 < dR; n0 += 64) {
           DevTile tl{};
@@ -941,12 +1009,10 @@ void Ctx::prepare(int st) {
           }
           if (!axpy) tl.flags |= TILE_NO_AXPY;
           tl.ldc = dL; tl.m0 = m0; tl.n0 = n0; tl.m = dL; tl.n = dR; tl.op0 = (int)qa; tl.nop = (int)(qb - qa);
-          t2[ti++] = tl;
+          t2[wave_tile(dL, dR, m0, n0, axpy) ? tw++ : tf++] = tl;
         }
-      qa = qb;
     }
   });
-  ops1.upload(o1, stream);
   ops2.upload(o2, stream);
   tiles1.upload(t1, stream);
   tiles2.upload(t2, stream);
@@ -1049,9 +1115,14 @@ void Ctx::apply_with(const double* xp, double* yp, double* bp, double* cp, doubl
   launch_bspmv(bitems.p, nbitems, breds.p, nbreds, bp, Mvals.p, M_nnz, M_split ? (mv_low ? 1 : 2) : 0, Mcol.p, xp,
                yp, stream);
   if (timing) hip_check(hipEventRecord(ev[1], stream), "event");
-  for (size_t b = 0; b + 1 < tb1.size(); b++) {  // GEMM tiles, batch by batch (W is shared)
-    launch_tiles(tiles1.p + tb1[b], ops1.p, tb1[b + 1] - tb1[b], xp, yp, stream, tiles_kernel);
-    launch_tiles(tiles2.p + tb2[b], ops2.p, tb2[b + 1] - tb2[b], xp, yp, stream, tiles_kernel, cp);
+  for (size_t b = 0; 2 * b + 2 < tb2.size(); b++) {  // GEMM tiles, batch by batch (W is shared)
+    for (int c = 0; c < TILE_BUCKETS; c++) {
+      const int i = TILE_BUCKETS * (int)b + c;
+      launch_tiles_stream1(tiles1.p + tb1[i], tb1[i + 1] - tb1[i], c, xp, stream);
+    }
+    const int w0 = tb2[2 * b], w1 = tb2[2 * b + 1], f1 = tb2[2 * b + 2];
+    launch_tiles_wave2(tiles2.p + w0, ops2.p, w1 - w0, xp, yp, stream, cp);
+    launch_tiles(tiles2.p + w1, ops2.p, f1 - w1, xp, yp, stream, tiles_kernel, cp);
   }
   launch_chain_reduce(treds.p, ntreds, cp, yp, stream);  // segments of split output blocks
   if (timing) hip_check(hipEventRecord(ev[2], stream), "event");

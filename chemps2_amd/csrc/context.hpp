@@ -142,14 +142,15 @@ struct Ctx : HostState {
   double F_H = 0.0, B_H = 0.0;
   int64_t V = 0;
   DBuf<double> combo_arena, W, dx, dy, partial, scal;
-  DBuf<DevTile> tiles1, tiles2;
+  DBuf<DevTile1> tiles1;  // phase 1: W tiles of the two-operand terms, each with its product
+  DBuf<DevTile> tiles2;
   std::vector<int> tb1, tb2;  // tile ranges of the batches that share the W scratch
   DBuf<DevBRed> treds;        // phase-2 output blocks split into segments: their reductions
   int ntreds = 0;
   int64_t tpart_off = 0;      // first partial sum of those segments in the chained partials
   int tile_segments = 8192;   // CHM_TILE_SEGMENTS: phase-2 workgroups aimed at per launch
   int64_t tile_w_cap = 0;  // doubles of W scratch per batch (CHM_TILE_W_CAP; 0: a quarter of free memory)
-  DBuf<DevGemmOp> ops1, ops2;
+  DBuf<DevGemmOp> ops2;
   int ntiles1 = 0, ntiles2 = 0;
   DBuf<DevDiagBlock> dblocks;
   std::vector<DevDiagBlock> hblocks;  // block outputs as offsets into the diagonal
@@ -241,7 +242,7 @@ struct Ctx : HostState {
   bool svd_large_gesvdj = false;     // CHM_SVD_GESVDJ=1: large Split sectors by rocsolver_dgesvdj instead of dgesvd
   int chain_item_terms = CHAIN_ITEM_TERMS;    // CHM_CHAIN_ITEM_TERMS
   int chain_group_terms = CHAIN_GROUP_TERMS;  // CHM_CHAIN_GROUP_TERMS
-  int tiles_kernel = 2;  // CHM_TILES_MFMA: GEMM tiles in 2 k_tiles_wave, 1 k_tiles_mfma (LDS staging), 0 k_tiles (vector FMA)
+  int tiles_kernel = 2;  // CHM_TILES_MFMA: grouped GEMM tiles (Heff phase 2, updates) in 2 k_tiles_fit, 1 k_tiles_mfma, 0 k_tiles
   int cblocks_m_offs[CHAIN_MFMA_NT + 1] = {};
   DBuf<DevCBlock> cblocks;
   DBuf<DevCGroup> cgroups;

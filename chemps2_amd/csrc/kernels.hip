@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 
 #include "chm_internal.hpp"
 
@@ -137,6 +138,8 @@ __global__ __launch_bounds__(NT) void k_tiles(const DevTile* __restrict__ tiles,
 // registers while the current one computes from LDS, and the two LDS buffers alternate,
 // so one barrier per chunk suffices. axpy products (kind 1) are added first.
 typedef double chm_t4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const double gdbl;  // global (not flat) loads
+typedef __attribute__((address_space(1))) const unsigned char gbyte;
 
 __global__ __launch_bounds__(NT) void k_tiles_mfma(const DevTile* __restrict__ tiles, const DevGemmOp* __restrict__ ops,
                                                    const double* __restrict__ xbase, double* __restrict__ ybase,
@@ -318,41 +321,48 @@ __device__ __forceinline__ void tiles_fit_body(const DevTile& t, const DevGemmOp
   };
   skip();
   double ra[4], rb[4];
-  auto fetch = [&]() {  // this thread's share of chunk (o, k0), alpha folded into A
+  // this thread's elements of chunk (op, k0): (r, kk) of op(A), (kb, c) of op(B)
+  auto a_at = [&](const DevGemmOp& op, int q, int& r, int& kk) {
+    if (op.ta == 0) { r = tid & 63; kk = (tid >> 6) + 4 * q; } else { kk = tid & 15; r = (tid >> 4) + 16 * q; }
+  };
+  auto b_at = [&](const DevGemmOp& op, int q, int& kb, int& c) {
+    if (op.tb == 0) { kb = tid & 15; c = (tid >> 4) + 16 * q; } else { c = tid & 63; kb = (tid >> 6) + 4 * q; }
+  };
+  // raw global (not flat) loads only, a lane outside the block reading the block's first
+  // element: put applies the masks and alpha, so nothing here waits for the loads and the
+  // LDS waits of the compute do not wait for them either
+  auto fetch = [&]() {
     const DevGemmOp op = ops[t.op0 + o];
-    const double* A = rel(op.A, op.flags & 1, xbase);
-    const double* B = rel(op.B, (op.flags >> 1) & 1, xbase);
+    gdbl* A = (gdbl*)rel(op.A, op.flags & 1, xbase);
+    gdbl* B = (gdbl*)rel(op.B, (op.flags >> 1) & 1, xbase);
     const int krem = min(TK, op.k - k0);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      int r, kk;
-      if (op.ta == 0) { r = tid & 63; kk = (tid >> 6) + 4 * q; } else { kk = tid & 15; r = (tid >> 4) + 16 * q; }
-      double v = 0.0;
-      if (r < mrem && kk < krem)
-        v = op.ta ? A[(size_t)(k0 + kk) + (size_t)op.lda * (t.m0 + r)] : A[(size_t)(t.m0 + r) + (size_t)op.lda * (k0 + kk)];
-      ra[q] = op.alpha * v;
-      int c, kb;
-      if (op.tb == 0) { kb = tid & 15; c = (tid >> 4) + 16 * q; } else { c = tid & 63; kb = (tid >> 6) + 4 * q; }
-      double w = 0.0;
-      if (c < nrem && kb < krem)
-        w = op.tb ? B[(size_t)(t.n0 + c) + (size_t)op.ldb * (k0 + kb)] : B[(size_t)(k0 + kb) + (size_t)op.ldb * (t.n0 + c)];
-      rb[q] = w;
+      int r, kk, kb, c;
+      a_at(op, q, r, kk);
+      b_at(op, q, kb, c);
+      const size_t ia = op.ta ? (size_t)(k0 + kk) + (size_t)op.lda * (t.m0 + r) : (size_t)(t.m0 + r) + (size_t)op.lda * (k0 + kk);
+      const size_t ib = op.tb ? (size_t)(t.n0 + c) + (size_t)op.ldb * (k0 + kb) : (size_t)(k0 + kb) + (size_t)op.ldb * (t.n0 + c);
+      ra[q] = A[(r < mrem && kk < krem) ? ia : 0];
+      rb[q] = B[(c < nrem && kb < krem) ? ib : 0];
     }
   };
-  auto put = [&](int buf, int ta, int tb) {
+  auto put = [&](int buf, const DevGemmOp& op, int kc) {  // the chunk (op, kc) fetched last
+    const int krem = min(TK, op.k - kc);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      if (ta == 0) As[buf][(tid >> 6) + 4 * q][tid & 63] = ra[q];
-      else As[buf][tid & 15][(tid >> 4) + 16 * q] = ra[q];
-      if (tb == 0) Bs[buf][tid & 15][(tid >> 4) + 16 * q] = rb[q];
-      else Bs[buf][(tid >> 6) + 4 * q][tid & 63] = rb[q];
+      int r, kk, kb, c;
+      a_at(op, q, r, kk);
+      b_at(op, q, kb, c);
+      As[buf][kk][r] = (r < mrem && kk < krem) ? op.alpha * ra[q] : 0.0;
+      Bs[buf][kb][c] = (c < nrem && kb < krem) ? rb[q] : 0.0;
     }
   };
   int buf = 0;
   if (o < t.nop) fetch();
   while (o < t.nop) {
     const DevGemmOp op = ops[t.op0 + o];
-    put(buf, op.ta, op.tb);
+    put(buf, op, k0);
     __syncthreads();
     k0 += TK;
     skip();
@@ -396,6 +406,397 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     case 3: tiles_fit_body<3>(t, ops, xbase, ybase, pbase, As, Bs, ms, nsub); break;
     default: tiles_fit_body<4>(t, ops, xbase, ybase, pbase, As, Bs, ms, nsub); break;
   }
+}
+
+// Phase-1 tiles as a stream (k_tiles_stream1): tiles of one GEMM product each (the
+// intermediates W = op(L) x or x op(R) of the chained terms), each record holding the tile
+// and its product. A workgroup walks tiles blockIdx.x, blockIdx.x + gridDim.x, ... as one
+// sequence of 16-deep k chunks: the next chunk (of this tile or the next) is loaded while
+// the current one computes, so a tile's start no longer waits for its record and
+// operands. The next tile's record comes in a tile ahead as one vector load of a 32-bit
+// word per lane and is read with v_readlane when due. Operands are read through global
+// (not flat) pointers, whose loads the LDS wait of the compute does not wait for. All
+// tiles of a launch have C sub-tiles per wavefront (the host sorts them by C); sub-tiles
+// as in k_tiles_fit.
+static_assert(sizeof(DevTile1) == 80 && offsetof(DevTile1, ldc) == 32 && offsetof(DevTile1, flags) == 72,
+              "DevTile1 words");
+struct Ts1 {
+  double* C;
+  const double* A;
+  const double* B;
+  double alpha;
+  int ldc, m0, n0, mrem, nrem, lda, ldb, k, ta, tb, flags;
+};
+__device__ __forceinline__ int ts_rl(uint32_t v, int k) { return __builtin_amdgcn_readlane((int)v, k); }
+__device__ __forceinline__ uint64_t ts_rl64(uint32_t v, int k) {
+  return (uint64_t)(uint32_t)ts_rl(v, k) | ((uint64_t)(uint32_t)ts_rl(v, k + 1) << 32);
+}
+__device__ __forceinline__ Ts1 ts_rec(uint32_t v) {
+  Ts1 t;
+  t.C = (double*)ts_rl64(v, 0);
+  t.A = (const double*)ts_rl64(v, 2);
+  t.B = (const double*)ts_rl64(v, 4);
+  t.alpha = __longlong_as_double((long long)ts_rl64(v, 6));
+  t.ldc = ts_rl(v, 8);
+  t.m0 = ts_rl(v, 9);
+  t.n0 = ts_rl(v, 10);
+  t.mrem = ts_rl(v, 11);
+  t.nrem = ts_rl(v, 12);
+  t.lda = ts_rl(v, 13);
+  t.ldb = ts_rl(v, 14);
+  t.k = ts_rl(v, 15);
+  t.ta = ts_rl(v, 16);
+  t.tb = ts_rl(v, 17);
+  t.flags = ts_rl(v, 18);
+  return t;
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_tiles_stream1(
+    const DevTile1* __restrict__ tiles, int ntiles, const double* __restrict__ xbase) {
+  __shared__ double As[2][TK][TM + 1];
+  __shared__ double Bs[2][TK][TN + 1];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, lr = lane & 15, lg = lane >> 4;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int G = gridDim.x;
+  int ti = blockIdx.x;  // tile of the chunk being fetched
+  if (ti >= ntiles) return;
+  const uint32_t* __restrict__ tw = (const uint32_t*)tiles;
+  auto rec_words = [&](int i) -> uint32_t {
+    return (i < ntiles && lane < 20) ? ((__attribute__((address_space(1))) const uint32_t*)tw)[(size_t)i * 20 + lane] : 0u;
+  };
+
+  // the fetch side: tile fT, chunk k0, and the words of the next tile's record
+  Ts1 fT = ts_rec(rec_words(ti));
+  uint32_t vN = rec_words(ti + G);
+  int k0 = 0;
+
+  // the compute side: the tile of the chunk in LDS, its sub-tiles and accumulators
+  Ts1 cT = fT;
+  int offa[C], offb[C], si[C], sj[C];
+  bool on[C];
+  auto map_tile = [&](const Ts1& t) {
+    const int ms = (t.mrem + 15) >> 4, ns = (t.nrem + 15) >> 4, nsub = ms * ns;
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const int sq = wv + 4 * q;
+      on[q] = sq < nsub;
+      const int se = on[q] ? sq : 0;
+      si[q] = se % ms;
+      sj[q] = se / ms;
+      offa[q] = 16 * si[q] + lr;
+      offb[q] = 16 * sj[q] + lr;
+    }
+  };
+  map_tile(cT);
+  chm_t4 acc[C];
+#pragma unroll
+  for (int q = 0; q < C; q++) acc[q] = chm_t4{0.0, 0.0, 0.0, 0.0};
+
+  double ra[4], rb[4];
+  // this thread's elements of chunk (fT, k0): (r, kk) of op(A), (kb, c) of op(B)
+  auto a_at = [&](int q, int& r, int& kk) {
+    if (fT.ta == 0) { r = tid & 63; kk = (tid >> 6) + 4 * q; } else { kk = tid & 15; r = (tid >> 4) + 16 * q; }
+  };
+  auto b_at = [&](int q, int& kb, int& c) {
+    if (fT.tb == 0) { kb = tid & 15; c = (tid >> 4) + 16 * q; } else { c = tid & 63; kb = (tid >> 6) + 4 * q; }
+  };
+  // raw loads only (a lane outside the block reads the block's first element): the
+  // masks and alpha are applied by put, so nothing here waits for the loads
+  auto fetch = [&]() {
+    gdbl* A = (gdbl*)((fT.flags & 1) ? xbase + (intptr_t)fT.A : fT.A);
+    gdbl* B = (gdbl*)((fT.flags & 2) ? xbase + (intptr_t)fT.B : fT.B);
+    const int krem = min(TK, fT.k - k0);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int r, kk, kb, c;
+      a_at(q, r, kk);
+      b_at(q, kb, c);
+      const size_t ia = fT.ta ? (size_t)(k0 + kk) + (size_t)fT.lda * (fT.m0 + r) : (size_t)(fT.m0 + r) + (size_t)fT.lda * (k0 + kk);
+      const size_t ib = fT.tb ? (size_t)(fT.n0 + c) + (size_t)fT.ldb * (k0 + kb) : (size_t)(k0 + kb) + (size_t)fT.ldb * (fT.n0 + c);
+      ra[q] = A[(r < fT.mrem && kk < krem) ? ia : 0];
+      rb[q] = B[(c < fT.nrem && kb < krem) ? ib : 0];
+    }
+  };
+  // the chunk fetched last into LDS buffer buf (fT and k0 still describe it)
+  auto put = [&](int buf) {
+    const int krem = min(TK, fT.k - k0);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int r, kk, kb, c;
+      a_at(q, r, kk);
+      b_at(q, kb, c);
+      As[buf][kk][r] = (r < fT.mrem && kk < krem) ? fT.alpha * ra[q] : 0.0;
+      Bs[buf][kb][c] = (c < fT.nrem && kb < krem) ? rb[q] : 0.0;
+    }
+  };
+  auto store = [&](const Ts1& t) {
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      if (!on[q]) continue;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int rr = 16 * si[q] + 4 * r + lg, cc = 16 * sj[q] + lr;
+        if (rr < t.mrem && cc < t.nrem) t.C[(size_t)(t.m0 + rr) + (size_t)t.ldc * (t.n0 + cc)] = acc[q][r];
+      }
+    }
+  };
+
+  int buf = 0;
+  fetch();
+  for (;;) {
+    put(buf);
+    __syncthreads();
+    // the next chunk: the rest of this product, else the first chunk of the next tile
+    bool more = true, last = false;
+    k0 += TK;
+    if (k0 >= fT.k) {
+      last = true;
+      ti += G;
+      if (ti >= ntiles) {
+        more = false;
+      } else {
+        k0 = 0;
+        fT = ts_rec(vN);
+        vN = rec_words(ti + G);
+      }
+    }
+    if (more) fetch();  // in flight while this chunk computes
+#pragma unroll
+    for (int ks = 0; ks < TK / 4; ks++) {  // k past the product's end is zero in LDS
+      const int kk = 4 * ks + lg;
+#pragma unroll
+      for (int q = 0; q < C; q++)
+        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[buf][kk][offa[q]], Bs[buf][kk][offb[q]], acc[q], 0, 0, 0);
+    }
+    if (last) {  // the chunk just computed ended tile cT
+      store(cT);
+#pragma unroll
+      for (int q = 0; q < C; q++) acc[q] = chm_t4{0.0, 0.0, 0.0, 0.0};
+      if (!more) break;
+      cT = fT;
+      map_tile(cT);
+    }
+    buf ^= 1;
+  }
+}
+
+// Wavefront tiles of at most 32 x 32 (k_tiles_wave1, k_tiles_wave2): one wavefront per
+// tile, no LDS and no barrier. Each lane loads its MFMA operands for a 16-deep k chunk
+// straight from global memory (a[ks][i] = alpha op(A)[16 i + lr, k0 + 4 ks + lg],
+// b[ks][j] = op(B)[k0 + 4 ks + lg, 16 j + lr]) as raw loads of clamped addresses; the next
+// chunk is in flight while the current one computes, and the masks and alpha are applied
+// when a chunk is taken for the MFMAs. Records come in ahead as vector loads of one word
+// per lane and are read with v_readlane when due.
+struct TwOp {  // one product on one tile, pointers resolved
+  const double* A;
+  const double* B;
+  double alpha;
+  int lda, ldb, k, ta, tb;
+  int m0, n0, mrem, nrem;
+};
+// raw loads of chunk (f, k0); 32-bit byte offsets (an operator block or x holds fewer
+// than 2^28 elements), so that the loads take the uniform base from scalar registers
+__device__ __forceinline__ void tw_fetch(const TwOp& f, int k0, int lr, int lg, double (&pa)[4][2], double (&pb)[4][2]) {
+  gbyte* A = (gbyte*)f.A;
+  gbyte* B = (gbyte*)f.B;
+  const int krem = f.k - k0;
+  const uint32_t sa_k = f.ta ? 1u : (uint32_t)f.lda, sa_r = f.ta ? (uint32_t)f.lda : 1u;
+  const uint32_t sb_k = f.tb ? (uint32_t)f.ldb : 1u, sb_c = f.tb ? 1u : (uint32_t)f.ldb;
+#pragma unroll
+  for (int ks = 0; ks < 4; ks++) {
+    const int kk = 4 * ks + lg;
+    const uint32_t kg = (uint32_t)(k0 + kk);
+    const bool kok = kk < krem;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const int r = 16 * i + lr;
+      const uint32_t ia = kg * sa_k + (uint32_t)(f.m0 + r) * sa_r;
+      pa[ks][i] = *(gdbl*)(A + ((kok && r < f.mrem) ? ia * 8u : 0u));
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int c = 16 * j + lr;
+      const uint32_t ib = kg * sb_k + (uint32_t)(f.n0 + c) * sb_c;
+      pb[ks][j] = *(gdbl*)(B + ((kok && c < f.nrem) ? ib * 8u : 0u));
+    }
+  }
+}
+__device__ __forceinline__ void tw_take(const TwOp& f, int k0, int lr, int lg, double (&pa)[4][2], double (&pb)[4][2]) {
+  const int krem = f.k - k0;
+#pragma unroll
+  for (int ks = 0; ks < 4; ks++) {
+    const bool kok = 4 * ks + lg < krem;
+#pragma unroll
+    for (int i = 0; i < 2; i++) pa[ks][i] = (kok && 16 * i + lr < f.mrem) ? f.alpha * pa[ks][i] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 2; j++) pb[ks][j] = (kok && 16 * j + lr < f.nrem) ? pb[ks][j] : 0.0;
+  }
+}
+__device__ __forceinline__ void tw_mma(chm_t4 (&acc)[2][2], const double (&pa)[4][2], const double (&pb)[4][2]) {
+#pragma unroll
+  for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks][i], pb[ks][j], acc[i][j], 0, 0, 0);
+}
+__device__ __forceinline__ void tw_zero(chm_t4 (&acc)[2][2]) {
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = chm_t4{0.0, 0.0, 0.0, 0.0};
+}
+// the tile's elements of this lane: out[(m0 + row) + ldc (n0 + col)] (+)= acc
+__device__ __forceinline__ void tw_store(const chm_t4 (&acc)[2][2], double* out, int ldc, int m0, int n0, int mrem,
+                                         int nrem, bool add, int lr, int lg) {
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int rr = 16 * i + 4 * r + lg, cc = 16 * j + lr;
+        if (rr < mrem && cc < nrem) {
+          double* p = out + (size_t)(m0 + rr) + (size_t)ldc * (n0 + cc);
+          *p = add ? *p + acc[i][j][r] : acc[i][j][r];
+        }
+      }
+}
+
+// Phase-1 tiles (DevTile1) of at most 32 x 32: wavefront w of the grid walks tiles w,
+// w + W, ... (W wavefronts in the grid) as one chunk sequence; the next tile's record
+// comes in a tile ahead.
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_tiles_wave1(
+    const DevTile1* __restrict__ tiles, int ntiles, const double* __restrict__ xbase) {
+  const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
+  const int W = gridDim.x * (NT / 64);
+  int ti = blockIdx.x * (NT / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (ti >= ntiles) return;
+  const uint32_t* __restrict__ tw = (const uint32_t*)tiles;
+  auto rec_words = [&](int i) -> uint32_t {
+    return (i < ntiles && lane < 20) ? ((__attribute__((address_space(1))) const uint32_t*)tw)[(size_t)i * 20 + lane] : 0u;
+  };
+  auto as_op = [&](const Ts1& t) {
+    TwOp f;
+    f.A = (t.flags & 1) ? xbase + (intptr_t)t.A : t.A;
+    f.B = (t.flags & 2) ? xbase + (intptr_t)t.B : t.B;
+    f.alpha = t.alpha;
+    f.lda = t.lda; f.ldb = t.ldb; f.k = t.k; f.ta = t.ta; f.tb = t.tb;
+    f.m0 = t.m0; f.n0 = t.n0; f.mrem = t.mrem; f.nrem = t.nrem;
+    return f;
+  };
+  Ts1 fT = ts_rec(rec_words(ti));  // the tile of the chunk being fetched
+  TwOp f = as_op(fT);
+  uint32_t vN = rec_words(ti + W);
+  int k0 = 0;
+  Ts1 cT = fT;  // the tile of the chunk being computed
+  chm_t4 acc[2][2];
+  tw_zero(acc);
+  // moves the fetch side to the next chunk; last: the chunk before ended its tile
+  auto advance = [&](bool& last) -> bool {
+    k0 += TK;
+    last = false;
+    if (k0 < fT.k) return true;
+    last = true;
+    ti += W;
+    if (ti >= ntiles) return false;
+    k0 = 0;
+    fT = ts_rec(vN);
+    f = as_op(fT);
+    vN = rec_words(ti + W);
+    return true;
+  };
+  auto finish = [&](bool last, bool more) -> bool {  // after the MFMAs of a chunk; false: done
+    if (!last) return true;
+    tw_store(acc, cT.C, cT.ldc, cT.m0, cT.n0, cT.mrem, cT.nrem, false, lr, lg);
+    tw_zero(acc);
+    if (!more) return false;
+    cT = fT;
+    return true;
+  };
+  double a0[4][2], b0[4][2], a1[4][2], b1[4][2];
+  tw_fetch(f, k0, lr, lg, a0, b0);
+  for (;;) {
+    bool last;
+    tw_take(f, k0, lr, lg, a0, b0);
+    bool more = advance(last);
+    if (more) tw_fetch(f, k0, lr, lg, a1, b1);  // in flight while chunk 0 computes
+    tw_mma(acc, a0, b0);
+    if (!finish(last, more)) break;
+    tw_take(f, k0, lr, lg, a1, b1);
+    more = advance(last);
+    if (more) tw_fetch(f, k0, lr, lg, a0, b0);
+    tw_mma(acc, a1, b1);
+    if (!finish(last, more)) break;
+  }
+}
+
+// Grouped tiles (DevTile + DevGemmOp lists, phase 2) of at most 32 x 32 whose products are
+// all GEMMs with k > 0: one wavefront per tile walks the chunks of its products; the next
+// product's record comes in a product ahead.
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_tiles_wave2(
+    const DevTile* __restrict__ tiles, int ntiles, const DevGemmOp* __restrict__ ops, const double* __restrict__ xbase,
+    double* __restrict__ ybase, double* __restrict__ pbase) {
+  const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
+  const int ti = blockIdx.x * (NT / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (ti >= ntiles) return;
+  const DevTile t = tiles[ti];
+  if (t.nop <= 0) {
+    chm_t4 z[2][2];
+    tw_zero(z);
+    tw_store(z, tile_out(t, ybase, pbase), t.ldc, t.m0, t.n0, min(TM, t.m - t.m0), min(TN, t.n - t.n0),
+             (t.flags & 2) != 0, lr, lg);
+    return;
+  }
+  const uint32_t* __restrict__ ow = (const uint32_t*)(ops + t.op0);
+  auto op_words = [&](int i) -> uint32_t {
+    return (i < t.nop && lane < 14) ? ((__attribute__((address_space(1))) const uint32_t*)ow)[(size_t)i * 14 + lane] : 0u;
+  };
+  const int mrem = min(TM, t.m - t.m0), nrem = min(TN, t.n - t.n0);
+  auto as_op = [&](uint32_t v) {
+    TwOp f;
+    const int fl = ts_rl(v, 12);
+    const double* A = (const double*)ts_rl64(v, 0);
+    const double* B = (const double*)ts_rl64(v, 2);
+    f.A = (fl & 1) ? xbase + (intptr_t)A : A;
+    f.B = (fl & 2) ? xbase + (intptr_t)B : B;
+    f.alpha = __longlong_as_double((long long)ts_rl64(v, 4));
+    f.lda = ts_rl(v, 6); f.ldb = ts_rl(v, 7); f.k = ts_rl(v, 8); f.ta = ts_rl(v, 9); f.tb = ts_rl(v, 10);
+    f.m0 = t.m0; f.n0 = t.n0; f.mrem = mrem; f.nrem = nrem;
+    return f;
+  };
+  int o = 0, k0 = 0;
+  TwOp f = as_op(op_words(0));
+  uint32_t vN = op_words(1);
+  chm_t4 acc[2][2];
+  tw_zero(acc);
+  auto advance = [&]() -> bool {
+    k0 += TK;
+    if (k0 < f.k) return true;
+    o++;
+    if (o >= t.nop) return false;
+    k0 = 0;
+    f = as_op(vN);
+    vN = op_words(o + 1);
+    return true;
+  };
+  double a0[4][2], b0[4][2], a1[4][2], b1[4][2];
+  tw_fetch(f, k0, lr, lg, a0, b0);
+  for (;;) {
+    tw_take(f, k0, lr, lg, a0, b0);
+    bool more = advance();
+    if (more) tw_fetch(f, k0, lr, lg, a1, b1);  // in flight while chunk 0 computes
+    tw_mma(acc, a0, b0);
+    if (!more) break;
+    tw_take(f, k0, lr, lg, a1, b1);
+    more = advance();
+    if (more) tw_fetch(f, k0, lr, lg, a0, b0);
+    tw_mma(acc, a1, b1);
+    if (!more) break;
+  }
+  tw_store(acc, tile_out(t, ybase, pbase), t.ldc, t.m0, t.n0, mrem, nrem, (t.flags & 2) != 0, lr, lg);
 }
 
 // Assembles the explicit sub-block of every (kappa, kappa') pair:
@@ -1867,6 +2268,28 @@ void launch_tiles(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const 
   if (mode == 2) hipLaunchKernelGGL(k_tiles_fit, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase, pbase);
   else if (mode == 1) hipLaunchKernelGGL(k_tiles_mfma, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase, pbase);
   else hipLaunchKernelGGL(k_tiles, dim3(ntiles), dim3(NT), 0, s, tiles, ops, xbase, ybase, pbase);
+}
+
+void launch_tiles_stream1(const DevTile1* tiles, int ntiles, int c, const double* xbase, hipStream_t s) {
+  if (ntiles <= 0) return;
+  if (c == 0) {  // tiles of at most 32 x 32: one wavefront each
+    const int g = std::min((ntiles + 3) / 4, TILE_WAVE_GROUPS);
+    hipLaunchKernelGGL(k_tiles_wave1, dim3(g), dim3(NT), 0, s, tiles, ntiles, xbase);
+    return;
+  }
+  const int g = std::min(ntiles, TILE_STREAM_GROUPS);
+  switch (c) {
+    case 1: hipLaunchKernelGGL(k_tiles_stream1<1>, dim3(g), dim3(NT), 0, s, tiles, ntiles, xbase); break;
+    case 2: hipLaunchKernelGGL(k_tiles_stream1<2>, dim3(g), dim3(NT), 0, s, tiles, ntiles, xbase); break;
+    case 3: hipLaunchKernelGGL(k_tiles_stream1<3>, dim3(g), dim3(NT), 0, s, tiles, ntiles, xbase); break;
+    default: hipLaunchKernelGGL(k_tiles_stream1<4>, dim3(g), dim3(NT), 0, s, tiles, ntiles, xbase); break;
+  }
+}
+
+void launch_tiles_wave2(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const double* xbase, double* ybase,
+                        hipStream_t s, double* pbase) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_tiles_wave2, dim3((ntiles + 3) / 4), dim3(NT), 0, s, tiles, ntiles, ops, xbase, ybase, pbase);
 }
 
 void launch_assemble(const DevPair* pairs, const DevKTerm* terms, int64_t npairs, double* M, int64_t nnz, bool split,
