@@ -767,9 +767,12 @@ void Ctx::prepare(int st) {
   tb2.assign(1, 0);
   int no1 = 0;
   std::vector<std::array<int64_t, TILE_BUCKETS>> bcount(1, std::array<int64_t, TILE_BUCKETS>{});
-  auto each_tile = [](const WRec& w, auto&& f) {
-    for (int m0 = 0; m0 < w.rows; m0 += 64)
-      for (int n0 = 0; n0 < w.cols; n0 += 64) f(m0, n0, tile_bucket(std::min(64, w.rows - m0), std::min(64, w.cols - n0)));
+  // tiles of 64 x 64 by bucket, or (wave_tiles) of 32 x 32, all on wavefronts
+  const int ts1 = wave_tiles ? 32 : 64;
+  auto each_tile = [&](const WRec& w, auto&& f) {
+    for (int m0 = 0; m0 < w.rows; m0 += ts1)
+      for (int n0 = 0; n0 < w.cols; n0 += ts1)
+        f(m0, n0, wave_tiles ? 0 : tile_bucket(std::min(64, w.rows - m0), std::min(64, w.cols - n0)));
   };
   {
     size_t nextb = 1;
@@ -857,8 +860,14 @@ void Ctx::prepare(int st) {
     segq0[bi + 1] = (int64_t)segq.size();
   }
   segq.push_back(nq);
-  auto wave_tile = [](int dL, int dR, int m0, int n0, bool ax) {
-    return !ax && std::min(64, dL - m0) <= 32 && std::min(64, dR - n0) <= 32;
+  // the tiles of an output block (segment): 64 x 64, those of at most 32 x 32 without axpy
+  // on wavefronts; or (wave_tiles) 32 x 32 on wavefronts unless the segment has an axpy
+  auto each_tile2 = [&](int dL, int dR, bool ax, auto&& f) {
+    const bool q32 = wave_tiles && !ax;
+    const int st = q32 ? 32 : 64;
+    for (int m0 = 0; m0 < dL; m0 += st)
+      for (int n0 = 0; n0 < dR; n0 += st)
+        f(m0, n0, q32 || (!ax && std::min(64, dL - m0) <= 32 && std::min(64, dR - n0) <= 32));
   };
   std::vector<int64_t> tile2w(nblk), tile2f(nblk), bpart(nblk, -1);
   int64_t nt2 = 0, tpart_n = 0;
@@ -874,8 +883,7 @@ void Ctx::prepare(int st) {
         const int dL = dLof(k), dR = dRof(k);
         int64_t cw = 0, cf = 0;
         for (int64_t sgi = segq0[bi]; sgi < segq0[bi + 1]; sgi++)
-          for (int m0 = 0; m0 < dL; m0 += 64)
-            for (int n0 = 0; n0 < dR; n0 += 64) (wave_tile(dL, dR, m0, n0, segax[sgi]) ? cw : cf)++;
+          each_tile2(dL, dR, segax[sgi] != 0, [&](int, int, bool wv) { (wv ? cw : cf)++; });
         tile2w[bi] = nt2 + nw;
         tile2f[bi] = nf;  // shifted below, once the batch's wavefront tiles are counted
         nw += cw;
@@ -952,7 +960,7 @@ void Ctx::prepare(int st) {
         DevTile1 tl{};
         tl.C = W.p + w.off;
         tl.A = op.A; tl.B = op.B; tl.alpha = op.alpha;
-        tl.ldc = w.rows; tl.m0 = m0; tl.n0 = n0; tl.mrem = std::min(64, w.rows - m0); tl.nrem = std::min(64, w.cols - n0);
+        tl.ldc = w.rows; tl.m0 = m0; tl.n0 = n0; tl.mrem = std::min(ts1, w.rows - m0); tl.nrem = std::min(ts1, w.cols - n0);
         tl.lda = op.lda; tl.ldb = op.ldb; tl.k = op.k; tl.ta = op.ta; tl.tb = op.tb; tl.flags = op.flags;
         t1[pos[c]++] = tl;
       });
@@ -997,8 +1005,7 @@ void Ctx::prepare(int st) {
       const int64_t sgi = segq0[bi] + sg;
       const size_t qa = segq[sgi], qb = (sg == ns - 1) ? q1 : segq[sgi + 1];
       const bool axpy = segax[sgi] != 0;
-      for (int m0 = 0; m0 < dL; m0 += 64)
-        for (int n0 = 0; n0 < dR; n0 += 64) {
+      each_tile2(dL, dR, axpy, [&](int m0, int n0, bool wv) {
           DevTile tl{};
           if (ns == 1) {
             tl.C = (double*)(intptr_t)S.k2i[k];
@@ -1009,8 +1016,8 @@ void Ctx::prepare(int st) {
           }
           if (!axpy) tl.flags |= TILE_NO_AXPY;
           tl.ldc = dL; tl.m0 = m0; tl.n0 = n0; tl.m = dL; tl.n = dR; tl.op0 = (int)qa; tl.nop = (int)(qb - qa);
-          t2[wave_tile(dL, dR, m0, n0, axpy) ? tw++ : tf++] = tl;
-        }
+          t2[wv ? tw++ : tf++] = tl;
+        });
     }
   });
   ops2.upload(o2, stream);
@@ -2282,6 +2289,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_CHAIN_MFMA")) ctx->c.use_chain_mfma = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_TILES_MFMA")) ctx->c.tiles_kernel = std::max(0, std::min(2, std::atoi(e)));
   if (const char* e = std::getenv("CHM_TILE_SEGMENTS")) ctx->c.tile_segments = std::max(1, std::atoi(e));
+  if (const char* e = std::getenv("CHM_WAVE_TILES")) ctx->c.wave_tiles = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_CHAIN_ITEM_TERMS")) ctx->c.chain_item_terms = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("CHM_CHAIN_GROUP_TERMS")) ctx->c.chain_group_terms = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("CHM_SVD_GESVDJ")) ctx->c.svd_large_gesvdj = std::atoi(e) != 0;

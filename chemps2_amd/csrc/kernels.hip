@@ -746,7 +746,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3, 8))) void
   if (t.nop <= 0) {
     chm_t4 z[2][2];
     tw_zero(z);
-    tw_store(z, tile_out(t, ybase, pbase), t.ldc, t.m0, t.n0, min(TM, t.m - t.m0), min(TN, t.n - t.n0),
+    tw_store(z, tile_out(t, ybase, pbase), t.ldc, t.m0, t.n0, min(32, t.m - t.m0), min(32, t.n - t.n0),
              (t.flags & 2) != 0, lr, lg);
     return;
   }
@@ -754,7 +754,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3, 8))) void
   auto op_words = [&](int i) -> uint32_t {
     return (i < t.nop && lane < 14) ? ((__attribute__((address_space(1))) const uint32_t*)ow)[(size_t)i * 14 + lane] : 0u;
   };
-  const int mrem = min(TM, t.m - t.m0), nrem = min(TN, t.n - t.n0);
+  const int mrem = min(32, t.m - t.m0), nrem = min(32, t.n - t.n0);  // the wavefront's 32 x 32
   auto as_op = [&](uint32_t v) {
     TwOp f;
     const int fl = ts_rl(v, 12);
