@@ -92,7 +92,8 @@ def traffic_from(fetch_csv, write_csv, steps, out_name):
     rd = sum(fk) * fscale * 1024.0 / steps
     wr = sum(wk) * wscale * 1024.0 / steps
     out = {
-        "kernel": "every kernel of one Heff application (k_bspmv, k_tiles_fit, k_fchain, k_chain and their reductions)",
+        "kernel": "every kernel of one Heff application (k_bspmv, k_tiles_stream1, k_tiles_wave1, k_tiles_wave2, "
+                  "k_tiles_fit, k_fchain, k_chain and their reductions)",
         "workload": "Heff x at sites 15-16 of the warmed-up L=32 D=2000 random D2h state",
         "bytes_per_launch": rd + wr, "read_bytes": rd, "write_bytes": wr,
         "fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4),
