@@ -720,9 +720,10 @@ void Ctx::prepare(int st) {
   }
   const double t_route = now_s();
   // ---- chained GEMM tiles for the remaining terms (in (out, in) order)
-  std::vector<DevGemmOp> o1, o2;
-  std::vector<DevTile1> t1;
-  std::vector<DevTile> t2;
+  std::vector<DevGemmOp> o1;  // phase-1 products (host only: copied into the tile records)
+  DevTile1* t1 = nullptr;      // the device records, built in page-locked staging
+  DevGemmOp* o2 = nullptr;
+  DevTile* t2 = nullptr;
   int64_t Wsize = 0;
   struct WRec { int64_t off; int rows, cols; };
   std::vector<WRec> wrec(chain.size(), {-1, 0, 0});
@@ -918,9 +919,11 @@ void Ctx::prepare(int st) {
   ntreds = (int)tred.size();
   W.alloc(std::max<int64_t>(Wmax, 1));
   o1.resize(no1);
-  t1.resize(nt1);
-  o2.resize(nq);
-  t2.resize(nt2);
+  tstage.reset();
+  tstage.reserve(4 * 256 + nt1 * sizeof(DevTile1) + nq * sizeof(DevGemmOp) + nt2 * sizeof(DevTile));
+  t1 = tstage.take<DevTile1>(nt1);
+  o2 = tstage.take<DevGemmOp>(nq);
+  t2 = tstage.take<DevTile>(nt2);
   // phase 1: one group per two-operand term
   parallel_for(((int64_t)nq + 4095) / 4096, nthr, [&](int64_t c, int) {
     const size_t qe = std::min(nq, (size_t)(c + 1) * 4096);
@@ -1020,11 +1023,17 @@ void Ctx::prepare(int st) {
         });
     }
   });
-  ops2.upload(o2, stream);
-  tiles1.upload(t1, stream);
-  tiles2.upload(t2, stream);
-  ntiles1 = (int)t1.size();
-  ntiles2 = (int)t2.size();
+  {  // asynchronous uploads from page-locked memory (complete at the synchronisation ending prepare)
+    auto up = [&](auto& dbuf, const auto* h, int64_t n) {
+      dbuf.alloc(std::max<int64_t>(n, 1));
+      if (n > 0) hip_check(hipMemcpyAsync(dbuf.p, h, n * sizeof(*h), hipMemcpyHostToDevice, stream), "tile lists");
+    };
+    up(ops2, o2, (int64_t)nq);
+    up(tiles1, t1, nt1);
+    up(tiles2, t2, nt2);
+  }
+  ntiles1 = (int)nt1;
+  ntiles2 = (int)nt2;
   stats[6] = tot.pairs;
   stats[7] = (int64_t)chain.size() + stats[15];
 

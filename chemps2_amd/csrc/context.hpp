@@ -283,6 +283,7 @@ struct Ctx : HostState {
   // update executor: staging and scratch kept across calls (no page faults per call)
   HostStage ustage;
   HostStage cstage;  // chained-kernel records of the prepared site
+  HostStage tstage;  // GEMM-tile records of the prepared site
   DBuf<DevCombo> cmb_dc;  // combination records (site preparation and operator updates)
   DBuf<DevSrc> cmb_ds;
   DBuf<int64_t> cmb_cs;
