@@ -724,7 +724,6 @@ void Ctx::prepare(int st) {
   DevTile1* t1 = nullptr;      // the device records, built in page-locked staging
   DevGemmOp* o2 = nullptr;
   DevTile* t2 = nullptr;
-  int64_t Wsize = 0;
   struct WRec { int64_t off; int rows, cols; };
   std::vector<WRec> wrec(chain.size(), {-1, 0, 0});
   auto rel_x = [&](int k) { return (const double*)(intptr_t)S.k2i[k]; };
@@ -758,16 +757,15 @@ void Ctx::prepare(int st) {
       q0 = q1;
     }
   }
-  // Index pass (serial, integers only): every two-operand term's phase-1 op and its W
-  // offset, and the phase-1 tiles per batch and per bucket (tile_bucket), whose ranges
-  // tb1 holds as [B b + c, B b + c + 1) with B = TILE_BUCKETS. The records are then filled
-  // on the host threads (ops) and in order (tiles, by bucket).
+  // Index pass: every two-operand term's phase-1 op and its W offset, and the phase-1
+  // tiles per batch and per bucket (tile_bucket), whose ranges tb1 holds as
+  // [B b + c, B b + c + 1) with B = TILE_BUCKETS. It runs on the host threads over chunks
+  // of terms that stay within one batch (counts, a prefix over the chunks, then the
+  // offsets); the records are filled the same way after the ops.
   const size_t nq = chain.size();
   std::vector<int> opi1(nq, -1);
   tb1.assign(1, 0);
   tb2.assign(1, 0);
-  int no1 = 0;
-  std::vector<std::array<int64_t, TILE_BUCKETS>> bcount(1, std::array<int64_t, TILE_BUCKETS>{});
   // tiles of 64 x 64 by bucket, or (wave_tiles) of 32 x 32, all on wavefronts
   const int ts1 = wave_tiles ? 32 : 64;
   auto each_tile = [&](const WRec& w, auto&& f) {
@@ -775,23 +773,53 @@ void Ctx::prepare(int st) {
       for (int n0 = 0; n0 < w.cols; n0 += ts1)
         f(m0, n0, wave_tiles ? 0 : tile_bucket(std::min(64, w.rows - m0), std::min(64, w.cols - n0)));
   };
-  {
-    size_t nextb = 1;
-    for (size_t q = 0; q < nq; q++) {
-      if (nextb < bstart.size() && q == bstart[nextb]) {
-        bcount.push_back(std::array<int64_t, TILE_BUCKETS>{});
-        Wsize = 0;
-        nextb++;
-      }
+  auto w_of = [&](const Term& t) -> WRec {
+    return t.xr_first ? WRec{0, dLof(t.in), dRof(t.out)} : WRec{0, dLof(t.out), dRof(t.in)};
+  };
+  struct QChunk {
+    size_t q0, q1;
+    int batch;
+    int64_t ops = 0, w = 0;
+    std::array<int64_t, TILE_BUCKETS> tiles{};
+  };
+  std::vector<QChunk> qch;
+  const int nbatch = std::max<int>(1, (int)bstart.size());
+  for (int bt = 0; bt < nbatch; bt++) {
+    const size_t b0 = bstart.empty() ? 0 : bstart[bt], b1 = (bt + 1 < (int)bstart.size()) ? bstart[bt + 1] : nq;
+    for (size_t q = b0; q < b1; q += 8192) qch.push_back(QChunk{q, std::min(b1, q + 8192), bt});
+  }
+  parallel_for((int64_t)qch.size(), nthr, [&](int64_t ci, int) {
+    QChunk& C = qch[ci];
+    for (size_t q = C.q0; q < C.q1; q++) {
       const Term& t = *chain[q];
       if (t.tl == 'I' || t.tr == 'I') continue;
-      const int dL = dLof(t.out), dR = dRof(t.out), dLi = dLof(t.in), dRi = dRof(t.in);
-      const WRec w = t.xr_first ? WRec{Wsize, dLi, dR} : WRec{Wsize, dL, dRi};
-      wrec[q] = w;
-      Wsize += ((int64_t)w.rows * w.cols + 7) / 8 * 8;
-      opi1[q] = no1++;
-      each_tile(w, [&](int, int, int c) { bcount.back()[c]++; });
+      const WRec w = w_of(t);
+      C.ops++;
+      C.w += ((int64_t)w.rows * w.cols + 7) / 8 * 8;
+      each_tile(w, [&](int, int, int c) { C.tiles[c]++; });
     }
+  });
+  // prefix over the chunks: op index and W offset (per batch) at every chunk's start,
+  // tiles per batch and bucket
+  std::vector<int64_t> cops(qch.size()), cw(qch.size());
+  std::vector<std::array<int64_t, TILE_BUCKETS>> bcount(nbatch, std::array<int64_t, TILE_BUCKETS>{});
+  std::vector<std::array<int64_t, TILE_BUCKETS>> ctile(qch.size());
+  int no1 = 0;
+  {
+    int64_t ops = 0, wcur = 0;
+    int bt = -1;
+    for (size_t ci = 0; ci < qch.size(); ci++) {
+      if (qch[ci].batch != bt) { bt = qch[ci].batch; wcur = 0; }
+      cops[ci] = ops;
+      cw[ci] = wcur;
+      ops += qch[ci].ops;
+      wcur += qch[ci].w;
+      for (int c = 0; c < TILE_BUCKETS; c++) {
+        ctile[ci][c] = bcount[bt][c];  // position within (batch, bucket), shifted below
+        bcount[bt][c] += qch[ci].tiles[c];
+      }
+    }
+    no1 = (int)ops;
   }
   int64_t nt1 = 0;
   for (auto& bc : bcount)
@@ -799,6 +827,21 @@ void Ctx::prepare(int st) {
       nt1 += bc[c];
       tb1.push_back((int)std::min<int64_t>(nt1, INT32_MAX));
     }
+  for (size_t ci = 0; ci < qch.size(); ci++)
+    for (int c = 0; c < TILE_BUCKETS; c++) ctile[ci][c] += tb1[TILE_BUCKETS * qch[ci].batch + c];
+  parallel_for((int64_t)qch.size(), nthr, [&](int64_t ci, int) {
+    const QChunk& C = qch[ci];
+    int64_t op = cops[ci], wo = cw[ci];
+    for (size_t q = C.q0; q < C.q1; q++) {
+      const Term& t = *chain[q];
+      if (t.tl == 'I' || t.tr == 'I') continue;
+      WRec w = w_of(t);
+      w.off = wo;
+      wrec[q] = w;
+      wo += ((int64_t)w.rows * w.cols + 7) / 8 * 8;
+      opi1[q] = (int)op++;
+    }
+  });
   // Phase 2: an output block whose products are long (in 16-deep k chunks per tile) is
   // split into segments of consecutive products, so that the launch has many more
   // workgroups than the device holds at once; each segment writes its partial block into
@@ -945,17 +988,9 @@ void Ctx::prepare(int st) {
       o1[opi1[q]] = op;
     }
   });
-  {
-    size_t nextb = 1;
-    int b = 0;
-    int64_t pos[TILE_BUCKETS];
-    for (int c = 0; c < TILE_BUCKETS; c++) pos[c] = tb1[c];
-    for (size_t q = 0; q < nq; q++) {
-      if (nextb < bstart.size() && q == bstart[nextb]) {
-        b++;
-        nextb++;
-        for (int c = 0; c < TILE_BUCKETS; c++) pos[c] = tb1[TILE_BUCKETS * b + c];
-      }
+  parallel_for((int64_t)qch.size(), nthr, [&](int64_t ci, int) {
+    std::array<int64_t, TILE_BUCKETS> pos = ctile[ci];
+    for (size_t q = qch[ci].q0; q < qch[ci].q1; q++) {
       if (opi1[q] < 0) continue;
       const WRec& w = wrec[q];
       const DevGemmOp& op = o1[opi1[q]];
@@ -968,7 +1003,7 @@ void Ctx::prepare(int st) {
         t1[pos[c]++] = tl;
       });
     }
-  }
+  });
   // phase 2: one group per output block (or per segment of it), added to the explicit
   // product
   parallel_for(nblk, nthr, [&](int64_t bi, int) {
