@@ -300,6 +300,45 @@ def cpu_baseline(L, D, irr, T, V, ckpt, rtol, budget_s, fp64_sites):
     }
 
 
+N2_PUBLISHED = {"D": 2000, "seconds_per_sweep": 219.0, "energy": -109.28216077, "discarded_weight": 1.8910e-07,
+                "hardware": "16 Intel Xeon E5-2670 cores @ 2.6 GHz, OpenMP", "source": "sphinx/resources.rst:45-56"}
+
+
+def n2_ccpvdz(rtol):
+    """The reference's published sweep point: N2/cc-pVDZ (tests/golden/N2.CCPVDZ.FCIDUMP,
+    14 electrons in 28 orbitals, D2h, orbitals reordered as Problem::SetupReorderD2h does,
+    Davidson tolerance 1e-5). Warm-up D = 250, 500, 1000, 1500 (one sweep each), then one
+    timed left+right sweep at D = 2000, beside the published 219 s per sweep."""
+    import chemps2_amd
+    from chemps2_amd.problems import molpro_to_xor, mx_table, permute, read_fcidump, reorder_d2h
+
+    d = read_fcidump(os.path.join(ROOT, "tests", "golden", "N2.CCPVDZ.FCIDUMP"))
+    irr = [molpro_to_xor(7, m) for m in d["molpro_irreps"]]
+    irr2, T2, V2 = permute(irr, d["T"], d["V"], reorder_d2h(irr))
+    L, N = d["L"], d["N"]
+    mx = mx_table(T2, V2, N)
+    c = chemps2_amd.Context(0)
+    c.set_problem(L, 7, [int(v) for v in irr2], N, 0, 0, mx)
+    t0 = time.time()
+    c.dmrg_init(250, 1)
+    for Dw in (250, 500, 1000, 1500):
+        c.dmrg_solve([(Dw, 1e-12, 1, 0.0, rtol)])
+    warm = time.time() - t0
+    t0 = time.time()
+    e = c.dmrg_solve([(2000, 1e-12, 1, 0.0, rtol)]) + d["econst"]
+    secs = time.time() - t0
+    phases, nmv = c.dmrg_stats()
+    hf, hms = c.dmrg_heff_stats()
+    c.close()
+    log(f"N2/cc-pVDZ: D=2000 sweep {secs:.1f}s (warm-up {warm:.1f}s), E={e:.8f}; published "
+        f"{N2_PUBLISHED['seconds_per_sweep']} s, E={N2_PUBLISHED['energy']}")
+    return {"seconds_per_sweep": round(secs, 3), "energy": e, "warm_up_seconds": round(warm, 2),
+            "heff_applications": nmv, "heff_tflops_sweep_average": round(hf / max(hms, 1e-9) / 1e9, 4),
+            "phase_seconds": {k: round(v, 3) for k, v in phases.items()},
+            "published": N2_PUBLISHED,
+            "speedup_vs_published": round(N2_PUBLISHED["seconds_per_sweep"] / secs, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,6 +352,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-n2", action="store_true", help="skip the N2/cc-pVDZ D=2000 sweep beside the published one")
     ap.add_argument("--save-warm", metavar="H5", help="only the warm-up: save its state to H5 and exit")
     ap.add_argument("--site-from", metavar="H5", help="only the site benchmark, from a state --save-warm wrote, "
                     "its timed applications between two 1 GiB copy marks (for rocprofv3 --pmc)")
@@ -376,6 +416,10 @@ def main():
         s["speedup_vs_fp64"] = round(ref["seconds"] / s["seconds"], 3)
         log(f"{name}: |dE| vs fp64 {s['delta_E_vs_fp64']:.2e}, sweep {s['seconds']:.1f}s vs {ref['seconds']:.1f}s")
 
+    n2 = None
+    if not args.no_n2:
+        n2 = n2_ccpvdz(args.rtol)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -412,6 +456,7 @@ def main():
         "warm_up": warm,
         "site_bench": sb,
         "sweep": sweeps,
+        "n2_ccpvdz_published_point": n2,
     }
     print(json.dumps(out), flush=True)
 

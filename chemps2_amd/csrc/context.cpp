@@ -22,6 +22,12 @@ Ctx::~Ctx() {
   drop_plan();
   reap_join();
   destroy_rocblas(rocblas);
+  for (auto h : svd_handles) destroy_rocblas(h);
+  for (auto e : svd_ev_out)
+    if (e) (void)hipEventDestroy(e);
+  if (svd_ev_in) (void)hipEventDestroy(svd_ev_in);
+  for (auto st : svd_streams)
+    if (st) (void)hipStreamDestroy(st);
   if (s_buf >= 0) bufs[s_buf] = nullptr;  // owned by Sdev
   for (double* p : bufs)
     if (p) (void)hipFree(p);

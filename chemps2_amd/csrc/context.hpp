@@ -240,7 +240,7 @@ struct Ctx : HostState {
   // below (its operand loads from global memory are not coalesced).
   bool use_chain_mfma = false;
   DBuf<DevCBlock> cblocks_m;         // k_chain_mfma items sorted by tile count
-  bool svd_large_gesvdj = false;     // CHM_SVD_GESVDJ=1: large Split sectors by rocsolver_dgesvdj instead of dgesvd
+  bool svd_large_gesvdj = true;      // CHM_SVD_GESVDJ=0: large Split sectors by rocsolver_dgesvd instead of dgesvdj
   int chain_item_terms = CHAIN_ITEM_TERMS;    // CHM_CHAIN_ITEM_TERMS
   int chain_group_terms = CHAIN_GROUP_TERMS;  // CHM_CHAIN_GROUP_TERMS
   int tiles_kernel = 2;  // CHM_TILES_MFMA: grouped GEMM tiles (Heff phase 2, updates) in 2 k_tiles_fit, 1 k_tiles_mfma, 0 k_tiles
@@ -322,6 +322,12 @@ struct Ctx : HostState {
   int32_t s_buf = -1;
   int s_site = -1;
   void* rocblas = nullptr;  // rocblas_handle for the SVDs of Split
+  static constexpr int SVD_STREAMS = 4;  // side streams of the large Split SVDs
+  std::vector<hipStream_t> svd_streams;
+  std::vector<void*> svd_handles;
+  std::vector<hipEvent_t> svd_ev_out;
+  hipEvent_t svd_ev_in = nullptr;
+  void side_streams(int n);
   int svd_jacobi_maxn = SVD_JACOBI_MAXN;  // CHM_SVD_JACOBI_MAXN: larger centre sectors use rocSOLVER
   double total_min = 1e8, max_disc_last = 0.0, last_min = 1e8;
   double tm[6] = {0, 0, 0, 0, 0, 0};  // seconds: join, prepare, davidson, split, update, total

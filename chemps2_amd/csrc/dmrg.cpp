@@ -281,20 +281,39 @@ double Ctx::split(int st, int D, bool mr, bool change) {
       jres.alloc(std::max<size_t>(cs.size(), 1));
       jsw.alloc(std::max<size_t>(cs.size(), 1));
     }
+    // The large sectors are independent and one rocSOLVER SVD fills little of the device:
+    // they run on SVD_STREAMS side streams (default priority, one rocBLAS handle each),
+    // which wait for the assembled matrices on the context's stream and which that stream
+    // waits for in turn.
+    int nlarge = 0;
+    for (auto& c : cs) nlarge += (c.cd > 0 && !c.jac) ? 1 : 0;
+    const int nside = nlarge > 1 ? std::min(nlarge, SVD_STREAMS) : 0;
+    if (nside > 0) {
+      side_streams(nside);
+      hip_check(hipEventRecord(svd_ev_in, stream), "svd event");
+      for (int q = 0; q < nside; q++) hip_check(hipStreamWaitEvent(svd_streams[q], svd_ev_in, 0), "svd wait");
+    }
+    int li = 0;
     for (auto& c : cs) {
       if (c.cd > 0 && !c.jac) {
+        rocblas_handle h = nside > 0 ? (rocblas_handle)svd_handles[li % nside] : hnd;
+        li++;
         rocblas_status st2;
         if (svd_large_gesvdj)  // Jacobi eigenvectors of the Gram matrix, then QR (rocSOLVER's gesvdj)
-          st2 = rocsolver_dgesvdj(hnd, rocblas_svect_singular, rocblas_svect_singular, c.dLt, c.dRt, M.p + c.moff, c.dLt,
+          st2 = rocsolver_dgesvdj(h, rocblas_svect_singular, rocblas_svect_singular, c.dLt, c.dRt, M.p + c.moff, c.dLt,
                                   0.0, jres.p + ci, 100, jsw.p + ci, Sg.p + c.soff, U.p + c.uoff, c.dLt, VT.p + c.voff,
                                   c.cd, info.p + ci);
         else
-          st2 = rocsolver_dgesvd(hnd, rocblas_svect_singular, rocblas_svect_singular, c.dLt, c.dRt, M.p + c.moff, c.dLt,
+          st2 = rocsolver_dgesvd(h, rocblas_svect_singular, rocblas_svect_singular, c.dLt, c.dRt, M.p + c.moff, c.dLt,
                                  Sg.p + c.soff, U.p + c.uoff, c.dLt, VT.p + c.voff, c.cd, E.p + c.soff,
                                  rocblas_outofplace, info.p + ci);
         if (st2 != rocblas_status_success) throw Error{ERR_HIP, "rocSOLVER SVD failed"};
       }
       ci++;
+    }
+    for (int q = 0; q < nside; q++) {
+      hip_check(hipEventRecord(svd_ev_out[q], svd_streams[q]), "svd event");
+      hip_check(hipStreamWaitEvent(stream, svd_ev_out[q], 0), "svd wait");
     }
     hip_check(hipStreamSynchronize(stream), "split SVD");
   }
@@ -511,6 +530,23 @@ double Ctx::dmrg_solve(int n, const int* D, const double* econv, const int* max_
 
 void destroy_rocblas(void* h) {
   if (h) rocblas_destroy_handle((rocblas_handle)h);
+}
+
+// n side streams for the large Split SVDs, each with its rocBLAS handle, created once
+void Ctx::side_streams(int n) {
+  if (!svd_ev_in) hip_check(hipEventCreateWithFlags(&svd_ev_in, hipEventDisableTiming), "svd event");
+  while ((int)svd_streams.size() < n) {
+    hipStream_t st = nullptr;
+    hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "svd stream");
+    svd_streams.push_back(st);
+    rocblas_handle h;
+    if (rocblas_create_handle(&h) != rocblas_status_success) throw Error{ERR_HIP, "rocblas_create_handle"};
+    rocblas_set_stream(h, st);
+    svd_handles.push_back(h);
+    hipEvent_t e = nullptr;
+    hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "svd event");
+    svd_ev_out.push_back(e);
+  }
 }
 
 }  // namespace chm
