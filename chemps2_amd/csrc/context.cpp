@@ -881,7 +881,7 @@ void Ctx::prepare(int st) {
       const int k = chain[bq[bi]]->out;
       wtot += bwork[bi] * ((dLof(k) + 63) / 64) * ((dRof(k) + 63) / 64);
     }
-    const int64_t wseg = std::max<int64_t>(64, wtot / std::max(1, tile_segments));
+    const int64_t wseg = std::max<int64_t>(std::max(1, tile_seg_min), wtot / std::max(1, tile_segments));
     for (int64_t bi = 0; bi < nblk; bi++) bseg[bi] = (int)std::max<int64_t>(1, std::min<int64_t>(bwork[bi] / wseg, 64));
   }
   // Segment boundaries (about equal chunk counts) and whether a segment holds an axpy
@@ -2339,6 +2339,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_CHAIN_MFMA")) ctx->c.use_chain_mfma = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_TILES_MFMA")) ctx->c.tiles_kernel = std::max(0, std::min(2, std::atoi(e)));
   if (const char* e = std::getenv("CHM_TILE_SEGMENTS")) ctx->c.tile_segments = std::max(1, std::atoi(e));
+  if (const char* e = std::getenv("CHM_TILE_SEG_MIN")) ctx->c.tile_seg_min = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("CHM_WAVE_TILES")) ctx->c.wave_tiles = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_CHAIN_ITEM_TERMS")) ctx->c.chain_item_terms = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("CHM_CHAIN_GROUP_TERMS")) ctx->c.chain_group_terms = std::max(1, std::atoi(e));

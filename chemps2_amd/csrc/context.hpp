@@ -149,6 +149,7 @@ struct Ctx : HostState {
   int ntreds = 0;
   int64_t tpart_off = 0;      // first partial sum of those segments in the chained partials
   int tile_segments = 8192;   // CHM_TILE_SEGMENTS: phase-2 workgroups aimed at per launch
+  int tile_seg_min = 64;      // CHM_TILE_SEG_MIN: fewest 16-deep k chunks per segment
   bool wave_tiles = false;    // CHM_WAVE_TILES=1: every tile 32 x 32 on one wavefront (no LDS sharing)
   int64_t tile_w_cap = 0;  // doubles of W scratch per batch (CHM_TILE_W_CAP; 0: a quarter of free memory)
   DBuf<DevGemmOp> ops2;

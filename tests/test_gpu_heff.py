@@ -119,6 +119,46 @@ def test_explicit_threshold_paths(oracle_lib, explicit_max, chain_kernel, tiles_
     c.close()
 
 
+def test_gemm_tile_routes(oracle_lib, monkeypatch):
+    # every chained term through the GEMM tiles (explicit limit 0, no chained kernels):
+    # the default kernels (streamed phase-1 workgroups, wavefront tiles of at most
+    # 32 x 32, grouped phase-2 tiles), every tile on 32 x 32 wavefronts, phase-2 output
+    # blocks cut into segments of a few k chunks (partial sums and their reduction),
+    # and the older phase-2 kernels, each against the oracle
+    T, V = hubbard(10, U=4.0)
+    o = oracle_lib.OracleDMRG(10, 0, [0] * 10, 10, 0, 0, T, V)
+    o.init(1000, 3)
+    o.sweep_left_to(5, 1000, rtol=1e-3)
+    o.site(5)
+    monkeypatch.setenv("CHM_EXPLICIT_MAX", "0")
+    monkeypatch.setenv("CHM_CHAIN_KERNEL", "0")
+    monkeypatch.setenv("CHM_FCHAIN", "0")
+    tiles2 = {}
+    for name, env in [("default", {}), ("wave", {"CHM_WAVE_TILES": "1"}),
+                      ("segments", {"CHM_TILE_SEGMENTS": "100000000"}),
+                      ("wave+segments", {"CHM_WAVE_TILES": "1", "CHM_TILE_SEGMENTS": "100000000"}),
+                      ("segments, k_tiles_mfma", {"CHM_TILE_SEGMENTS": "100000000", "CHM_TILES_MFMA": "1"}),
+                      ("segments, k_tiles", {"CHM_TILE_SEGMENTS": "100000000", "CHM_TILES_MFMA": "0"})]:
+        for k in ("CHM_WAVE_TILES", "CHM_TILE_SEGMENTS", "CHM_TILE_SEG_MIN", "CHM_TILES_MFMA"):
+            monkeypatch.delenv(k, raising=False)
+        if "CHM_TILE_SEGMENTS" in env:
+            env = dict(env, CHM_TILE_SEG_MIN="2")
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        c = chemps2_amd.Context(0)
+        try:
+            print(f"tile route {name}", flush=True)
+            check_apply_diag(c, o, 5, seed=3)
+            st = c.plan_stats()
+            assert st["explicit_pairs"] == 0 and st["chain_kernel_terms"] == 0 and st["tiles1"] > 0
+            tiles2[name] = st["tiles2"]
+        finally:
+            c.close()
+    # segments multiply the phase-2 tiles; 32 x 32 tiles cut the larger blocks in four
+    assert tiles2["segments"] > tiles2["default"]
+    assert tiles2["wave"] >= tiles2["default"]
+
+
 @pytest.mark.parametrize("site", [0, 3, 6])
 @pytest.mark.parametrize("mfma", ["1", "0"])
 def test_chain_kernel_d2h(oracle_lib, site, mfma, monkeypatch):
