@@ -22,6 +22,12 @@ __device__ inline const double* rel(const double* p, int is_rel, const double* b
   return is_rel ? base + (intptr_t)p : p;
 }
 
+// word k of a record held one 32-bit word per lane (uniform result)
+__device__ __forceinline__ int ts_rl(uint32_t v, int k) { return __builtin_amdgcn_readlane((int)v, k); }
+__device__ __forceinline__ uint64_t ts_rl64(uint32_t v, int k) {
+  return (uint64_t)(uint32_t)ts_rl(v, k) | ((uint64_t)(uint32_t)ts_rl(v, k + 1) << 32);
+}
+
 // the output of tile t: relative to y (flags bit 0), to the partial sums (TILE_C_PART) or
 // absolute
 __device__ inline double* tile_out(const DevTile& t, double* ybase, double* pbase) {
@@ -392,6 +398,144 @@ __device__ __forceinline__ void tiles_fit_body(const DevTile& t, const DevGemmOp
   }
 }
 
+// tiles_fit_body for tiles without axpy products (TILE_NO_AXPY): the record of the next
+// product comes in a product ahead as one vector load of a 32-bit word per lane (read with
+// v_readlane when due), so a product switch does not wait for a scalar load before the
+// next chunk's operands can be requested.
+struct FitOp {
+  const double* A;
+  const double* B;
+  double alpha;
+  int lda, ldb, k, ta, tb;
+};
+template <int C>
+__device__ __forceinline__ void tiles_fit_body_pf(const DevTile& t, const DevGemmOp* __restrict__ ops,
+                                                  const double* __restrict__ xbase, double* __restrict__ ybase,
+                                                  double* __restrict__ pbase, double (*As)[TK][TM + 1],
+                                                  double (*Bs)[TK][TN + 1], int ms, int nsub) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, lr = lane & 15, lg = lane >> 4;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int mrem = min(TM, t.m - t.m0), nrem = min(TN, t.n - t.n0);
+  int si[C], sj[C], offa[C], offb[C];
+  bool on[C];
+#pragma unroll
+  for (int q = 0; q < C; q++) {
+    const int sq = wv + 4 * q;
+    on[q] = sq < nsub;
+    const int se = on[q] ? sq : 0;
+    si[q] = se % ms;
+    sj[q] = se / ms;
+    offa[q] = 16 * si[q] + lr;
+    offb[q] = 16 * sj[q] + lr;
+  }
+  chm_t4 acc[C];
+#pragma unroll
+  for (int q = 0; q < C; q++) acc[q] = chm_t4{0.0, 0.0, 0.0, 0.0};
+  const uint32_t* __restrict__ ow = (const uint32_t*)(ops + t.op0);
+  auto op_words = [&](int i) -> uint32_t {
+    return (i < t.nop && lane < 14) ? ((__attribute__((address_space(1))) const uint32_t*)ow)[(size_t)i * 14 + lane] : 0u;
+  };
+  auto decode = [&](uint32_t v) {
+    FitOp f;
+    const int fl = ts_rl(v, 12);
+    const double* A = (const double*)ts_rl64(v, 0);
+    const double* B = (const double*)ts_rl64(v, 2);
+    f.A = (fl & 1) ? xbase + (intptr_t)A : A;
+    f.B = (fl & 2) ? xbase + (intptr_t)B : B;
+    f.alpha = __longlong_as_double((long long)ts_rl64(v, 4));
+    f.lda = ts_rl(v, 6); f.ldb = ts_rl(v, 7); f.k = ts_rl(v, 8); f.ta = ts_rl(v, 9); f.tb = ts_rl(v, 10);
+    return f;
+  };
+  auto a_at = [&](const FitOp& op, int q, int& r, int& kk) {
+    if (op.ta == 0) { r = tid & 63; kk = (tid >> 6) + 4 * q; } else { kk = tid & 15; r = (tid >> 4) + 16 * q; }
+  };
+  auto b_at = [&](const FitOp& op, int q, int& kb, int& c) {
+    if (op.tb == 0) { kb = tid & 15; c = (tid >> 4) + 16 * q; } else { c = tid & 63; kb = (tid >> 6) + 4 * q; }
+  };
+  double ra[4], rb[4];
+  auto fetch = [&](const FitOp& op, int kc) {  // raw global loads of chunk (op, kc)
+    gdbl* A = (gdbl*)op.A;
+    gdbl* B = (gdbl*)op.B;
+    const int krem = min(TK, op.k - kc);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int r, kk, kb, c;
+      a_at(op, q, r, kk);
+      b_at(op, q, kb, c);
+      const size_t ia = op.ta ? (size_t)(kc + kk) + (size_t)op.lda * (t.m0 + r) : (size_t)(t.m0 + r) + (size_t)op.lda * (kc + kk);
+      const size_t ib = op.tb ? (size_t)(t.n0 + c) + (size_t)op.ldb * (kc + kb) : (size_t)(kc + kb) + (size_t)op.ldb * (t.n0 + c);
+      ra[q] = A[(r < mrem && kk < krem) ? ia : 0];
+      rb[q] = B[(c < nrem && kb < krem) ? ib : 0];
+    }
+  };
+  auto put = [&](int buf, const FitOp& op, int kc) {
+    const int krem = min(TK, op.k - kc);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int r, kk, kb, c;
+      a_at(op, q, r, kk);
+      b_at(op, q, kb, c);
+      As[buf][kk][r] = (r < mrem && kk < krem) ? op.alpha * ra[q] : 0.0;
+      Bs[buf][kb][c] = (c < nrem && kb < krem) ? rb[q] : 0.0;
+    }
+  };
+  // the chunk sequence: product o (its fields in cur), chunk k0; vN: words of product o + 1
+  int o = 0, k0 = 0;
+  FitOp cur = decode(op_words(0));
+  uint32_t vN = op_words(1);
+  auto next_op = [&]() {  // to the next product with k > 0 (or o = nop)
+    for (;;) {
+      o++;
+      if (o >= t.nop) return;
+      cur = decode(vN);
+      vN = op_words(o + 1);
+      if (cur.k > 0) return;
+    }
+  };
+  if (cur.k <= 0) next_op();
+  int buf = 0;
+  FitOp pc = cur;  // the product of the chunk in ra / rb
+  int pk = 0;
+  if (o < t.nop) fetch(cur, k0);
+  while (o < t.nop) {
+    put(buf, pc, pk);
+    __syncthreads();
+    k0 += TK;
+    if (k0 >= cur.k) {
+      k0 = 0;
+      next_op();
+    }
+    if (o < t.nop) {
+      fetch(cur, k0);  // in flight while this chunk computes
+      pc = cur;
+      pk = k0;
+    }
+#pragma unroll
+    for (int ks = 0; ks < TK / 4; ks++) {  // k past the product's end is zero in LDS
+      const int kk = 4 * ks + lg;
+#pragma unroll
+      for (int q = 0; q < C; q++)
+        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[buf][kk][offa[q]], Bs[buf][kk][offb[q]], acc[q], 0, 0, 0);
+    }
+    buf ^= 1;
+  }
+  double* Cp = tile_out(t, ybase, pbase);
+  const bool add = (t.flags & 2) != 0;
+#pragma unroll
+  for (int q = 0; q < C; q++) {
+    if (!on[q]) continue;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int rr = 16 * si[q] + 4 * r + lg, cc = 16 * sj[q] + lr;
+      if (rr < mrem && cc < nrem) {
+        double* p = Cp + (size_t)(t.m0 + rr) + (size_t)t.ldc * (t.n0 + cc);
+        *p = add ? *p + acc[q][r] : acc[q][r];
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_tiles_fit(const DevTile* __restrict__ tiles, const DevGemmOp* __restrict__ ops,
                                                   const double* __restrict__ xbase, double* __restrict__ ybase,
                                                   double* __restrict__ pbase) {
@@ -400,6 +544,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) void
   const DevTile t = tiles[blockIdx.x];
   const int mrem = min(TM, t.m - t.m0), nrem = min(TN, t.n - t.n0);
   const int ms = (mrem + 15) >> 4, ns = (nrem + 15) >> 4, nsub = ms * ns;
+  if (t.flags & TILE_NO_AXPY) {
+    switch ((nsub + 3) >> 2) {
+      case 1: tiles_fit_body_pf<1>(t, ops, xbase, ybase, pbase, As, Bs, ms, nsub); break;
+      case 2: tiles_fit_body_pf<2>(t, ops, xbase, ybase, pbase, As, Bs, ms, nsub); break;
+      case 3: tiles_fit_body_pf<3>(t, ops, xbase, ybase, pbase, As, Bs, ms, nsub); break;
+      default: tiles_fit_body_pf<4>(t, ops, xbase, ybase, pbase, As, Bs, ms, nsub); break;
+    }
+    return;
+  }
   switch ((nsub + 3) >> 2) {
     case 1: tiles_fit_body<1>(t, ops, xbase, ybase, pbase, As, Bs, ms, nsub); break;
     case 2: tiles_fit_body<2>(t, ops, xbase, ybase, pbase, As, Bs, ms, nsub); break;
@@ -427,10 +580,6 @@ struct Ts1 {
   double alpha;
   int ldc, m0, n0, mrem, nrem, lda, ldb, k, ta, tb, flags;
 };
-__device__ __forceinline__ int ts_rl(uint32_t v, int k) { return __builtin_amdgcn_readlane((int)v, k); }
-__device__ __forceinline__ uint64_t ts_rl64(uint32_t v, int k) {
-  return (uint64_t)(uint32_t)ts_rl(v, k) | ((uint64_t)(uint32_t)ts_rl(v, k + 1) << 32);
-}
 __device__ __forceinline__ Ts1 ts_rec(uint32_t v) {
   Ts1 t;
   t.C = (double*)ts_rl64(v, 0);
