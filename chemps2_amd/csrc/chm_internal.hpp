@@ -379,7 +379,7 @@ struct DevSvd {
   int* info;  // set to 1 when the sweeps end before every pair is orthogonal
   int m, n;
   int wide;
-  int pad;
+  int max_sweeps;  // at most SVD_MAX_SWEEPS
 };
 constexpr int SVD_JACOBI_MAXN = 256;  // largest n of the batched Jacobi kernel
 

@@ -494,6 +494,10 @@ void Ctx::prepare(int st) {
   // ---- chained terms: output blocks whose own and input dimensions are at most
   // CHAIN_MAXD run in the output-stationary kernel (two launches: block products up to
   // CHAIN_SMALLP, and larger ones when the device grants the LDS); the rest as GEMM tiles
+  // the output-stationary chained kernels (k_chain, k_chain_mfma, k_fchain): in the
+  // mixed form by default (its FP32 products are k_fchain's), else only when asked for;
+  // the GEMM tiles are faster on the FP64 form
+  const bool chain_on = use_chain_kernel < 0 ? heff_form == 1 : use_chain_kernel != 0;
   {
     // output blocks in chunks on the host threads, each with local records; the chunks
     // are then concatenated in order (group, term and partial offsets shifted), so the
@@ -534,7 +538,7 @@ void Ctx::prepare(int st) {
       for (size_t q0 = qb; q0 < qe;) {
         const int k = chain[q0]->out;
         size_t q1 = q0;
-        bool small = use_chain_kernel && dLof(k) <= lim && dRof(k) <= lim;
+        bool small = chain_on && dLof(k) <= lim && dRof(k) <= lim;
         int dmax = std::max(dLof(k), dRof(k));
         while (q1 < qe && chain[q1]->out == k) {
           const int in = chain[q1]->in;
@@ -2335,7 +2339,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_ITEM_ELEMS")) ctx->c.item_elems = std::max(256LL, std::atoll(e));
   if (const char* e = std::getenv("CHM_VERBOSE")) ctx->c.verbose = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_SVD_JACOBI_MAXN")) ctx->c.svd_jacobi_maxn = std::atoi(e);
-  if (const char* e = std::getenv("CHM_CHAIN_KERNEL")) ctx->c.use_chain_kernel = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CHM_CHAIN_KERNEL")) ctx->c.use_chain_kernel = std::atoi(e) != 0 ? 1 : 0;
   if (const char* e = std::getenv("CHM_CHAIN_MFMA")) ctx->c.use_chain_mfma = std::atoi(e) != 0;
   if (const char* e = std::getenv("CHM_TILES_MFMA")) ctx->c.tiles_kernel = std::max(0, std::min(2, std::atoi(e)));
   if (const char* e = std::getenv("CHM_TILE_SEGMENTS")) ctx->c.tile_segments = std::max(1, std::atoi(e));
@@ -2344,6 +2348,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_CHAIN_ITEM_TERMS")) ctx->c.chain_item_terms = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("CHM_CHAIN_GROUP_TERMS")) ctx->c.chain_group_terms = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("CHM_SVD_GESVDJ")) ctx->c.svd_large_gesvdj = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CHM_SVD_MAX_SWEEPS")) ctx->c.svd_max_sweeps = std::max(0, std::atoi(e));
   if (const char* e = std::getenv("CHM_TILE_W_CAP")) ctx->c.tile_w_cap = std::max(1LL, std::atoll(e));
   if (const char* e = std::getenv("CHM_OPS_KERNEL")) ctx->c.use_ops_kernel = std::atoi(e) != 0;
   ctx->c.env_pinned[0] = std::getenv("CHM_EXPLICIT_MAX") != nullptr;

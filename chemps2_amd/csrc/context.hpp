@@ -235,7 +235,7 @@ struct Ctx : HostState {
   int tune_row = -1;  // row used at the last prepare (-1: defaults)
   int64_t site_dmax = 0;
   void apply_tuning();
-  bool use_chain_kernel = true;      // CHM_CHAIN_KERNEL=0: every chained term through k_tiles
+  int use_chain_kernel = -1;  // CHM_CHAIN_KERNEL: 1 chained kernels for blocks up to 64, 0 GEMM tiles only, -1 by form
   // CHM_CHAIN_MFMA=1: chained blocks up to 48 on the matrix cores (k_chain_mfma). Off by
   // default: at L = 32, D = 2000 it matches k_chain for blocks of 33-48 and is slower
   // below (its operand loads from global memory are not coalesced).
@@ -324,6 +324,8 @@ struct Ctx : HostState {
   int s_site = -1;
   void* rocblas = nullptr;  // rocblas_handle for the SVDs of Split
   static constexpr int SVD_STREAMS = 4;  // side streams of the large Split SVDs
+  int64_t jacobi_retries = 0;            // Jacobi sectors decomposed again by rocSOLVER (not converged)
+  int svd_max_sweeps = 0;                // CHM_SVD_MAX_SWEEPS: sweep limit of the Jacobi kernel (0: its own)
   std::vector<hipStream_t> svd_streams;
   std::vector<void*> svd_handles;
   std::vector<hipEvent_t> svd_ev_out;

@@ -214,47 +214,50 @@ double Ctx::split(int st, int D, bool mr, bool change) {
   DBuf<rocblas_int> info;
   info.alloc(std::max<size_t>(cs.size(), 1));
   hip_check(hipMemsetAsync(info.p, 0, std::max<size_t>(cs.size(), 1) * sizeof(rocblas_int), stream), "clear info");
-  // assemble the centre matrices from the S blocks with their recoupling factors
-  std::vector<DevCopy> items;
-  for (auto& c : cs) {
-    if (c.cd <= 0) continue;
-    int lo = 0;
-    for (const Mult& a : c.lf) {
-      const int s1 = (a.n + 1 == c.nm) ? 1 : 0;
-      int ro = 0;
-      for (const Mult& b : c.rt) {
-        const int s2 = (b.n == c.nm + 1) ? 1 : 0;
-        const int fase = phase(a.s + b.s + s1 + s2);
-        const int tjmin = std::max(std::abs(b.s - a.s), std::abs(s2 - s1)), tjmax = std::min(s1 + s2, a.s + b.s);
-        DevCopy it{};
-        if (c.jac && c.dLt < c.dRt) {  // the Jacobi kernel works on M^T
-          it.dst = M.p + c.moff + ro + (int64_t)c.dRt * lo;
-          it.ldd = c.dRt; it.trans = 1;
-        } else {
-          it.dst = M.p + c.moff + lo + (int64_t)c.dLt * ro;
-          it.ldd = c.dLt;
+  // Assembly and SVD of the selected sectors (sel[q]): the centre matrices from the S
+  // blocks with their recoupling factors, then the batched Jacobi kernel for the small
+  // sectors and rocSOLVER for the others.
+  auto assemble_and_svd = [&](const std::vector<char>& sel) {
+    std::vector<DevCopy> items;
+    for (size_t q = 0; q < cs.size(); q++) {
+      const auto& c = cs[q];
+      if (c.cd <= 0 || !sel[q]) continue;
+      int lo = 0;
+      for (const Mult& a : c.lf) {
+        const int s1 = (a.n + 1 == c.nm) ? 1 : 0;
+        int ro = 0;
+        for (const Mult& b : c.rt) {
+          const int s2 = (b.n == c.nm + 1) ? 1 : 0;
+          const int fase = phase(a.s + b.s + s1 + s2);
+          const int tjmin = std::max(std::abs(b.s - a.s), std::abs(s2 - s1)), tjmax = std::min(s1 + s2, a.s + b.s);
+          DevCopy it{};
+          if (c.jac && c.dLt < c.dRt) {  // the Jacobi kernel works on M^T
+            it.dst = M.p + c.moff + ro + (int64_t)c.dRt * lo;
+            it.ldd = c.dRt; it.trans = 1;
+          } else {
+            it.dst = M.p + c.moff + lo + (int64_t)c.dLt * ro;
+            it.ldd = c.dLt;
+          }
+          it.rows = a.d; it.cols = b.d; it.lds = a.d;
+          int nsrc = 0;
+          for (int tj = tjmin; tj <= tjmax; tj += 2) {
+            const int k = Sj.kappa(a.n, a.s, a.i, c.nm - a.n, b.n - c.nm, tj, b.n, b.s, b.i);
+            if (k < 0) throw Error{ERR_STATE, "split: missing two-site block"};
+            const double pref = fase * std::sqrt(1.0 * (tj + 1) * (b.s + 1)) * wigner6j(a.s, b.s, tj, s2, s1, c.sm);
+            if (nsrc == 0) { it.src0 = Sdev.p + Sj.k2i[k]; it.c0 = pref; }
+            else { it.src1 = Sdev.p + Sj.k2i[k]; it.c1 = pref; }
+            nsrc++;
+          }
+          if (nsrc > 2) throw Error{ERR_STATE, "split: more than two spin couplings"};
+          it.keep_rows = nsrc ? a.d : 0;
+          it.keep_cols = nsrc ? b.d : 0;
+          if (!it.src0) it.src0 = Sdev.p;  // not read: keep_rows = 0
+          items.push_back(it);
+          ro += b.d;
         }
-        it.rows = a.d; it.cols = b.d; it.lds = a.d;
-        int nsrc = 0;
-        for (int tj = tjmin; tj <= tjmax; tj += 2) {
-          const int k = Sj.kappa(a.n, a.s, a.i, c.nm - a.n, b.n - c.nm, tj, b.n, b.s, b.i);
-          if (k < 0) throw Error{ERR_STATE, "split: missing two-site block"};
-          const double pref = fase * std::sqrt(1.0 * (tj + 1) * (b.s + 1)) * wigner6j(a.s, b.s, tj, s2, s1, c.sm);
-          if (nsrc == 0) { it.src0 = Sdev.p + Sj.k2i[k]; it.c0 = pref; }
-          else { it.src1 = Sdev.p + Sj.k2i[k]; it.c1 = pref; }
-          nsrc++;
-        }
-        if (nsrc > 2) throw Error{ERR_STATE, "split: more than two spin couplings"};
-        it.keep_rows = nsrc ? a.d : 0;
-        it.keep_cols = nsrc ? b.d : 0;
-        if (!it.src0) it.src0 = Sdev.p;  // not read: keep_rows = 0
-        items.push_back(it);
-        ro += b.d;
+        lo += a.d;
       }
-      lo += a.d;
     }
-  }
-  {
     DBuf<DevCopy> d_items;
     d_items.upload(items, stream);
     launch_copies(d_items.p, (int)items.size(), stream);
@@ -262,11 +265,12 @@ double Ctx::split(int st, int D, bool mr, bool change) {
     std::vector<DevSvd> jobs;
     for (size_t q = 0; q < cs.size(); q++) {
       const auto& c = cs[q];
-      if (c.cd > 0 && c.jac) {
+      if (c.cd > 0 && c.jac && sel[q]) {
         DevSvd j{};
         j.A = M.p + c.moff; j.Q = Qw.p + c.qoff; j.U = U.p + c.uoff; j.VT = VT.p + c.voff; j.S = Sg.p + c.soff;
         j.info = info.p + q;
         j.m = std::max(c.dLt, c.dRt); j.n = c.cd; j.wide = c.dLt < c.dRt;
+        j.max_sweeps = svd_max_sweeps;
         jobs.push_back(j);
       }
     }
@@ -274,7 +278,6 @@ double Ctx::split(int st, int D, bool mr, bool change) {
     d_jobs.upload(jobs, stream);
     launch_svd_jacobi(d_jobs.p, (int)jobs.size(), stream);
     hip_check(hipGetLastError(), "split SVD kernel");
-    int ci = 0;
     DBuf<double> jres;
     DBuf<rocblas_int> jsw;
     if (svd_large_gesvdj) {
@@ -286,7 +289,7 @@ double Ctx::split(int st, int D, bool mr, bool change) {
     // which wait for the assembled matrices on the context's stream and which that stream
     // waits for in turn.
     int nlarge = 0;
-    for (auto& c : cs) nlarge += (c.cd > 0 && !c.jac) ? 1 : 0;
+    for (size_t q = 0; q < cs.size(); q++) nlarge += (cs[q].cd > 0 && !cs[q].jac && sel[q]) ? 1 : 0;
     const int nside = nlarge > 1 ? std::min(nlarge, SVD_STREAMS) : 0;
     if (nside > 0) {
       side_streams(nside);
@@ -294,38 +297,65 @@ double Ctx::split(int st, int D, bool mr, bool change) {
       for (int q = 0; q < nside; q++) hip_check(hipStreamWaitEvent(svd_streams[q], svd_ev_in, 0), "svd wait");
     }
     int li = 0;
-    for (auto& c : cs) {
-      if (c.cd > 0 && !c.jac) {
-        rocblas_handle h = nside > 0 ? (rocblas_handle)svd_handles[li % nside] : hnd;
-        li++;
-        rocblas_status st2;
-        if (svd_large_gesvdj)  // Jacobi eigenvectors of the Gram matrix, then QR (rocSOLVER's gesvdj)
-          st2 = rocsolver_dgesvdj(h, rocblas_svect_singular, rocblas_svect_singular, c.dLt, c.dRt, M.p + c.moff, c.dLt,
-                                  0.0, jres.p + ci, 100, jsw.p + ci, Sg.p + c.soff, U.p + c.uoff, c.dLt, VT.p + c.voff,
-                                  c.cd, info.p + ci);
-        else
-          st2 = rocsolver_dgesvd(h, rocblas_svect_singular, rocblas_svect_singular, c.dLt, c.dRt, M.p + c.moff, c.dLt,
-                                 Sg.p + c.soff, U.p + c.uoff, c.dLt, VT.p + c.voff, c.cd, E.p + c.soff,
-                                 rocblas_outofplace, info.p + ci);
-        if (st2 != rocblas_status_success) throw Error{ERR_HIP, "rocSOLVER SVD failed"};
-      }
-      ci++;
+    for (size_t q = 0; q < cs.size(); q++) {
+      const auto& c = cs[q];
+      if (c.cd <= 0 || c.jac || !sel[q]) continue;
+      rocblas_handle h = nside > 0 ? (rocblas_handle)svd_handles[li % nside] : hnd;
+      li++;
+      rocblas_status st2;
+      if (svd_large_gesvdj)  // Jacobi eigenvectors of the Gram matrix, then QR (rocSOLVER's gesvdj)
+        st2 = rocsolver_dgesvdj(h, rocblas_svect_singular, rocblas_svect_singular, c.dLt, c.dRt, M.p + c.moff, c.dLt,
+                                0.0, jres.p + q, 100, jsw.p + q, Sg.p + c.soff, U.p + c.uoff, c.dLt, VT.p + c.voff,
+                                c.cd, info.p + q);
+      else
+        st2 = rocsolver_dgesvd(h, rocblas_svect_singular, rocblas_svect_singular, c.dLt, c.dRt, M.p + c.moff, c.dLt,
+                               Sg.p + c.soff, U.p + c.uoff, c.dLt, VT.p + c.voff, c.cd, E.p + c.soff,
+                               rocblas_outofplace, info.p + q);
+      if (st2 != rocblas_status_success) throw Error{ERR_HIP, "rocSOLVER SVD failed"};
     }
     for (int q = 0; q < nside; q++) {
       hip_check(hipEventRecord(svd_ev_out[q], svd_streams[q]), "svd event");
       hip_check(hipStreamWaitEvent(stream, svd_ev_out[q], 0), "svd wait");
     }
     hip_check(hipStreamSynchronize(stream), "split SVD");
+  };
+  assemble_and_svd(std::vector<char>(cs.size(), 1));
+  {
+    // a Jacobi sector whose sweeps ended before every pair was orthogonal (the kernel's
+    // info flag) is assembled again and decomposed by rocSOLVER's dgesvd, as the
+    // reference decomposes every sector (Sobject.cpp:444-518); a rocSOLVER failure is an
+    // error
+    std::vector<rocblas_int> inf(std::max<size_t>(cs.size(), 1));
+    copy_sync(stream, inf.data(), info.p, inf.size() * sizeof(rocblas_int), hipMemcpyDeviceToHost, "svd info");
+    std::vector<char> redo(cs.size(), 0);
+    bool any = false;
+    for (size_t q = 0; q < cs.size(); q++) {
+      if (inf[q] == 0) continue;
+      if (!cs[q].jac) throw Error{ERR_NUMERIC, "SVD did not converge"};
+      cs[q].jac = false;
+      redo[q] = 1;
+      any = true;
+      jacobi_retries++;
+    }
+    if (any) {
+      hip_check(hipMemsetAsync(info.p, 0, std::max<size_t>(cs.size(), 1) * sizeof(rocblas_int), stream), "clear info");
+      const bool gj = svd_large_gesvdj;
+      svd_large_gesvdj = false;
+      try {
+        assemble_and_svd(redo);
+      } catch (...) {
+        svd_large_gesvdj = gj;
+        throw;
+      }
+      svd_large_gesvdj = gj;
+      copy_sync(stream, inf.data(), info.p, inf.size() * sizeof(rocblas_int), hipMemcpyDeviceToHost, "svd info");
+      for (size_t q = 0; q < cs.size(); q++)
+        if (redo[q] && inf[q] != 0) throw Error{ERR_NUMERIC, "SVD did not converge"};
+    }
   }
   const double t1 = now();
   std::vector<double> sv(std::max<int64_t>(ssz, 1));
   copy_sync(stream, sv.data(), Sg.p, sv.size() * sizeof(double), hipMemcpyDeviceToHost, "singular values");
-  {
-    std::vector<rocblas_int> inf(std::max<size_t>(cs.size(), 1));
-    copy_sync(stream, inf.data(), info.p, inf.size() * sizeof(rocblas_int), hipMemcpyDeviceToHost, "svd info");
-    for (auto v : inf)
-      if (v != 0) throw Error{ERR_NUMERIC, "SVD did not converge"};
-  }
   // truncation (Sobject.cpp:444-518)
   double disc = 0.0;
   bool update = false;
@@ -362,7 +392,7 @@ double Ctx::split(int st, int D, bool mr, bool change) {
   mps_alloc(st + 1);
   if (mps[st].size > 0) hip_check(hipMemsetAsync(mps_buf[st], 0, mps[st].size * sizeof(double), stream), "clear");
   if (mps[st + 1].size > 0) hip_check(hipMemsetAsync(mps_buf[st + 1], 0, mps[st + 1].size * sizeof(double), stream), "clear");
-  items.clear();
+  std::vector<DevCopy> items;
   for (auto& c : cs) {
     const int dm = sec.dim(st + 1, c.nm, c.sm, c.im);
     if (dm <= 0) continue;
@@ -403,8 +433,9 @@ double Ctx::split(int st, int D, bool mr, bool change) {
     int nj = 0, nr = 0, maxcd = 0;
     for (auto& c : cs)
       if (c.cd > 0) { (c.jac ? nj : nr)++; maxcd = std::max(maxcd, c.cd); }
-    std::fprintf(stderr, "  split %d: %zu centre sectors (%d Jacobi, %d rocSOLVER, largest %d): SVD %.3f s, rest %.3f s\n", st,
-                 cs.size(), nj, nr, maxcd, t1 - t0, now() - t1);
+    std::fprintf(stderr, "  split %d: %zu centre sectors (%d Jacobi, %d rocSOLVER, largest %d): SVD %.3f s, rest %.3f s; "
+                 "Jacobi sectors redone so far %lld\n", st, cs.size(), nj, nr, maxcd, t1 - t0, now() - t1,
+                 (long long)jacobi_retries);
   }
   s_site = -1;
   site = -1;

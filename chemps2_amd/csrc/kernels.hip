@@ -2005,7 +2005,8 @@ __global__ __launch_bounds__(SVD_THREADS) void k_svd_jacobi(const DevSvd* __rest
   const int n2 = n + (n & 1);
   const double tol = sqrt((double)m) * SVD_EPS;
   bool converged = n <= 1;
-  for (int sweep = 0; sweep < SVD_MAX_SWEEPS && n > 1; sweep++) {
+  const int max_sweeps = it.max_sweeps > 0 ? min(it.max_sweeps, SVD_MAX_SWEEPS) : SVD_MAX_SWEEPS;
+  for (int sweep = 0; sweep < max_sweeps && n > 1; sweep++) {
     if (tid == 0) rotated = 0;
     __syncthreads();
     for (int r = 0; r < n2 - 1; r++) {

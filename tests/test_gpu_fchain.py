@@ -46,7 +46,7 @@ def test_fchain_oracle_state(site):
     x = np.random.default_rng(site).normal(size=n)
     y0, f0 = o.apply(x)
     print(f"case fused chain kernel, L=12 D=150 site {site}, V={n}", flush=True)
-    for env in ({"CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80", "CHM_EXPLICIT_MAX": "0"}, {"CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80"}):
+    for env in ({"CHM_CHAIN_KERNEL": "1", "CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80", "CHM_EXPLICIT_MAX": "0"}, {"CHM_CHAIN_KERNEL": "1", "CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80"}):
         c = _ctx(env)
         try:
             assert load_site(c, o, site) == n
@@ -67,7 +67,7 @@ def test_fchain_cylinder_D1000():
     x = np.random.default_rng(3).normal(size=n)
     y0, _ = o.apply(x)
     print(f"case fused chain kernel, cylinder D=1000 site 15, V={n}", flush=True)
-    c = _ctx({"CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80"})
+    c = _ctx({"CHM_CHAIN_KERNEL": "1", "CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80"})
     try:
         c.set_problem(32, 0, irr, 28, 0, 0, mx_table(T, V, 28))
         c.set_sectors_auto(1000)
@@ -94,7 +94,7 @@ def test_fchain_fp32_mixed_form(site):
     x = np.random.default_rng(site).normal(size=n)
     y0, _ = o.apply(x)
     print(f"case FP32 fused chain kernel, mixed form, L=12 D=150 site {site}, V={n}", flush=True)
-    for env in ({"CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80", "CHM_EXPLICIT_MAX": "0"}, {"CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80"}):
+    for env in ({"CHM_CHAIN_KERNEL": "1", "CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80", "CHM_EXPLICIT_MAX": "0"}, {"CHM_CHAIN_KERNEL": "1", "CHM_FCHAIN_MIN": "0", "CHM_FCHAIN_MAXD": "80"}):
         c = _ctx(env)
         try:
             c.set_heff_form(chemps2_amd.FORM_MIXED)

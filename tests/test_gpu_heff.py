@@ -166,6 +166,7 @@ def test_chain_kernel_d2h(oracle_lib, site, mfma, monkeypatch):
     # five term modes, transposed and plain operator blocks), on the matrix cores
     # (k_chain_mfma) or in LDS tiles (k_chain)
     monkeypatch.setenv("CHM_EXPLICIT_MAX", "0")
+    monkeypatch.setenv("CHM_CHAIN_KERNEL", "1")
     monkeypatch.setenv("CHM_CHAIN_MFMA", mfma)
     c = chemps2_amd.Context(0)
     irr, T, V = random_abinitio(8, 11, group=7)
