@@ -498,7 +498,15 @@ void Ctx::prepare(int st) {
   // mixed form by default (its FP32 products are k_fchain's), else only when asked for;
   // the GEMM tiles are faster on the FP64 form
   const bool chain_on = use_chain_kernel < 0 ? heff_form == 1 : use_chain_kernel != 0;
-  {
+  if (!chain_on) {  // every chained term stays for the GEMM tiles; the chained lists are empty
+    ncblocks = 0;
+    ncblocks_l = 0;
+    for (int bk = 0; bk < FCHAIN_NB; bk++) ncblocks_f[bk] = 0;
+    for (int n = 0; n <= CHAIN_MFMA_NT; n++) cblocks_m_offs[n] = 0;
+    ncreds = 0;
+    cpart_need = 0;
+    stats[15] = 0;
+  } else {
     // output blocks in chunks on the host threads, each with local records; the chunks
     // are then concatenated in order (group, term and partial offsets shifted), so the
     // lists do not depend on the thread count
@@ -861,21 +869,29 @@ void Ctx::prepare(int st) {
     const int k = (t.tr == 'I' || (t.tl != 'I' && t.xr_first)) ? dLof(t.in) : dRof(t.in);
     return (k + 15) / 16;
   };
-  std::vector<size_t> bq;  // first term of every output block
+  // output blocks (the chained terms are in (out, in) order) and their work, on the host
+  // threads over chunks of terms
+  std::vector<size_t> bq;      // first term of every output block
   std::vector<int64_t> bwork;  // 16-deep k chunks per tile of every output block
+  std::vector<int> cq(nq);     // chunks of every term
   {
-    size_t nextb = 1;
-    for (size_t q0 = 0; q0 < nq;) {
-      if (nextb < bstart.size() && q0 == bstart[nextb]) nextb++;
-      const int k = chain[q0]->out;
-      size_t q1 = q0;
-      int64_t w = 0;
-      while (q1 < nq && chain[q1]->out == k) w += chunks_of(*chain[q1++]);
-      bq.push_back(q0);
-      bwork.push_back(w);
-      q0 = q1;
-    }
+    const int64_t nc = ((int64_t)nq + 16383) / 16384;
+    std::vector<std::vector<size_t>> starts(nc);
+    parallel_for(nc, nthr, [&](int64_t c, int) {
+      const size_t q0 = (size_t)c * 16384, q1 = std::min(nq, q0 + 16384);
+      for (size_t q = q0; q < q1; q++) {
+        cq[q] = (int)chunks_of(*chain[q]);
+        if (q == 0 || chain[q]->out != chain[q - 1]->out) starts[c].push_back(q);
+      }
+    });
+    for (auto& v : starts) bq.insert(bq.end(), v.begin(), v.end());
     bq.push_back(nq);
+    bwork.resize(bq.size() - 1);
+    parallel_for((int64_t)bwork.size(), nthr, [&](int64_t bi, int) {
+      int64_t w = 0;
+      for (size_t q = bq[bi]; q < bq[bi + 1]; q++) w += cq[q];
+      bwork[bi] = w;
+    });
   }
   const int64_t nblk = (int64_t)bwork.size();
   std::vector<int> bseg(nblk, 1);
@@ -889,12 +905,14 @@ void Ctx::prepare(int st) {
     for (int64_t bi = 0; bi < nblk; bi++) bseg[bi] = (int)std::max<int64_t>(1, std::min<int64_t>(bwork[bi] / wseg, 64));
   }
   // Segment boundaries (about equal chunk counts) and whether a segment holds an axpy
-  // product; then the tiles per batch in two ranges, tb2[2 b] (tiles of at most 32 x 32
-  // without axpy: one wavefront each, k_tiles_wave2) and tb2[2 b + 1] (the rest, k_tiles_fit).
+  // product, per block on the host threads; then the tiles per batch in two ranges,
+  // tb2[2 b] (tiles of at most 32 x 32 without axpy: one wavefront each, k_tiles_wave2)
+  // and tb2[2 b + 1] (the rest, k_tiles_fit).
   std::vector<int64_t> segq0(nblk + 1, 0);  // first segment of every block in segq / segax
-  std::vector<size_t> segq;                 // first term of every segment
-  std::vector<char> segax;
-  for (int64_t bi = 0; bi < nblk; bi++) {
+  for (int64_t bi = 0; bi < nblk; bi++) segq0[bi + 1] = segq0[bi] + bseg[bi];
+  std::vector<size_t> segq(segq0[nblk] + 1);  // first term of every segment
+  std::vector<char> segax(segq0[nblk]);
+  parallel_for(nblk, nthr, [&](int64_t bi, int) {
     const size_t q0 = bq[bi], q1 = bq[bi + 1];
     const int ns = bseg[bi];
     size_t qa = q0;
@@ -904,16 +922,15 @@ void Ctx::prepare(int st) {
       const int64_t goal = bwork[bi] * (sg + 1) / ns;
       if (sg == ns - 1) qb = q1;
       else
-        while (qb < q1 && done < goal) done += chunks_of(*chain[qb++]);
+        while (qb < q1 && done < goal) done += cq[qb++];
       char ax = 0;
       for (size_t q = qa; q < qb; q++) ax |= (chain[q]->tl == 'I' && chain[q]->tr == 'I') ? 1 : 0;
-      segq.push_back(qa);
-      segax.push_back(ax);
+      segq[segq0[bi] + sg] = qa;
+      segax[segq0[bi] + sg] = ax;
       qa = qb;
     }
-    segq0[bi + 1] = (int64_t)segq.size();
-  }
-  segq.push_back(nq);
+  });
+  segq[segq0[nblk]] = nq;
   // the tiles of an output block (segment): 64 x 64, those of at most 32 x 32 without axpy
   // on wavefronts; or (wave_tiles) 32 x 32 on wavefronts unless the segment has an axpy
   auto each_tile2 = [&](int dL, int dR, bool ax, auto&& f) {
