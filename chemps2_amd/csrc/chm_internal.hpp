@@ -401,19 +401,21 @@ void launch_add_noise(double* x, int64_t n, uint64_t seed, double level, hipStre
 void launch_tiles(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const double* xbase, double* ybase,
                   hipStream_t s, int mode = 2, double* pbase = nullptr);
 // Tiles of one GEMM product each, overwriting their outputs, all of one bucket
-// (tile_bucket): c = 0, at most 32 x 32, one wavefront per tile (k_tiles_wave1); c = 1..4,
-// c sub-tiles per wavefront of a workgroup per tile (k_tiles_stream1). The workgroups
-// (TILE_STREAM_GROUPS, or TILE_WAVE_GROUPS of four wavefronts for c = 0: what the device
-// holds at once) walk the list.
+// (tile_bucket): c = 0, at most 32 x 32, and c = 5 / 6, at most 64 x 16 / 16 x 64, one
+// wavefront per tile (k_tiles_wave1); c = 1..4, c sub-tiles per wavefront of a workgroup
+// per tile (k_tiles_stream1). The workgroups (TILE_STREAM_GROUPS, or TILE_WAVE_GROUPS of
+// four wavefronts for the wavefront buckets: what the device holds at once) walk the list.
 // Grouped tiles of at most 32 x 32 whose products are all GEMMs with k > 0: one
 // wavefront per tile (k_tiles_wave2).
 void launch_tiles_wave2(const DevTile* tiles, const DevGemmOp* ops, int ntiles, const double* xbase, double* ybase,
                         hipStream_t s, double* pbase = nullptr);
 constexpr int TILE_STREAM_GROUPS = 1024;
 constexpr int TILE_WAVE_GROUPS = 768;
-constexpr int TILE_BUCKETS = 5;
+constexpr int TILE_BUCKETS = 7;
 inline int tile_bucket(int mrem, int nrem) {
   if (mrem <= 32 && nrem <= 32) return 0;
+  if (nrem <= 16) return 5;
+  if (mrem <= 16) return 6;
   return ((mrem + 15) / 16 * ((nrem + 15) / 16) + 3) / 4;
 }
 void launch_tiles_stream1(const DevTile1* tiles, int ntiles, int c, const double* xbase, hipStream_t s);

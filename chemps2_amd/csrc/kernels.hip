@@ -745,9 +745,12 @@ struct TwOp {  // one product on one tile, pointers resolved
   int lda, ldb, k, ta, tb;
   int m0, n0, mrem, nrem;
 };
+// The helpers below work on an MS x NS grid of 16 x 16 sub-tiles per wavefront (2 x 2:
+// tiles of at most 32 x 32; 4 x 1 and 1 x 4: the thin tiles of at most 64 x 16 and 16 x 64).
 // raw loads of chunk (f, k0); 32-bit byte offsets (an operator block or x holds fewer
 // than 2^28 elements), so that the loads take the uniform base from scalar registers
-__device__ __forceinline__ void tw_fetch(const TwOp& f, int k0, int lr, int lg, double (&pa)[4][2], double (&pb)[4][2]) {
+template <int MS, int NS>
+__device__ __forceinline__ void tw_fetch(const TwOp& f, int k0, int lr, int lg, double (&pa)[4][MS], double (&pb)[4][NS]) {
   gbyte* A = (gbyte*)f.A;
   gbyte* B = (gbyte*)f.B;
   const int krem = f.k - k0;
@@ -759,51 +762,55 @@ __device__ __forceinline__ void tw_fetch(const TwOp& f, int k0, int lr, int lg, 
     const uint32_t kg = (uint32_t)(k0 + kk);
     const bool kok = kk < krem;
 #pragma unroll
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < MS; i++) {
       const int r = 16 * i + lr;
       const uint32_t ia = kg * sa_k + (uint32_t)(f.m0 + r) * sa_r;
       pa[ks][i] = *(gdbl*)(A + ((kok && r < f.mrem) ? ia * 8u : 0u));
     }
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
+    for (int j = 0; j < NS; j++) {
       const int c = 16 * j + lr;
       const uint32_t ib = kg * sb_k + (uint32_t)(f.n0 + c) * sb_c;
       pb[ks][j] = *(gdbl*)(B + ((kok && c < f.nrem) ? ib * 8u : 0u));
     }
   }
 }
-__device__ __forceinline__ void tw_take(const TwOp& f, int k0, int lr, int lg, double (&pa)[4][2], double (&pb)[4][2]) {
+template <int MS, int NS>
+__device__ __forceinline__ void tw_take(const TwOp& f, int k0, int lr, int lg, double (&pa)[4][MS], double (&pb)[4][NS]) {
   const int krem = f.k - k0;
 #pragma unroll
   for (int ks = 0; ks < 4; ks++) {
     const bool kok = 4 * ks + lg < krem;
 #pragma unroll
-    for (int i = 0; i < 2; i++) pa[ks][i] = (kok && 16 * i + lr < f.mrem) ? f.alpha * pa[ks][i] : 0.0;
+    for (int i = 0; i < MS; i++) pa[ks][i] = (kok && 16 * i + lr < f.mrem) ? f.alpha * pa[ks][i] : 0.0;
 #pragma unroll
-    for (int j = 0; j < 2; j++) pb[ks][j] = (kok && 16 * j + lr < f.nrem) ? pb[ks][j] : 0.0;
+    for (int j = 0; j < NS; j++) pb[ks][j] = (kok && 16 * j + lr < f.nrem) ? pb[ks][j] : 0.0;
   }
 }
-__device__ __forceinline__ void tw_mma(chm_t4 (&acc)[2][2], const double (&pa)[4][2], const double (&pb)[4][2]) {
+template <int MS, int NS>
+__device__ __forceinline__ void tw_mma(chm_t4 (&acc)[MS][NS], const double (&pa)[4][MS], const double (&pb)[4][NS]) {
 #pragma unroll
   for (int ks = 0; ks < 4; ks++)
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < MS; i++)
 #pragma unroll
-      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks][i], pb[ks][j], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < NS; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[ks][i], pb[ks][j], acc[i][j], 0, 0, 0);
 }
-__device__ __forceinline__ void tw_zero(chm_t4 (&acc)[2][2]) {
+template <int MS, int NS>
+__device__ __forceinline__ void tw_zero(chm_t4 (&acc)[MS][NS]) {
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < MS; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = chm_t4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < NS; j++) acc[i][j] = chm_t4{0.0, 0.0, 0.0, 0.0};
 }
 // the tile's elements of this lane: out[(m0 + row) + ldc (n0 + col)] (+)= acc
-__device__ __forceinline__ void tw_store(const chm_t4 (&acc)[2][2], double* out, int ldc, int m0, int n0, int mrem,
+template <int MS, int NS>
+__device__ __forceinline__ void tw_store(const chm_t4 (&acc)[MS][NS], double* out, int ldc, int m0, int n0, int mrem,
                                          int nrem, bool add, int lr, int lg) {
 #pragma unroll
-  for (int i = 0; i < 2; i++)
+  for (int i = 0; i < MS; i++)
 #pragma unroll
-    for (int j = 0; j < 2; j++)
+    for (int j = 0; j < NS; j++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int rr = 16 * i + 4 * r + lg, cc = 16 * j + lr;
@@ -814,9 +821,10 @@ __device__ __forceinline__ void tw_store(const chm_t4 (&acc)[2][2], double* out,
       }
 }
 
-// Phase-1 tiles (DevTile1) of at most 32 x 32: wavefront w of the grid walks tiles w,
+// Phase-1 tiles (DevTile1) of MS x NS sub-tiles: wavefront w of the grid walks tiles w,
 // w + W, ... (W wavefronts in the grid) as one chunk sequence; the next tile's record
 // comes in a tile ahead.
+template <int MS, int NS>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_tiles_wave1(
     const DevTile1* __restrict__ tiles, int ntiles, const double* __restrict__ xbase) {
   const int lane = threadIdx.x & 63, lr = lane & 15, lg = lane >> 4;
@@ -841,7 +849,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3, 8))) void
   uint32_t vN = rec_words(ti + W);
   int k0 = 0;
   Ts1 cT = fT;  // the tile of the chunk being computed
-  chm_t4 acc[2][2];
+  chm_t4 acc[MS][NS];
   tw_zero(acc);
   // moves the fetch side to the next chunk; last: the chunk before ended its tile
   auto advance = [&](bool& last) -> bool {
@@ -865,7 +873,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3, 8))) void
     cT = fT;
     return true;
   };
-  double a0[4][2], b0[4][2], a1[4][2], b1[4][2];
+  double a0[4][MS], b0[4][NS], a1[4][MS], b1[4][NS];
   tw_fetch(f, k0, lr, lg, a0, b0);
   for (;;) {
     bool last;
@@ -2424,7 +2432,13 @@ void launch_tiles_stream1(const DevTile1* tiles, int ntiles, int c, const double
   if (ntiles <= 0) return;
   if (c == 0) {  // tiles of at most 32 x 32: one wavefront each
     const int g = std::min((ntiles + 3) / 4, TILE_WAVE_GROUPS);
-    hipLaunchKernelGGL(k_tiles_wave1, dim3(g), dim3(NT), 0, s, tiles, ntiles, xbase);
+    hipLaunchKernelGGL((k_tiles_wave1<2, 2>), dim3(g), dim3(NT), 0, s, tiles, ntiles, xbase);
+    return;
+  }
+  if (c == 5 || c == 6) {  // thin tiles of at most 64 x 16 or 16 x 64: one wavefront each
+    const int g = std::min((ntiles + 3) / 4, TILE_WAVE_GROUPS);
+    if (c == 5) hipLaunchKernelGGL((k_tiles_wave1<4, 1>), dim3(g), dim3(NT), 0, s, tiles, ntiles, xbase);
+    else hipLaunchKernelGGL((k_tiles_wave1<1, 4>), dim3(g), dim3(NT), 0, s, tiles, ntiles, xbase);
     return;
   }
   const int g = std::min(ntiles, TILE_STREAM_GROUPS);
