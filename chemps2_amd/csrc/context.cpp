@@ -1348,8 +1348,9 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
   int* seg_of = u_seg.data();
   int64_t* woff = u_woff.data();
   int64_t* order = u_order.data();
-  auto small_gemm = [](int m, int nn, int k, int ta, int lda, int tb, int ldb) {
-    return m <= OPS_MAXD && nn <= OPS_MAXD && (int64_t)m * k <= OPS_P && (int64_t)k * nn <= OPS_P &&
+  const int omax = ops_maxd;
+  auto small_gemm = [omax](int m, int nn, int k, int ta, int lda, int tb, int ldb) {
+    return m <= omax && nn <= omax && (int64_t)m * k <= OPS_P && (int64_t)k * nn <= OPS_P &&
            lda == (ta ? k : m) && ldb == (tb ? nn : k);
   };
   const bool ops_on = use_ops_kernel;
@@ -1430,7 +1431,7 @@ void Ctx::run_terms(const std::vector<USeg>& segs, const std::vector<std::pair<R
     for (int64_t q = c * 1024; q < std::min<int64_t>(nq, (c + 1) * 1024); q++) {
       const int m = outs[q].second.first, nn = outs[q].second.second;
       if (m <= 0 || nn <= 0) continue;
-      bool sm = ops_on && m <= OPS_MAXD && nn <= OPS_MAXD;
+      bool sm = ops_on && m <= omax && nn <= omax;
       for (int64_t pos = qstart[q]; sm && pos < qstart[q + 1]; pos++) {
         const int64_t g = order[pos];
         const UTerm& u = segs[seg_of[g]].terms[g - s0[seg_of[g]]];
@@ -2368,6 +2369,7 @@ int chm_ctx_create(int device, chm_ctx** out) {
   if (const char* e = std::getenv("CHM_SVD_MAX_SWEEPS")) ctx->c.svd_max_sweeps = std::max(0, std::atoi(e));
   if (const char* e = std::getenv("CHM_TILE_W_CAP")) ctx->c.tile_w_cap = std::max(1LL, std::atoll(e));
   if (const char* e = std::getenv("CHM_OPS_KERNEL")) ctx->c.use_ops_kernel = std::atoi(e) != 0;
+  if (const char* e = std::getenv("CHM_OPS_MAXD")) ctx->c.ops_maxd = std::min(chm::OPS_MAXD, std::max(1, std::atoi(e)));
   ctx->c.env_pinned[0] = std::getenv("CHM_EXPLICIT_MAX") != nullptr;
   ctx->c.env_pinned[1] = std::getenv("CHM_CHAIN_ITEM_TERMS") != nullptr;
   ctx->c.env_pinned[2] = std::getenv("CHM_CHAIN_GROUP_TERMS") != nullptr;

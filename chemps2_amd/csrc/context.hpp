@@ -297,7 +297,8 @@ struct Ctx : HostState {
   DBuf<DevOItem> u_it1, u_it2;
   DBuf<DevPRed> u_rd;
   DBuf<double> u_part;
-  bool use_ops_kernel = true;  // CHM_OPS_KERNEL=0: every update product through k_tiles
+  bool use_ops_kernel = false;  // CHM_OPS_KERNEL=1: small update products through k_ops, others on tiles
+  int ops_maxd = OPS_MAXD;     // CHM_OPS_MAXD: largest m, n routed to k_ops (at most OPS_MAXD)
   double prep_s[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // seconds of the last prepare, by phase
   bool verbose = false;                         // CHM_VERBOSE: phase times on stderr
   int64_t item_elems = 32768;        // CHM_ITEM_ELEMS: matrix elements per work item

@@ -45,12 +45,13 @@ def test_update_operators(ctx, oracle_lib, index, mr):
     assert n > 50
 
 
-@pytest.mark.parametrize("index,mr", [(3, True), (3, False)])
-@pytest.mark.parametrize("tiles_mfma", ["1", "0"])
-def test_update_operators_tile_path(oracle_lib, monkeypatch, index, mr, tiles_mfma):
-    # every update product through the GEMM tiles (CHM_OPS_KERNEL=0), on the matrix
-    # cores or with vector FMA, agrees as well
-    monkeypatch.setenv("CHM_OPS_KERNEL", "0")
+@pytest.mark.parametrize("index,mr", [(3, True), (3, False), (0, True), (6, False)])
+@pytest.mark.parametrize("ops_kernel,tiles_mfma", [("1", "2"), ("0", "1"), ("0", "0")])
+def test_update_operators_other_paths(oracle_lib, monkeypatch, index, mr, ops_kernel, tiles_mfma):
+    # small products through k_ops and the rest on the default tiles (CHM_OPS_KERNEL=1),
+    # and every product through the older GEMM tiles, on the matrix cores or with vector
+    # FMA, agree as well
+    monkeypatch.setenv("CHM_OPS_KERNEL", ops_kernel)
     monkeypatch.setenv("CHM_TILES_MFMA", tiles_mfma)
     c = chemps2_amd.Context(0)
     try:
