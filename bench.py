@@ -62,6 +62,14 @@ def read_traffic(name):
 
 
 def traffic_from(fetch_csv, write_csv, steps, out_name):
+    """traffic_numbers() of the two counter files, written to profiles/out_name."""
+    out = traffic_numbers(fetch_csv, write_csv, steps)
+    with open(os.path.join(ROOT, "profiles", out_name), "w") as f:
+        json.dump(out, f, indent=1)
+    return out
+
+
+def traffic_numbers(fetch_csv, write_csv, steps):
     """HBM bytes per Heff application from two rocprofv3 --pmc runs of --site-from (one
     with FETCH_SIZE, one with WRITE_SIZE, in KiB per dispatch): the counted products are
     the dispatches between the two 1 GiB copy marks, and each counter is scaled by what it
@@ -99,8 +107,6 @@ def traffic_from(fetch_csv, write_csv, steps, out_name):
         "fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4),
         "dispatches_per_application": len(fk) / steps, "applications": steps,
     }
-    with open(os.path.join(ROOT, "profiles", out_name), "w") as f:
-        json.dump(out, f, indent=1)
     return out
 
 
