@@ -61,6 +61,27 @@ def read_traffic(name):
         return None, None
 
 
+def roofline_record(F_H, B_H, ms, traffic, traffic_source):
+    """The roofline of one Heff application: F_H flops in ms milliseconds of kernels
+    against min(FP64 matrix-core peak, F_H / B_H x HBM bandwidth)."""
+    tflops = F_H / (ms * 1e-3) / 1e12
+    ai = F_H / B_H
+    peak = min(PEAK_FP64_TFLOPS, ai * PEAK_HBM_GBS / 1e3)
+    return {
+        "bound": "mfma" if ai * PEAK_HBM_GBS / 1e3 >= PEAK_FP64_TFLOPS else "hbm",
+        "achieved": round(tflops, 4),
+        "peak": round(peak, 2),
+        "unit": "TFLOP/s",
+        "frac": round(tflops / peak, 4),
+        "traffic": traffic,
+        "kernel": "Heff x at sites 15-16 (all its kernels), F_H / t",
+        "F_H": F_H,
+        "B_H": B_H,
+        "traffic_over_B_H": round(traffic / B_H, 2) if traffic else None,
+        "traffic_source": traffic_source,
+    }
+
+
 def traffic_from(fetch_csv, write_csv, steps, out_name):
     """traffic_numbers() of the two counter files, written to profiles/out_name."""
     out = traffic_numbers(fetch_csv, write_csv, steps)
@@ -204,27 +225,13 @@ def site_bench(L, D, irr, mx, ckpt, site, steps, warmup, marks=False):
     # device time of the product's kernels (events around its kernel groups on the
     # library stream); ms above also holds the host gaps between the calls
     ms_k = sum(kms) / max(cnt, 1)
-    tflops = F_H / (ms_k * 1e-3) / 1e12
-    ai = F_H / B_H
-    peak = min(PEAK_FP64_TFLOPS, ai * PEAK_HBM_GBS / 1e3)
     traffic, tsrc = read_traffic("heff_site15_L32_D2000_traffic.json")
+    roof = roofline_record(F_H, B_H, ms_k, traffic, tsrc)
+    tflops, ai = roof["achieved"], F_H / B_H
     log(f"site {site}: V={n} F_H={F_H:.4e} B_H={B_H:.4e} AI={ai:.1f} flop/B: {ms_k:.3f} ms of kernels "
         f"({ms:.3f} ms per call), {tflops:.2f} TFLOP/s "
         f"(kernels: tiles {kms[0] / max(cnt, 1):.3f}, explicit {kms[1] / max(cnt, 1):.3f}, "
         f"chained {kms[2] / max(cnt, 1):.3f} ms)")
-    roof = {
-        "bound": "mfma" if ai * PEAK_HBM_GBS / 1e3 >= PEAK_FP64_TFLOPS else "hbm",
-        "achieved": round(tflops, 4),
-        "peak": round(peak, 2),
-        "unit": "TFLOP/s",
-        "frac": round(tflops / peak, 4),
-        "traffic": traffic,
-        "kernel": "Heff x at sites 15-16 (all its kernels), F_H / t",
-        "F_H": F_H,
-        "B_H": B_H,
-        "traffic_over_B_H": round(traffic / B_H, 2) if traffic else None,
-        "traffic_source": tsrc,
-    }
     return {
         "site": site, "V": n, "F_H": F_H, "B_H": B_H, "ms_kernels": round(ms_k, 4), "ms_per_call": round(ms, 4),
         "tflops": round(tflops, 4),

@@ -56,3 +56,25 @@ def test_traffic_numbers(tmp_path, fetch_per_gib, write_per_gib):
     assert out["fetch_scale"] == pytest.approx(1.0 / fetch_per_gib, abs=1e-4)
     assert out["dispatches_per_application"] == 3
     assert out["applications"] == steps
+
+
+def test_roofline_record_matrix_core_bound():
+    # the middle-site product of the L=32 D=2000 benchmark as measured on one MI355X:
+    # 16.7922 ms of kernels for F_H = 2.2925e11 flops over B_H = 7.2768e8 algorithmic bytes
+    r = bench.roofline_record(229251582970.0, 727681104.0, 16.7922, 54519203899.42752, "profiles/x.json")
+    assert r["bound"] == "mfma"           # 315 flop/B x 8 TB/s is far above 78.6 TFLOP/s
+    assert r["peak"] == 78.6
+    assert r["achieved"] == 13.6523
+    assert r["frac"] == 0.1737
+    assert r["traffic_over_B_H"] == 74.92
+    assert r["unit"] == "TFLOP/s" and r["traffic_source"] == "profiles/x.json"
+
+
+def test_roofline_record_hbm_bound():
+    # 1 flop per byte: the bound is 8 TFLOP/s of HBM, and 1 TFLOP/s is 1/8 of it
+    r = bench.roofline_record(1e9, 1e9, 1.0, None, None)
+    assert r["bound"] == "hbm"
+    assert r["peak"] == 8.0
+    assert r["achieved"] == 1.0
+    assert r["frac"] == 0.125
+    assert r["traffic_over_B_H"] is None
